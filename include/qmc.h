@@ -1,0 +1,253 @@
+/*
+ * qmc.h -- C ABI of rust_qmc_amd, the MI355X (gfx950) walker-ensemble
+ * engine for variational Monte Carlo of Slater-Jastrow wavefunctions.
+ *
+ * The ABI mirrors the rust_qmc trait surface (paths relative to the rust_qmc
+ * crate root):
+ *   MultiWfn          src/wavefunction/traits.rs:48-59
+ *   OptimizableWfn    src/wavefunction/traits.rs:98-115
+ *   EnergyCalculator  src/sampling/traits.rs:6-8
+ *   ForceCalculator   src/sampling/traits.rs:20-56
+ *   MCMCSimulation    src/sampling/vmc.rs:38-189
+ * Each entry point below names the reference item it replaces.
+ *
+ * Conventions
+ *   - Every function returns QMC_OK (0) or a negative QMC_ERR_* code;
+ *     qmc_last_error() returns a human-readable message for the last
+ *     failure on the calling thread.
+ *   - Positions are double [W][N][3] in the reference's electron order
+ *     (Vec<Vector3<f64>> per walker).  Buffers may be host or device
+ *     memory; the library detects which and stages host memory itself.
+ *   - `stream` is a hipStream_t or NULL for the default stream; calls are
+ *     synchronous with respect to the host when they return results in
+ *     host memory.
+ *   - The reference returns linear psi, grad psi and lap psi.  This ABI
+ *     returns ln|psi| with its sign, grad ln|psi| and lap(psi)/psi, which
+ *     stay finite where the linear values under- or overflow (benzene:
+ *     ln|psi| ~ -650).  evaluate = sign*exp(logabs),
+ *     derivative = evaluate*grad, laplacian = evaluate*lap.
+ *   - A system handle may be shared read-only between threads; calls that
+ *     mutate it (set_params, set_nuclei, set_precision, set_flags) need
+ *     external synchronisation.
+ */
+#ifndef RUST_QMC_AMD_QMC_H
+#define RUST_QMC_AMD_QMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define QMC_OK 0
+#define QMC_ERR_INVALID (-1)     /* bad argument (null pointer, size mismatch) */
+#define QMC_ERR_UNSUPPORTED (-2) /* shape not compiled into the engine */
+#define QMC_ERR_HIP (-3)         /* HIP runtime failure */
+#define QMC_ERR_SINGULAR (-4)    /* a Slater matrix was exactly singular */
+#define QMC_ERR_NOMEM (-5)
+
+/* ---- precision modes ---- */
+#define QMC_FP64 0  /* everything in FP64 */
+#define QMC_MIXED 1 /* FP32 orbitals, Jastrow and Sherman-Morrison; FP64
+                       inverse refresh, log-det, energy accumulators */
+
+/* ---- behaviour flags (qmc_system_desc.flags, qmc_set_flags) ----
+ * The reference's own quirks, reproduced when the flag is set. */
+#define QMC_FLAG_J2_GRAD_QUIRK (1u << 0)     /* Jastrow2 gradient times F, jastrow.rs:102 */
+#define QMC_FLAG_J2_LAP_QUIRK (1u << 1)      /* Jastrow2 Laplacian w/o cross terms, jastrow.rs:110-125 */
+#define QMC_FLAG_HF_NN_REFSIGN (1u << 2)     /* HF force subtracts the nucleus-nucleus force, methane.rs:1259-1269 */
+#define QMC_FLAG_SINGULAR_IDENTITY (1u << 3) /* singular Slater matrix -> identity inverse, methane.rs:829 */
+#define QMC_FLAGS_REFERENCE \
+  (QMC_FLAG_J2_GRAD_QUIRK | QMC_FLAG_J2_LAP_QUIRK | QMC_FLAG_HF_NN_REFSIGN | QMC_FLAG_SINGULAR_IDENTITY)
+
+/* ---- system description ---- */
+#define QMC_AO_GTO 0      /* CGTO, methane.rs:488-591 (unnormalised primitives) */
+#define QMC_AO_STO 1      /* STO radial sum c r^p exp(-zeta r), slater.rs:48-145 */
+#define QMC_AO_SLATER1S 2 /* Slater1s exp(-alpha r), slater.rs:10-42 */
+
+typedef struct {
+  int32_t kind;   /* QMC_AO_* */
+  int32_t center; /* nucleus index */
+  int32_t l, m;   /* GTO angular part: l = 0 (s) or 1 (p, m = 0,1,2 -> x,y,z) */
+  int32_t nprim;
+  int32_t prim0; /* offset into the prim_* arrays */
+} qmc_ao;
+
+#define QMC_JAS_NONE 0
+#define QMC_JAS_PAIR 1 /* Jastrow1 / Jastrow2: u = sum_{i<j} -F/(2(1+r_ij/F)), jastrow.rs:15-126 */
+#define QMC_JAS_PADE 2 /* Jastrow3 (jastrow.rs:138-450) plus an optional Pade power series */
+
+typedef struct {
+  int32_t n_elec;
+  const int32_t* spins; /* [n_elec], +1 or -1 */
+  int32_t n_nuc;
+  const double* nuc_pos;    /* [n_nuc][3] */
+  const double* nuc_charge; /* [n_nuc] */
+  int32_t n_ao;
+  const qmc_ao* aos;        /* [n_ao] */
+  const double* prim_exp;   /* GTO alpha or STO zeta */
+  const double* prim_coef;  /* contraction coefficients */
+  const int32_t* prim_pow;  /* STO powers (unused by GTO; may be NULL without STO) */
+  int32_t n_mo;             /* rows of each MO matrix, >= electrons of each spin */
+  const double* mo_up;      /* [n_mo][n_ao]; spin-up determinant uses rows 0..n_up-1 */
+  const double* mo_dn;      /* [n_mo][n_ao]; may equal mo_up (restricted orbitals) */
+  int32_t jastrow_kind;     /* QMC_JAS_* */
+  double jas_F;             /* QMC_JAS_PAIR cusp parameter F */
+  double b_ee, b_en, a_anti, a_para; /* QMC_JAS_PADE (Jastrow3 fields) */
+  int32_t k_ee, k_en;       /* Pade series lengths; 0, 0 is the reference Jastrow3 */
+  const double* c_anti;     /* [k_ee] coefficients of x^2..x^(k_ee+1), antiparallel pairs */
+  const double* c_para;     /* [k_ee] parallel pairs */
+  int32_t n_species;        /* nuclear species of the e-n series */
+  const int32_t* species;   /* [n_nuc] species index of each nucleus (NULL if k_en = 0) */
+  const double* d_en;       /* [n_species][k_en] coefficients of y^2..y^(k_en+1) */
+  const int32_t* init_nuc;  /* [n_elec] nucleus each electron is initialised around */
+  double init_sigma;        /* width of the initial Gaussian cloud */
+  uint32_t flags;           /* QMC_FLAG_* */
+} qmc_system_desc;
+
+typedef struct qmc_system qmc_system;
+
+/* Build a system from a description (all arrays are copied). */
+int qmc_system_create(const qmc_system_desc* desc, int32_t precision, qmc_system** out);
+int qmc_system_destroy(qmc_system* sys);
+
+/* Constructors of the reference's systems.  flags select reference quirks. */
+/* H2MoleculeMO {Slater1s(alpha, c1), Slater1s(alpha, c2), Jastrow1{F}}, hydrogen.rs:28-35 */
+int qmc_h2_molecule_mo_create(double alpha, const double c1[3], const double c2[3], double F,
+                              double init_sigma, int32_t precision, qmc_system** out);
+/* Lithium{STOSlaterDet{orbitals, spins}, Jastrow2{F, n}}, lithium.rs:9-19.
+ * Orbital o has nprim[o] terms (coef, pow, zeta) starting at sum(nprim[:o]);
+ * all centred at the origin with nuclear charge Z (lithium.rs:60-63). */
+int qmc_sto_slater_det_create(int32_t n, const int32_t* nprim, const double* coef, const int32_t* pow,
+                              const double* zeta, const int32_t* spins, double F, double Z,
+                              uint32_t flags, int32_t precision, qmc_system** out);
+/* Li atom with seeded STO orbitals (the Li benchmark workload): electrons
+ * [up, down, up] in orbitals [1s, 1s, 2s] as src/main.rs:36-49; each
+ * orbital A sum_{nu=0..2} c_nu r^nu exp(-xi r) with xi = 2.7 (1s), 0.65
+ * (2s), c_nu = 2U-1 from Philox counter (3o+nu, 1, 0, 4) and A normalising
+ * the orbital; Jastrow2 with F = 1. */
+int qmc_lithium_seeded_create(uint64_t seed, uint32_t flags, int32_t precision, qmc_system** out);
+/* MethaneGTO::new(b_ee, b_en), methane.rs:623-685; basis 0 = the reference's
+ * 17-AO 6-31G with its fixed MO coefficients, 1 = STO-3G (9 AOs) with
+ * seeded orthonormal 5x9 MO coefficients; ch_bond = C-H distance (bohr). */
+#define QMC_BASIS_631G 0
+#define QMC_BASIS_STO3G 1
+int qmc_methane_gto_create(double b_ee, double b_en, double ch_bond, int32_t basis, uint64_t mo_seed,
+                           int32_t precision, qmc_system** out);
+/* BenzeneGTO::new(b_ee, b_en), benzene.rs:71-112, with C-C / C-H lengths */
+int qmc_benzene_gto_create(double b_ee, double b_en, double cc_bond, double ch_bond, int32_t precision,
+                           qmc_system** out);
+
+int qmc_set_precision(qmc_system* sys, int32_t precision);
+int qmc_set_flags(qmc_system* sys, uint32_t flags);
+int qmc_system_info(const qmc_system* sys, int32_t* n_elec, int32_t* n_up, int32_t* n_nuc, int32_t* n_ao);
+/* MO matrix of one spin (0 up, 1 down) as [n_mo][n_ao] */
+int qmc_get_mo(const qmc_system* sys, int32_t spin, double* mo);
+/* MethaneGTO::nuclear_repulsion, methane.rs:908-924 */
+int qmc_nuclear_repulsion(const qmc_system* sys, double* vnn);
+
+/* OptimizableWfn (traits.rs:98-115); parameter order [b_ee, b_en, a_anti,
+ * a_para] as methane.rs:1191-1198, then the Pade series coefficients */
+int qmc_num_params(const qmc_system* sys, int32_t* n);
+int qmc_get_params(const qmc_system* sys, double* p);
+int qmc_set_params(qmc_system* sys, const double* p, int32_t n);
+/* switch the Pade series on (k_ee, k_en > 0) or off; coefficients start at 0 */
+int qmc_set_jastrow_series(qmc_system* sys, int32_t k_ee, int32_t k_en);
+
+/* ForceCalculator geometry (sampling/traits.rs:20-35) */
+int qmc_get_nuclei(const qmc_system* sys, double* R, double* Z);
+int qmc_set_nuclei(qmc_system* sys, const double* R, int32_t n_nuc);
+
+/* ---- batched per-configuration trait methods ---- */
+/* MultiWfn::evaluate (traits.rs:53) as ln|psi| and sign */
+int qmc_eval_log_psi(const qmc_system* sys, int64_t W, const double* pos, double* logabs, double* sign,
+                     void* stream);
+/* MultiWfn::derivative / psi (traits.rs:56); drift = 2 g (vmc.rs:255-261) */
+int qmc_grad_log_psi(const qmc_system* sys, int64_t W, const double* pos, double* grad, void* stream);
+/* MultiWfn::laplacian / psi (traits.rs:59), per electron [W][N] */
+int qmc_lap_over_psi(const qmc_system* sys, int64_t W, const double* pos, double* lap, void* stream);
+/* EnergyCalculator::local_energy (sampling/traits.rs:7) */
+int qmc_local_energy(const qmc_system* sys, int64_t W, const double* pos, double* eloc, void* stream);
+/* all of the above in one pass; any output may be NULL */
+int qmc_evaluate(const qmc_system* sys, int64_t W, const double* pos, double* logabs, double* sign,
+                 double* grad, double* lap, double* eloc, void* stream);
+
+/* ---- walker ensembles resident in HBM ---- */
+typedef struct qmc_ensemble qmc_ensemble;
+
+/* W walkers initialised from Philox counter (walker, 0, electron, 1):
+ * electron e at nucleus init_nuc[e] + init_sigma * N(0,1)^3.  Replaces
+ * MCMCSimulation::initialize (vmc.rs:57-66). */
+int qmc_ensemble_create(const qmc_system* sys, int64_t W, uint64_t seed, qmc_ensemble** out);
+int qmc_ensemble_destroy(qmc_ensemble* ens);
+int qmc_ensemble_get_positions(const qmc_ensemble* ens, double* pos);
+int qmc_ensemble_set_positions(qmc_ensemble* ens, const double* pos);
+/* current ln|psi|, sign and cached E_L per walker */
+int qmc_ensemble_get_state(const qmc_ensemble* ens, double* logabs, double* sign, double* eloc);
+/* recompute inverses, ln|D|, ln J and E_L from the positions (FP64 refresh) */
+int qmc_ensemble_refresh(qmc_ensemble* ens);
+
+#define QMC_MOVE_SINGLE 0 /* single-electron Gaussian moves, Sherman-Morrison */
+#define QMC_MOVE_ALL 1    /* all-electron moves, MCMCSimulation::metropolis_step (vmc.rs:95-119) */
+
+typedef struct {
+  int32_t move;             /* QMC_MOVE_* */
+  int32_t n_steps;          /* sweeps to run */
+  double step_size;         /* Gaussian proposal width (bohr) */
+  int32_t refresh_interval; /* FP64 inverse refresh every this many sweeps (0: never) */
+  int32_t steps_per_launch; /* sweeps per persistent launch (0: library default) */
+} qmc_sweep_params;
+
+/* Run sweeps.  Step counters continue across calls.  Outputs (any may be
+ * NULL, host or device): per-sweep ensemble sums of E_L and E_L^2 and the
+ * number of accepted moves, each [n_steps]; eloc_trace [n_steps][W]. */
+int qmc_ensemble_sweeps(qmc_ensemble* ens, const qmc_sweep_params* p, double* e_sum, double* e2_sum,
+                        double* accepted, double* eloc_trace);
+
+/* Kernel timing: when enabled, every sweep launch of qmc_ensemble_sweeps is
+ * bracketed by HIP events on the launch stream; qmc_ensemble_timing
+ * returns the summed sweep-kernel milliseconds and launch count of the
+ * last qmc_ensemble_sweeps call. */
+int qmc_ensemble_set_timing(qmc_ensemble* ens, int32_t on);
+int qmc_ensemble_timing(const qmc_ensemble* ens, double* sweep_kernel_ms, int64_t* launches);
+
+/* MCMCParams (vmc.rs:12-21) plus the options the reference hard-wires */
+typedef struct {
+  int64_t n_walkers;
+  int64_t n_steps;
+  double initial_step_size;
+  double max_step_size;
+  double min_step_size;
+  double target_acceptance;
+  int64_t adaptation_interval;
+  int64_t n_burnin;         /* leading steps left out of the statistics (reference: 0) */
+  int32_t move;             /* QMC_MOVE_ALL reproduces metropolis_step */
+  int32_t refresh_interval;
+} qmc_mcmc_params;
+
+/* MCMCResults (vmc.rs:30-35) plus acceptance and the final step size */
+typedef struct {
+  double energy;
+  double error;
+  double autocorrelation_time;
+  double acceptance;
+  double final_step_size;
+} qmc_vmc_results;
+
+/* MCMCSimulation::run (vmc.rs:69-92) with compute_results (vmc.rs:131-188).
+ * energy_trace (host) receives the per-step mean energies [n_steps]. */
+int qmc_vmc_run(const qmc_system* sys, const qmc_mcmc_params* p, uint64_t seed, qmc_vmc_results* res,
+                double* energy_trace);
+
+/* vmc.rs:141-163 and vmc.rs:166-188 on a host series */
+double qmc_autocorrelation_time(const double* e, int64_t n);
+double qmc_blocking_error(const double* e, int64_t n, double tau);
+
+const char* qmc_last_error(void);
+const char* qmc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RUST_QMC_AMD_QMC_H */

@@ -150,7 +150,8 @@ void orc_mcmc_time_steps(const orc_system* s, int64_t W, int64_t warm, int64_t s
 }
 
 /* Single-electron Metropolis sweeps, the move scheme of the GPU sampler:
- * for each electron e in index order, r_e' = r_e + step * z with z from
+ * for each electron e in sector order (spin-up electrons by increasing
+ * index, then spin-down), r_e' = r_e + step * z with z from
  * counter (w, step, 2e, SINGLE), accepted when U < |psi'/psi|^2 with U
  * from (w, step, 2e+1, SINGLE) word 0.  The ratio is taken from full
  * recomputation of ln|psi| (brute force, no incremental update).  After
@@ -160,13 +161,19 @@ void orc_single_electron_steps(const orc_system* s, int64_t W, double* pos, doub
                                double* eloc, int64_t* accepted) {
     int n = s->n_elec;
     int64_t acc = 0;
+    int order[ORC_MAXE], q = 0;
+    for (int e = 0; e < n; ++e)
+        if (s->spins[e] == 1) order[q++] = e;
+    for (int e = 0; e < n; ++e)
+        if (s->spins[e] != 1) order[q++] = e;
     for (int64_t w = 0; w < W; ++w) {
         double(*r)[3] = (double(*)[3])(pos + w * n * 3);
         double lp, sg;
         orc_log_psi(s, (const double(*)[3])r, &lp, &sg);
         for (int k = 0; k < nsteps; ++k) {
             uint32_t st = step0 + (uint32_t)k;
-            for (int e = 0; e < n; ++e) {
+            for (int qq = 0; qq < n; ++qq) {
+                const int e = order[qq];
                 double z[3], old[3];
                 orc_normals3(seed, (uint32_t)w, st, (uint32_t)(2 * e), ORC_RNG_SINGLE, z);
                 for (int q = 0; q < 3; ++q) { old[q] = r[e][q]; r[e][q] += step * z[q]; }

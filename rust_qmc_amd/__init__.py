@@ -1,0 +1,434 @@
+"""rust_qmc_amd: MI355X-native VMC walker-ensemble engine.
+
+Python mirror of the rust_qmc trait surface over the C ABI of
+include/qmc.h.  Class and method names follow the reference crate (paths
+relative to its root):
+
+  H2MoleculeMO      src/systems/hydrogen.rs:28-35
+  Lithium           src/systems/lithium.rs:9-19 (STOSlaterDet x Jastrow2)
+  MethaneGTO        src/systems/methane.rs:601-685
+  BenzeneGTO        src/systems/benzene.rs:49-112
+  MCMCParams / MCMCSimulation / MCMCResults   src/sampling/vmc.rs:12-189
+
+The trait methods evaluate / derivative / laplacian return the reference's
+linear quantities (psi, grad psi, lap psi per electron) computed as
+sign*exp(log psi) times the log-domain results of the GPU; log_psi,
+grad_log_psi and lap_over_psi return the log-domain values themselves.
+Every method runs on the GPU through libqmc_hip.so; nothing here computes
+physics on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (QMC_FP64, QMC_MIXED, QMC_FLAGS_REFERENCE, QMC_MOVE_ALL, QMC_MOVE_SINGLE, QmcError, check,
+                   lib)
+
+__all__ = [
+    "QmcError", "System", "H2MoleculeMO", "Lithium", "STOSlaterDet", "MethaneGTO", "BenzeneGTO", "Ensemble",
+    "DeviceBuffer", "check", "lib",
+    "MCMCParams", "MCMCResults", "MCMCSimulation", "autocorrelation_time", "blocking_error", "QMC_FP64",
+    "QMC_MIXED", "QMC_MOVE_SINGLE", "QMC_MOVE_ALL", "QMC_FLAGS_REFERENCE",
+]
+
+_PREC = {"fp64": QMC_FP64, "mixed": QMC_MIXED, QMC_FP64: QMC_FP64, QMC_MIXED: QMC_MIXED}
+
+
+def _ptr(a):
+    """ctypes pointer for a numpy array or a torch tensor (host or device)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class System:
+    """A wavefunction + Hamiltonian handle (MultiWfn + EnergyCalculator +
+    OptimizableWfn + ForceCalculator)."""
+
+    def __init__(self, handle: C.c_void_p, precision):
+        self._h = handle
+        self.precision = _PREC[precision]
+        ne, nu, nn, na = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().qmc_system_info(self._h, C.byref(ne), C.byref(nu), C.byref(nn), C.byref(na)))
+        self.n_elec, self.n_up, self.n_nuc, self.n_ao = ne.value, nu.value, nn.value, na.value
+        self.n_dn = self.n_elec - self.n_up
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.qmc_system_destroy(h)
+            self._h = None
+
+    @staticmethod
+    def _new(fn, *args):
+        h = C.c_void_p()
+        check(fn(*args, C.byref(h)))
+        return h
+
+    @classmethod
+    def from_desc(cls, desc: dict, precision="fp64"):
+        """Generic description (see qmc_system_desc in include/qmc.h)."""
+        keep = []
+
+        def arr(x, dt):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(x, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        aos = (_lib.QmcAO * len(desc["aos"]))(*[_lib.QmcAO(*a) for a in desc["aos"]])
+        keep.append(aos)
+        mo_up = _f64(desc["mo_up"])
+        d = _lib.QmcSystemDesc(
+            n_elec=len(desc["spins"]), spins=arr(desc["spins"], np.int32), n_nuc=len(desc["charges"]),
+            nuc_pos=arr(desc["nuclei"], np.float64), nuc_charge=arr(desc["charges"], np.float64),
+            n_ao=len(desc["aos"]), aos=C.cast(aos, C.c_void_p), prim_exp=arr(desc["prim_exp"], np.float64),
+            prim_coef=arr(desc["prim_coef"], np.float64), prim_pow=arr(desc.get("prim_pow"), np.int32),
+            n_mo=mo_up.shape[0], mo_up=arr(mo_up, np.float64), mo_dn=arr(desc.get("mo_dn", mo_up), np.float64),
+            jastrow_kind=desc.get("jastrow_kind", 0), jas_F=desc.get("F", 1.0), b_ee=desc.get("b_ee", 0.0),
+            b_en=desc.get("b_en", 0.0), a_anti=desc.get("a_anti", 0.5), a_para=desc.get("a_para", 0.25),
+            k_ee=0, k_en=0, c_anti=None, c_para=None, n_species=0, species=None, d_en=None,
+            init_nuc=arr(desc.get("init_nuc", [0] * len(desc["spins"])), np.int32),
+            init_sigma=desc.get("init_sigma", 1.0), flags=desc.get("flags", 0))
+        h = cls._new(lib().qmc_system_create, C.byref(d), _PREC[precision])
+        return cls(h, precision)
+
+    # ---- configuration -------------------------------------------------------
+    def set_precision(self, precision):
+        check(lib().qmc_set_precision(self._h, _PREC[precision]))
+        self.precision = _PREC[precision]
+
+    def set_flags(self, flags):
+        check(lib().qmc_set_flags(self._h, flags))
+
+    def nuclear_repulsion(self):
+        v = C.c_double()
+        check(lib().qmc_nuclear_repulsion(self._h, C.byref(v)))
+        return v.value
+
+    # OptimizableWfn (traits.rs:98-115)
+    def num_params(self):
+        n = C.c_int32()
+        check(lib().qmc_num_params(self._h, C.byref(n)))
+        return n.value
+
+    def get_params(self):
+        p = np.zeros(max(1, self.num_params()))
+        check(lib().qmc_get_params(self._h, _ptr(p)))
+        return p[: self.num_params()]
+
+    def set_params(self, p):
+        p = _f64(p)
+        check(lib().qmc_set_params(self._h, _ptr(p), p.size))
+
+    def set_jastrow_series(self, k_ee, k_en):
+        check(lib().qmc_set_jastrow_series(self._h, k_ee, k_en))
+
+    # ForceCalculator geometry (sampling/traits.rs:20-35)
+    def num_nuclei(self):
+        return self.n_nuc
+
+    def get_nuclei(self):
+        R = np.zeros((self.n_nuc, 3))
+        check(lib().qmc_get_nuclei(self._h, _ptr(R), None))
+        return R
+
+    def get_charges(self):
+        Z = np.zeros(self.n_nuc)
+        check(lib().qmc_get_nuclei(self._h, None, _ptr(Z)))
+        return Z
+
+    def set_nuclei(self, R):
+        R = _f64(R).reshape(self.n_nuc, 3)
+        check(lib().qmc_set_nuclei(self._h, _ptr(R), self.n_nuc))
+
+    # ---- batched trait methods (positions [W][N][3]) ---------------------------
+    def evaluate_batch(self, pos, want=("logabs", "sign", "grad", "lap", "eloc")):
+        """Run qmc_evaluate on host positions [W][N][3]."""
+        pos = _f64(pos)
+        W = pos.shape[0]
+        mk = lambda *s: np.zeros(s)
+        n = self.n_elec
+        out = {}
+        out["logabs"] = mk(W) if "logabs" in want else None
+        out["sign"] = mk(W) if "sign" in want else None
+        out["grad"] = mk(W, n, 3) if "grad" in want else None
+        out["lap"] = mk(W, n) if "lap" in want else None
+        out["eloc"] = mk(W) if "eloc" in want else None
+        check(lib().qmc_evaluate(self._h, W, _ptr(pos), _ptr(out["logabs"]), _ptr(out["sign"]), _ptr(out["grad"]),
+                                 _ptr(out["lap"]), _ptr(out["eloc"]), None))
+        return {k: v for k, v in out.items() if v is not None}
+
+    def log_psi(self, r):
+        o = self.evaluate_batch(_f64(r)[None], ("logabs", "sign"))
+        return o["logabs"][0], o["sign"][0]
+
+    def grad_log_psi(self, r):
+        return self.evaluate_batch(_f64(r)[None], ("grad",))["grad"][0]
+
+    def lap_over_psi(self, r):
+        return self.evaluate_batch(_f64(r)[None], ("lap",))["lap"][0]
+
+    # MultiWfn (traits.rs:48-59), linear convention
+    def evaluate(self, r):
+        la, sg = self.log_psi(r)
+        return sg * np.exp(la)
+
+    def derivative(self, r):
+        o = self.evaluate_batch(_f64(r)[None], ("logabs", "sign", "grad"))
+        return o["sign"][0] * np.exp(o["logabs"][0]) * o["grad"][0]
+
+    def laplacian(self, r):
+        o = self.evaluate_batch(_f64(r)[None], ("logabs", "sign", "lap"))
+        return o["sign"][0] * np.exp(o["logabs"][0]) * o["lap"][0]
+
+    # EnergyCalculator (sampling/traits.rs:6-8)
+    def local_energy(self, r):
+        return self.evaluate_batch(_f64(r)[None], ("eloc",))["eloc"][0]
+
+    def initialize(self, seed=0, walker=0):
+        """MultiWfn::initialize with a Philox stream (walker `walker`)."""
+        ens = Ensemble(self, walker + 1, seed)
+        return ens.positions()[walker]
+
+
+class H2MoleculeMO(System):
+    """hydrogen.rs:28-35: (phi1+phi2)(r1) (phi1+phi2)(r2) J1(r12)."""
+
+    def __init__(self, alpha=1.3, center1=(0.0, 0.0, 0.7), center2=(0.0, 0.0, -0.7), cusp_param=3.0,
+                 init_sigma=1.0, precision="fp64"):
+        c1 = _f64(center1)
+        c2 = _f64(center2)
+        h = self._new(lib().qmc_h2_molecule_mo_create, alpha, _ptr(c1), _ptr(c2), cusp_param, init_sigma,
+                      _PREC[precision])
+        super().__init__(h, precision)
+
+
+class STOSlaterDet(System):
+    """STOSlaterDet x Jastrow2 (slater.rs:149-271, lithium.rs:9-82).
+    orbitals: list of (coefficients, powers, exponents); spins per column."""
+
+    def __init__(self, orbitals, spins, cusp_param=1.0, Z=3.0, flags=QMC_FLAGS_REFERENCE, precision="fp64"):
+        nprim = np.array([len(o[0]) for o in orbitals], dtype=np.int32)
+        coef = _f64(np.concatenate([o[0] for o in orbitals]))
+        pw = np.ascontiguousarray(np.concatenate([o[1] for o in orbitals]), dtype=np.int32)
+        zeta = _f64(np.concatenate([o[2] for o in orbitals]))
+        sp = np.ascontiguousarray(spins, dtype=np.int32)
+        h = self._new(lib().qmc_sto_slater_det_create, len(orbitals), _ptr(nprim), _ptr(coef), _ptr(pw), _ptr(zeta),
+                      _ptr(sp), cusp_param, Z, flags, _PREC[precision])
+        super().__init__(h, precision)
+
+
+# slater.rs:68-89 (init_li_sto): 7 primitives, powers [1,1,2,2,2,2,3]
+_LI_STO_EXP = [0.72089388, 2.61691643, 0.69257443, 1.37137558, 3.97864549, 13.52900016, 19.30801440]
+_LI_STO_POW = [1, 1, 2, 2, 2, 2, 3]
+_LI_STO_C = {
+    1: [-0.12220686, 1.11273225, 0.04125378, 0.09306499, -0.10260021, -0.00034191, 0.00021963],
+    2: [0.47750469, 0.11140449, -1.25954273, -0.18475003, -0.02736293, -0.00025064, 0.00057962],
+}
+
+
+def init_li_sto(n):
+    """slater.rs:68-89"""
+    return (_LI_STO_C.get(n, [0.0] * 7), _LI_STO_POW, _LI_STO_EXP)
+
+
+class Lithium(STOSlaterDet):
+    """src/main.rs:36-49: orbitals [1s, 1s, 2s], spins [+1, -1, +1], Jastrow2 F = 1.
+    seed=None builds the reference's init_li_sto atom; an integer seed builds
+    the seeded 3-term STO atom of the Li benchmark workload."""
+
+    def __init__(self, seed=None, flags=QMC_FLAGS_REFERENCE, precision="fp64"):
+        if seed is None:
+            super().__init__([init_li_sto(1), init_li_sto(1), init_li_sto(2)], [1, -1, 1], 1.0, 3.0, flags, precision)
+        else:
+            h = self._new(lib().qmc_lithium_seeded_create, seed, flags, _PREC[precision])
+            System.__init__(self, h, precision)
+
+
+class MethaneGTO(System):
+    """methane.rs:601-685.  basis='6-31G' is the reference's 17-AO system;
+    basis='STO-3G' is the 9-AO system with seeded orthonormal MOs."""
+
+    def __init__(self, b_ee=1.5, b_en=2.0, ch_bond=2.05, basis="6-31G", mo_seed=0, precision="fp64"):
+        bk = {"6-31G": _lib.QMC_BASIS_631G, "STO-3G": _lib.QMC_BASIS_STO3G}[basis]
+        h = self._new(lib().qmc_methane_gto_create, b_ee, b_en, ch_bond, bk, mo_seed, _PREC[precision])
+        super().__init__(h, precision)
+
+
+class BenzeneGTO(System):
+    """benzene.rs:49-112"""
+
+    def __init__(self, b_ee=1.5, b_en=2.0, cc_bond=2.64, ch_bond=2.042, precision="fp64"):
+        h = self._new(lib().qmc_benzene_gto_create, b_ee, b_en, cc_bond, ch_bond, _PREC[precision])
+        super().__init__(h, precision)
+
+
+_hip_rt = None
+
+
+def _hip():
+    """The HIP runtime libqmc_hip.so itself links (libamdhip64.so.7)."""
+    global _hip_rt
+    if _hip_rt is None:
+        lib()
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipDeviceSynchronize.argtypes = []
+        _hip_rt = h
+    return _hip_rt
+
+
+class DeviceBuffer:
+    """Device memory from the library's own HIP runtime, for callers that
+    hand device pointers to the ABI.  (PyTorch bundles a second HIP runtime;
+    the two are not mixed in one process.)"""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        if _hip().hipMalloc(C.byref(self.ptr), max(self.nbytes, 8)) != 0:
+            raise QmcError(_lib.QMC_ERR_HIP, "hipMalloc failed")
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if _hip().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, 1) != 0:
+            raise QmcError(_lib.QMC_ERR_HIP, "hipMemcpy H2D failed")
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        if _hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes, 2) != 0:
+            raise QmcError(_lib.QMC_ERR_HIP, "hipMemcpy D2H failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            _hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+
+class Ensemble:
+    """W walkers resident in HBM (qmc_ensemble)."""
+
+    def __init__(self, system: System, n_walkers: int, seed: int = 0):
+        self.system = system
+        self.W = n_walkers
+        self._h = C.c_void_p()
+        check(lib().qmc_ensemble_create(system._h, n_walkers, seed, C.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.qmc_ensemble_destroy(h)
+            self._h = None
+
+    def positions(self):
+        p = np.zeros((self.W, self.system.n_elec, 3))
+        check(lib().qmc_ensemble_get_positions(self._h, _ptr(p)))
+        return p
+
+    def set_positions(self, pos):
+        pos = _f64(pos)
+        check(lib().qmc_ensemble_set_positions(self._h, _ptr(pos)))
+
+    def state(self):
+        la, sg, el = np.zeros(self.W), np.zeros(self.W), np.zeros(self.W)
+        check(lib().qmc_ensemble_get_state(self._h, _ptr(la), _ptr(sg), _ptr(el)))
+        return la, sg, el
+
+    def refresh(self):
+        check(lib().qmc_ensemble_refresh(self._h))
+
+    def set_timing(self, on=True):
+        check(lib().qmc_ensemble_set_timing(self._h, int(on)))
+
+    def timing(self):
+        """(summed sweep-kernel ms, launches) of the last sweeps() call."""
+        ms, n = C.c_double(), C.c_int64()
+        check(lib().qmc_ensemble_timing(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def sweeps(self, n_steps, step_size, move=QMC_MOVE_SINGLE, refresh_interval=64, steps_per_launch=0,
+               trace=False):
+        """Returns (e_sum, e2_sum, accepted[, eloc_trace]) per sweep."""
+        p = _lib.QmcSweepParams(move, n_steps, step_size, refresh_interval, steps_per_launch)
+        es, e2, acc = np.zeros(n_steps), np.zeros(n_steps), np.zeros(n_steps)
+        tr = np.zeros((n_steps, self.W)) if trace else None
+        check(lib().qmc_ensemble_sweeps(self._h, C.byref(p), _ptr(es), _ptr(e2), _ptr(acc), _ptr(tr)))
+        return (es, e2, acc, tr) if trace else (es, e2, acc)
+
+
+@dataclass
+class MCMCParams:
+    """vmc.rs:12-21 plus n_burnin / move / refresh_interval."""
+    n_walkers: int
+    n_steps: int
+    initial_step_size: float
+    max_step_size: float
+    min_step_size: float
+    target_acceptance: float = 0.5
+    adaptation_interval: int = 100
+    n_burnin: int = 0
+    move: int = QMC_MOVE_ALL
+    refresh_interval: int = 64
+
+
+@dataclass
+class MCMCResults:
+    """vmc.rs:30-35 plus acceptance and final step size."""
+    energy: float
+    error: float
+    autocorrelation_time: float
+    acceptance: float
+    final_step_size: float
+
+
+class MCMCSimulation:
+    """vmc.rs:38-189 on the GPU."""
+
+    def __init__(self, wavefunction: System, params: MCMCParams, seed: int = 0):
+        self.wavefunction = wavefunction
+        self.params = params
+        self.seed = seed
+        self.energies = None
+
+    def run(self) -> MCMCResults:
+        p = self.params
+        cp = _lib.QmcMcmcParams(p.n_walkers, p.n_steps, p.initial_step_size, p.max_step_size, p.min_step_size,
+                                p.target_acceptance, p.adaptation_interval, p.n_burnin, p.move, p.refresh_interval)
+        res = _lib.QmcVmcResults()
+        tr = np.zeros(p.n_steps)
+        check(lib().qmc_vmc_run(self.wavefunction._h, C.byref(cp), self.seed, C.byref(res), _ptr(tr)))
+        self.energies = tr
+        return MCMCResults(res.energy, res.error, res.autocorrelation_time, res.acceptance, res.final_step_size)
+
+
+def autocorrelation_time(e):
+    e = _f64(e)
+    return lib().qmc_autocorrelation_time(_ptr(e), e.size)
+
+
+def blocking_error(e, tau):
+    e = _f64(e)
+    return lib().qmc_blocking_error(_ptr(e), e.size, tau)

@@ -1,0 +1,446 @@
+// qmc_device.hpp -- device-side building blocks of the walker engine for
+// gfx950: system parameters, Philox4x32-10, orbital / Jastrow evaluation,
+// register-resident Gauss-Jordan inversion and the per-electron local-energy
+// terms.  One thread owns one walker; every array below is indexed with
+// compile-time indices only, so it lives in VGPRs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qmc {
+
+constexpr int MAXN = 16;      // nuclei
+constexpr int MAXSH = 96;     // shells
+constexpr int MAXAO = 128;    // atomic orbitals
+constexpr int MAXPRIM = 512;  // primitive slots over all shells
+constexpr int MAXMO = 32;     // molecular orbitals per spin
+constexpr int MAXK = 8;       // Pade series terms
+constexpr int MAXSPEC = 4;    // nuclear species of the e-n series
+constexpr int MAXE = 64;      // electrons
+
+// shell kinds
+enum : int { SH_S = 0, SH_P = 1, SH_SP = 2, SH_P1 = 3, SH_STO = 4, SH_SLATER1S = 5 };
+// Jastrow kinds
+enum : int { JAS_NONE = 0, JAS_PAIR = 1, JAS_PADE = 2 };
+// Philox streams (fourth counter word)
+enum : uint32_t { RNG_INIT = 1u, RNG_SINGLE = 2u, RNG_ALL = 3u, RNG_SYS = 4u, RNG_DRIFT = 5u };
+
+// flags (mirrors include/qmc.h)
+enum : uint32_t {
+  F_J2_GRAD_QUIRK = 1u << 0,
+  F_J2_LAP_QUIRK = 1u << 1,
+  F_HF_NN_REFSIGN = 1u << 2,
+  F_SINGULAR_IDENTITY = 1u << 3,
+};
+
+// Everything a kernel needs to know about the system.  All fields are read
+// with wave-uniform addresses, so they come through the scalar cache.
+template <class T>
+struct DevSys {
+  int n_elec, n_up, n_dn, n_nuc, n_shell, n_ao, n_mo;
+  int jas_kind, en_on, k_ee, k_en, n_species;
+  uint32_t flags;
+  T b_ee, b_en, a_anti, a_para, cst_ee, grad_scale;
+  T c_anti[MAXK], c_para[MAXK];
+  T d_en[MAXSPEC][MAXK];
+  int species[MAXN];
+  T nuc[MAXN][3];
+  T Z[MAXN];
+  double vnn;
+  int sh_kind[MAXSH], sh_center[MAXSH], sh_nprim[MAXSH], sh_prim0[MAXSH], sh_ao0[MAXSH], sh_m[MAXSH];
+  T prim_exp[MAXPRIM], coef_s[MAXPRIM], coef_p[MAXPRIM];
+  int prim_pow[MAXPRIM];
+  int sector_elec[2][MAXE];   // global index of the k-th electron of each spin sector
+  int init_nuc[MAXE];
+  T init_sigma;
+  T moT[2][MAXAO][MAXMO];     // [spin][ao][mo], mo fastest
+};
+
+template <int N>
+struct Cap {
+  static constexpr int v = N > 0 ? N : 1;
+};
+
+// ---------------------------------------------------------------------------
+// precision-dependent math: FP64 uses the correctly rounded library
+// routines; FP32 uses the hardware transcendental instructions.
+// ---------------------------------------------------------------------------
+template <class T>
+struct M;
+template <>
+struct M<double> {
+  static __device__ __forceinline__ double ex(double x) { return exp(x); }
+  static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  static __device__ __forceinline__ double lg(double x) { return log(x); }
+};
+template <>
+struct M<float> {
+  static __device__ __forceinline__ float ex(float x) { return __expf(x); }
+  static __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float lg(float x) { return __logf(x); }
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11) and Box-Muller normals
+// ---------------------------------------------------------------------------
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                     uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return U4{c0, c1, c2, c3};
+}
+
+__device__ __forceinline__ double u01d(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; }
+__device__ __forceinline__ float u01f(uint32_t w) { return ((float)(w >> 8) + 0.5f) * 5.9604644775390625e-08f; }
+
+template <class T>
+struct Rng;
+template <>
+struct Rng<double> {
+  static __device__ __forceinline__ void normals3(U4 o, double z[3]) {
+    const double twopi = 6.283185307179586;
+    double u1 = u01d(o.x), u2 = u01d(o.y), u3 = u01d(o.z), u4 = u01d(o.w);
+    double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    double s, c;
+    sincos(twopi * u2, &s, &c);
+    z[0] = r1 * c;
+    z[1] = r1 * s;
+    z[2] = r2 * cos(twopi * u4);
+  }
+  static __device__ __forceinline__ double uniform(uint32_t w) { return u01d(w); }
+};
+template <>
+struct Rng<float> {
+  static __device__ __forceinline__ void normals3(U4 o, float z[3]) {
+    const float twopi = 6.2831853071795865f;
+    float u1 = u01f(o.x), u2 = u01f(o.y), u3 = u01f(o.z), u4 = u01f(o.w);
+    float r1 = __builtin_amdgcn_sqrtf(-2.0f * __logf(u1)), r2 = __builtin_amdgcn_sqrtf(-2.0f * __logf(u3));
+    z[0] = r1 * __cosf(twopi * u2);
+    z[1] = r1 * __sinf(twopi * u2);
+    z[2] = r2 * __cosf(twopi * u4);
+  }
+  static __device__ __forceinline__ float uniform(uint32_t w) { return u01f(w); }
+};
+
+// ---------------------------------------------------------------------------
+// Atomic orbitals contracted straight into MO accumulators.
+// GTO shells follow methane.rs:498-590 (unnormalised primitives, s and p);
+// STO shells follow slater.rs:92-144 (and slater.rs:17-41 for Slater1s).
+// ---------------------------------------------------------------------------
+template <int NS, class T>
+__device__ __forceinline__ void mo_add(const DevSys<T>& S, int spin, int ao, T v, T phi[Cap<NS>::v]) {
+#pragma unroll
+  for (int m = 0; m < NS; ++m) phi[m] += S.moT[spin][ao][m] * v;
+}
+
+// phi[m] = sum_ao C[spin][m][ao] chi_ao(r), m < NS
+template <int NS, class T>
+__device__ __forceinline__ void mo_values(const DevSys<T>& S, int spin, T x, T y, T z, T phi[Cap<NS>::v]) {
+#pragma unroll
+  for (int m = 0; m < NS; ++m) phi[m] = T(0);
+  for (int sh = 0; sh < S.n_shell; ++sh) {
+    const int c = S.sh_center[sh], kind = S.sh_kind[sh], p0 = S.sh_prim0[sh], np = S.sh_nprim[sh];
+    const int a0 = S.sh_ao0[sh];
+    T dx = x - S.nuc[c][0], dy = y - S.nuc[c][1], dz = z - S.nuc[c][2];
+    T r2 = dx * dx + dy * dy + dz * dz;
+    if (kind <= SH_P1) {
+      T rs = T(0), rp = T(0);
+      for (int k = 0; k < np; ++k) {
+        T e = M<T>::ex(-S.prim_exp[p0 + k] * r2);
+        rs += S.coef_s[p0 + k] * e;
+        rp += S.coef_p[p0 + k] * e;
+      }
+      if (kind == SH_S) {
+        mo_add<NS>(S, spin, a0, rs, phi);
+      } else if (kind == SH_P1) {
+        const int m = S.sh_m[sh];
+        mo_add<NS>(S, spin, a0, (m == 0 ? dx : (m == 1 ? dy : dz)) * rp, phi);
+      } else {
+        int b = a0;
+        if (kind == SH_SP) {
+          mo_add<NS>(S, spin, a0, rs, phi);
+          b = a0 + 1;
+        }
+        mo_add<NS>(S, spin, b, dx * rp, phi);
+        mo_add<NS>(S, spin, b + 1, dy * rp, phi);
+        mo_add<NS>(S, spin, b + 2, dz * rp, phi);
+      }
+    } else {
+      T s = M<T>::sq(r2);
+      T v = T(0);
+      for (int k = 0; k < np; ++k) {
+        T sp = T(1);
+        for (int q = 0; q < S.prim_pow[p0 + k]; ++q) sp *= s;
+        v += S.coef_s[p0 + k] * sp * M<T>::ex(-S.prim_exp[p0 + k] * s);
+      }
+      mo_add<NS>(S, spin, a0, v, phi);
+    }
+  }
+}
+
+// gradient and Laplacian of every MO at r: g[m][3], l[m]
+template <int NS, class T>
+__device__ __forceinline__ void mo_grad_lap(const DevSys<T>& S, int spin, T x, T y, T z, T g[Cap<NS>::v][3],
+                                            T l[Cap<NS>::v]) {
+#pragma unroll
+  for (int m = 0; m < NS; ++m) {
+    g[m][0] = g[m][1] = g[m][2] = T(0);
+    l[m] = T(0);
+  }
+  auto add = [&](int ao, T gx, T gy, T gz, T lap) {
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      const T c = S.moT[spin][ao][m];
+      g[m][0] += c * gx;
+      g[m][1] += c * gy;
+      g[m][2] += c * gz;
+      l[m] += c * lap;
+    }
+  };
+  for (int sh = 0; sh < S.n_shell; ++sh) {
+    const int c = S.sh_center[sh], kind = S.sh_kind[sh], p0 = S.sh_prim0[sh], np = S.sh_nprim[sh];
+    const int a0 = S.sh_ao0[sh];
+    T dx = x - S.nuc[c][0], dy = y - S.nuc[c][1], dz = z - S.nuc[c][2];
+    T r2 = dx * dx + dy * dy + dz * dz;
+    if (kind <= SH_P1) {
+      // R = sum c e, Ra = sum c a e, Raa = sum c a^2 e (s and p coefficient sets)
+      T rs = 0, rsa = 0, rsaa = 0, rp = 0, rpa = 0, rpaa = 0;
+      for (int k = 0; k < np; ++k) {
+        const T a = S.prim_exp[p0 + k];
+        T e = M<T>::ex(-a * r2);
+        T cs = S.coef_s[p0 + k] * e, cp = S.coef_p[p0 + k] * e;
+        rs += cs;
+        rsa += cs * a;
+        rsaa += cs * a * a;
+        rp += cp;
+        rpa += cp * a;
+        rpaa += cp * a * a;
+      }
+      if (kind == SH_S || kind == SH_SP) {
+        // grad = -2 Ra d; lap = -6 Ra + 4 r^2 Raa
+        add(a0, T(-2) * rsa * dx, T(-2) * rsa * dy, T(-2) * rsa * dz, T(-6) * rsa + T(4) * r2 * rsaa);
+      }
+      if (kind != SH_S) {
+        // p_m = d_m R: grad_j = delta_jm R - 2 d_m Ra d_j; lap = d_m (-10 Ra + 4 r^2 Raa)
+        const T lr = T(-10) * rpa + T(4) * r2 * rpaa;
+        const int b = (kind == SH_SP) ? a0 + 1 : a0;
+        if (kind == SH_P1) {
+          const int m = S.sh_m[sh];
+          T dm = (m == 0 ? dx : (m == 1 ? dy : dz));
+          T gx = T(-2) * dm * rpa * dx + (m == 0 ? rp : T(0));
+          T gy = T(-2) * dm * rpa * dy + (m == 1 ? rp : T(0));
+          T gz = T(-2) * dm * rpa * dz + (m == 2 ? rp : T(0));
+          add(a0, gx, gy, gz, dm * lr);
+        } else {
+          add(b, T(-2) * dx * rpa * dx + rp, T(-2) * dx * rpa * dy, T(-2) * dx * rpa * dz, dx * lr);
+          add(b + 1, T(-2) * dy * rpa * dx, T(-2) * dy * rpa * dy + rp, T(-2) * dy * rpa * dz, dy * lr);
+          add(b + 2, T(-2) * dz * rpa * dx, T(-2) * dz * rpa * dy, T(-2) * dz * rpa * dz + rp, dz * lr);
+        }
+      }
+    } else {
+      // radial STO: R' and R''; grad = R' d/s, lap = R'' + 2 R'/s
+      T s = M<T>::sq(r2);
+      T d1 = 0, d2 = 0;
+      for (int k = 0; k < np; ++k) {
+        const T zt = S.prim_exp[p0 + k];
+        const int p = S.prim_pow[p0 + k];
+        const T pf = T(p);
+        T spm2 = T(1), sp = T(1);
+        // s^p, s^(p-1), s^(p-2) with the p = 0, 1 terms multiplied by zero
+        for (int q = 0; q < p; ++q) sp *= s;
+        T spm1 = p >= 1 ? sp / s : T(0);
+        spm2 = p >= 2 ? spm1 / s : T(0);
+        T e = S.coef_s[p0 + k] * M<T>::ex(-zt * s);
+        d1 += e * (-zt * sp + pf * spm1);
+        d2 += e * (zt * zt * sp - T(2) * zt * pf * spm1 + pf * (pf - T(1)) * spm2);
+      }
+      T gx, gy, gz, lap;
+      if (s == T(0)) {
+        gx = gy = gz = T(0);
+        lap = (kind == SH_SLATER1S) ? -INFINITY : T(0);
+      } else {
+        T is = T(1) / s;
+        gx = d1 * dx * is;
+        gy = d1 * dy * is;
+        gz = d1 * dz * is;
+        lap = d2 + T(2) * d1 * is;
+      }
+      add(a0, gx, gy, gz, lap);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Jastrow pair functions (jastrow.rs:218-321 and the Pade series):
+//   u_ee = a x + cst + sum_k c_k x^(k+2),  x = r/(1+b r)
+//   u_en = -Z y + sum_k d_k y^(k+2),        y = r/(1+b_en r)
+// Jastrow1/Jastrow2 (jastrow.rs:30-125) are u_ee with a = 1/2, b = 1/F,
+// cst = -F/2, since -F/(2(1+r/F)) = r/(2(1+r/F)) - F/2.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void series(int K, const T* c, T x, T& s, T& s1, T& s2) {
+  T v = 0, v1 = 0, v2 = 0, xp = T(1);
+  for (int k = 0; k < K; ++k) {
+    const T n = T(k + 2);
+    v += c[k] * xp * x * x;
+    v1 += c[k] * n * xp * x;
+    v2 += c[k] * n * (n - T(1)) * xp;
+    xp *= x;
+  }
+  s = v;
+  s1 = v1;
+  s2 = v2;
+}
+
+// value only (cst excluded: it cancels in every ratio and is added to ln J separately)
+template <class T>
+__device__ __forceinline__ T uee_val(const DevSys<T>& S, T r, bool same) {
+  T D = T(1) + S.b_ee * r;
+  T x = r * M<T>::rcp(D);
+  T v = (same ? S.a_para : S.a_anti) * x;
+  if (S.k_ee > 0) {
+    T s, s1, s2;
+    series(S.k_ee, same ? S.c_para : S.c_anti, x, s, s1, s2);
+    v += s;
+  }
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T uen_val(const DevSys<T>& S, T r, int I) {
+  T D = T(1) + S.b_en * r;
+  T y = r * M<T>::rcp(D);
+  T v = -S.Z[I] * y;
+  if (S.k_en > 0) {
+    T s, s1, s2;
+    series(S.k_en, S.d_en[S.species[I]], y, s, s1, s2);
+    v += s;
+  }
+  return v;
+}
+
+// u'(r) and u''(r) + 2 u'(r)/r for an e-e pair
+template <class T>
+__device__ __forceinline__ void uee_d(const DevSys<T>& S, T r, T ir, bool same, T& u1, T& lap) {
+  T D = T(1) + S.b_ee * r;
+  T iD = M<T>::rcp(D);
+  T iD2 = iD * iD;
+  T a = same ? S.a_para : S.a_anti;
+  T s1 = 0, s2 = 0;
+  if (S.k_ee > 0) {
+    T s, x = r * iD;
+    series(S.k_ee, same ? S.c_para : S.c_anti, x, s, s1, s2);
+  }
+  T ap = a + s1;
+  u1 = ap * iD2;
+  lap = s2 * iD2 * iD2 - T(2) * S.b_ee * ap * iD2 * iD + T(2) * u1 * ir;
+}
+
+template <class T>
+__device__ __forceinline__ void uen_d(const DevSys<T>& S, T r, T ir, int I, T& u1, T& lap) {
+  T D = T(1) + S.b_en * r;
+  T iD = M<T>::rcp(D);
+  T iD2 = iD * iD;
+  T s1 = 0, s2 = 0;
+  if (S.k_en > 0) {
+    T s, y = r * iD;
+    series(S.k_en, S.d_en[S.species[I]], y, s, s1, s2);
+  }
+  T ap = -S.Z[I] + s1;
+  u1 = ap * iD2;
+  lap = s2 * iD2 * iD2 - T(2) * S.b_en * ap * iD2 * iD + T(2) * u1 * ir;
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Jordan inversion with partial pivoting, fully register resident
+// (row swaps by selects).  Returns ln|det| and sign; a singular matrix
+// (an exactly zero pivot) yields the identity, as methane.rs:829 does.
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ bool gj_invert(double a[N][N], double inv[N][N], double& logabs, double& sgn) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) inv[i][j] = (i == j) ? 1.0 : 0.0;
+  bool singular = false;
+  double la = 0.0, sg = 1.0;
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    int p = c;
+    double best = fabs(a[c][c]);
+#pragma unroll
+    for (int i = c + 1; i < N; ++i) {
+      double v = fabs(a[i][c]);
+      if (v > best) {
+        best = v;
+        p = i;
+      }
+    }
+    if (best == 0.0) singular = true;
+    if (p != c) sg = -sg;
+#pragma unroll
+    for (int i = c + 1; i < N; ++i) {
+      const bool sw = (p == i);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        double t = a[c][j], u = a[i][j];
+        a[c][j] = sw ? u : t;
+        a[i][j] = sw ? t : u;
+        double t2 = inv[c][j], u2 = inv[i][j];
+        inv[c][j] = sw ? u2 : t2;
+        inv[i][j] = sw ? t2 : u2;
+      }
+    }
+    const double pv = a[c][c];
+    la += log(fabs(pv));
+    if (pv < 0.0) sg = -sg;
+    const double ip = singular ? 0.0 : 1.0 / pv;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      a[c][j] *= ip;
+      inv[c][j] *= ip;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (i == c) continue;
+      const double f = a[i][c];
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        a[i][j] -= f * a[c][j];
+        inv[i][j] -= f * inv[c][j];
+      }
+    }
+  }
+  if (singular) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) inv[i][j] = (i == j) ? 1.0 : 0.0;
+    la = -INFINITY;
+    sg = 0.0;
+  }
+  logabs = la;
+  sgn = sg;
+  return singular;
+}
+
+// V_nn (methane.rs:908-924), computed on the host and carried in DevSys.
+
+}  // namespace qmc

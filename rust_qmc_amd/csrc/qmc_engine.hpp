@@ -1,0 +1,283 @@
+// qmc_engine.hpp -- the per-walker engine: one thread owns one walker whose
+// electron positions and per-spin inverse Slater matrices live in VGPRs
+// for the whole launch.  Spin sectors are walked with a cyclic rotation
+// of the sector's position rows and inverse columns, so the electron being
+// moved (or measured) always sits in slot 0 and every register array is
+// indexed with compile-time indices.
+#pragma once
+
+#include "qmc_device.hpp"
+
+namespace qmc {
+
+// rotate sector rows: slot k <- slot k+1, slot N-1 <- old slot 0
+template <int NA, class T>
+__device__ __forceinline__ void rotate(T ra[Cap<NA>::v][3], T ai[Cap<NA>::v][Cap<NA>::v]) {
+  if (NA <= 1) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    T t = ra[0][c];
+#pragma unroll
+    for (int k = 0; k + 1 < NA; ++k) ra[k][c] = ra[k + 1][c];
+    ra[NA - 1][c] = t;
+  }
+#pragma unroll
+  for (int m = 0; m < NA; ++m) {
+    T t = ai[m][0];
+#pragma unroll
+    for (int k = 0; k + 1 < NA; ++k) ai[m][k] = ai[m][k + 1];
+    ai[m][NA - 1] = t;
+  }
+}
+
+// Per-electron local-energy terms of the electron in slot 0 of sector A.
+// Returns the contribution to -1/2 sum lap(psi)/psi and adds the electron's
+// share of the potential (1/2 of each e-e pair) to pot.
+template <int NA, int NB, class T>
+__device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, const T ra[Cap<NA>::v][3],
+                                               const T ai[Cap<NA>::v][Cap<NA>::v],
+                                               const T rb[Cap<NB>::v][3], T& gx, T& gy, T& gz, T& lap,
+                                               double& pot) {
+  const T x = ra[0][0], y = ra[0][1], z = ra[0][2];
+  // determinant part: grad D / D and lap D / D from the inverse column
+  T g[Cap<NA>::v][3], l[Cap<NA>::v];
+  mo_grad_lap<NA>(S, spin, x, y, z, g, l);
+  T gdx = 0, gdy = 0, gdz = 0, ld = 0;
+#pragma unroll
+  for (int m = 0; m < NA; ++m) {
+    const T c = ai[m][0];
+    gdx += g[m][0] * c;
+    gdy += g[m][1] * c;
+    gdz += g[m][2] * c;
+    ld += l[m] * c;
+  }
+  // Jastrow part
+  T gux = 0, guy = 0, guz = 0, lu = 0, pairsq = 0;
+  double pe = 0.0;
+  auto ee = [&](T px, T py, T pz, bool same) {
+    T dx = x - px, dy = y - py, dz = z - pz;
+    T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+    T ir = M<T>::rcp(r);
+    T u1, lp;
+    uee_d(S, r, ir, same, u1, lp);
+    if (r >= T(1e-10)) {
+      T f = u1 * ir;
+      gux += f * dx;
+      guy += f * dy;
+      guz += f * dz;
+      lu += lp;
+      pairsq += u1 * u1;
+    }
+    pe += 0.5 * (double)ir;
+  };
+#pragma unroll
+  for (int j = 1; j < NA; ++j) ee(ra[j][0], ra[j][1], ra[j][2], true);
+#pragma unroll
+  for (int j = 0; j < NB; ++j) ee(rb[j][0], rb[j][1], rb[j][2], false);
+  for (int I = 0; I < S.n_nuc; ++I) {
+    T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+    T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+    T ir = M<T>::rcp(r);
+    pe -= (double)(S.Z[I] * ir);
+    if (S.en_on) {
+      T u1, lp;
+      uen_d(S, r, ir, I, u1, lp);
+      if (r >= T(1e-10)) {
+        T f = u1 * ir;
+        gux += f * dx;
+        guy += f * dy;
+        guz += f * dz;
+        lu += lp;
+      }
+    }
+  }
+  // Jastrow2's gradient carries an extra factor F (jastrow.rs:102) and its
+  // Laplacian omits the cross terms (jastrow.rs:110-125) under the flags.
+  const T gs = S.grad_scale;
+  T gjx = gs * gux, gjy = gs * guy, gjz = gs * guz;
+  T lj = (S.flags & F_J2_LAP_QUIRK) ? (lu + pairsq) : (lu + gux * gux + guy * guy + guz * guz);
+  gx = gdx + gjx;
+  gy = gdy + gjy;
+  gz = gdz + gjz;
+  lap = ld + lj + T(2) * (gdx * gjx + gdy * gjy + gdz * gjz);
+  pot += pe;
+}
+
+// Local energy E_L = -1/2 sum_i lap_i psi / psi + V, V = -sum Z/r + sum 1/r_ij + V_nn
+// (methane.rs:1154-1182).  Sectors are left in their canonical order.
+// When gout/lout are given, per-electron grad ln psi and lap psi / psi are
+// written to gout[(e*3+c)*ld], lout[e*ld] for global electron index e.
+template <int NU, int ND, class T>
+__device__ __forceinline__ double local_energy(const DevSys<T>& S, T ru[Cap<NU>::v][3],
+                                               T au[Cap<NU>::v][Cap<NU>::v], T rd[Cap<ND>::v][3],
+                                               T ad[Cap<ND>::v][Cap<ND>::v], double* gout, double* lout,
+                                               int64_t ld) {
+  double kin = 0.0, pot = 0.0;
+#pragma unroll 1
+  for (int k = 0; k < NU; ++k) {
+    T gx, gy, gz, lap;
+    electron_terms<NU, ND>(S, 0, ru, au, rd, gx, gy, gz, lap, pot);
+    kin += -0.5 * (double)lap;
+    if (gout) {
+      const int e = S.sector_elec[0][k];
+      gout[(e * 3 + 0) * ld] = gx;
+      gout[(e * 3 + 1) * ld] = gy;
+      gout[(e * 3 + 2) * ld] = gz;
+      lout[e * ld] = lap;
+    }
+    rotate<NU>(ru, au);
+  }
+#pragma unroll 1
+  for (int k = 0; k < ND; ++k) {
+    T gx, gy, gz, lap;
+    electron_terms<ND, NU>(S, 1, rd, ad, ru, gx, gy, gz, lap, pot);
+    kin += -0.5 * (double)lap;
+    if (gout) {
+      const int e = S.sector_elec[1][k];
+      gout[(e * 3 + 0) * ld] = gx;
+      gout[(e * 3 + 1) * ld] = gy;
+      gout[(e * 3 + 2) * ld] = gz;
+      lout[e * ld] = lap;
+    }
+    rotate<ND>(rd, ad);
+  }
+  return kin + pot + S.vnn;
+}
+
+// Full Jastrow exponent ln J (FP64), including the pair constant.
+template <int NU, int ND, class T>
+__device__ __forceinline__ double jastrow_u(const DevSys<double>& S, const T ru[Cap<NU>::v][3],
+                                            const T rd[Cap<ND>::v][3]) {
+  constexpr int N = NU + ND;
+  double r[N > 0 ? N : 1][3];
+#pragma unroll
+  for (int k = 0; k < NU; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[k][c] = (double)ru[k][c];
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[NU + k][c] = (double)rd[k][c];
+  double u = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) {
+      double dx = r[i][0] - r[j][0], dy = r[i][1] - r[j][1], dz = r[i][2] - r[j][2];
+      double rr = sqrt(dx * dx + dy * dy + dz * dz);
+      u += uee_val(S, rr, (i < NU) == (j < NU)) + S.cst_ee;
+    }
+  if (S.en_on)
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      for (int I = 0; I < S.n_nuc; ++I) {
+        double dx = r[i][0] - S.nuc[I][0], dy = r[i][1] - S.nuc[I][1], dz = r[i][2] - S.nuc[I][2];
+        u += uen_val(S, sqrt(dx * dx + dy * dy + dz * dz), I);
+      }
+  return u;
+}
+
+// Build one sector's Slater matrix in FP64 and invert it.
+template <int NA, class T>
+__device__ __forceinline__ bool sector_inverse(const DevSys<double>& S, int spin, const T ra[Cap<NA>::v][3],
+                                               T ai[Cap<NA>::v][Cap<NA>::v], double& logabs, double& sgn) {
+  if (NA == 0) {
+    logabs = 0.0;
+    sgn = 1.0;
+    return false;
+  }
+  constexpr int C = Cap<NA>::v;
+  double a[C][C], inv[C][C];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) {
+    double phi[C];
+    mo_values<C>(S, spin, (double)ra[k][0], (double)ra[k][1], (double)ra[k][2], phi);
+#pragma unroll
+    for (int m = 0; m < NA; ++m) a[k][m] = phi[m];
+  }
+  bool sing = gj_invert<C>(a, inv, logabs, sgn);
+#pragma unroll
+  for (int m = 0; m < NA; ++m)
+#pragma unroll
+    for (int k = 0; k < NA; ++k) ai[m][k] = (T)inv[m][k];
+  return sing;
+}
+
+// One sector's single-electron Metropolis moves (Gaussian proposal, ratio
+// from the inverse column, Jastrow difference from the moved electron's
+// pairs, Sherman-Morrison row update on acceptance).
+template <int NA, int NB, class T>
+__device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[Cap<NA>::v][3],
+                                             T ai[Cap<NA>::v][Cap<NA>::v], const T rb[Cap<NB>::v][3],
+                                             uint32_t w, uint32_t step, uint32_t k0, uint32_t k1, T sigma,
+                                             double& logdet, double& sgn, double& ju, int& acc) {
+#pragma unroll 1
+  for (int k = 0; k < NA; ++k) {
+    const uint32_t e = (uint32_t)S.sector_elec[spin][k];
+    T z[3];
+    Rng<T>::normals3(philox(w, step, 2u * e, RNG_SINGLE, k0, k1), z);
+    const T U = Rng<T>::uniform(philox(w, step, 2u * e + 1u, RNG_SINGLE, k0, k1).x);
+    const T ox = ra[0][0], oy = ra[0][1], oz = ra[0][2];
+    const T nx = ox + sigma * z[0], ny = oy + sigma * z[1], nz = oz + sigma * z[2];
+    T phi[Cap<NA>::v];
+    mo_values<NA>(S, spin, nx, ny, nz, phi);
+    T R = 0;
+#pragma unroll
+    for (int m = 0; m < NA; ++m) R += phi[m] * ai[m][0];
+    // Jastrow difference
+    T du = 0;
+    auto pair = [&](T px, T py, T pz, bool same) {
+      T ax = nx - px, ay = ny - py, az = nz - pz;
+      T bx = ox - px, by = oy - py, bz = oz - pz;
+      du += uee_val(S, M<T>::sq(ax * ax + ay * ay + az * az), same) -
+            uee_val(S, M<T>::sq(bx * bx + by * by + bz * bz), same);
+    };
+#pragma unroll
+    for (int j = 1; j < NA; ++j) pair(ra[j][0], ra[j][1], ra[j][2], true);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) pair(rb[j][0], rb[j][1], rb[j][2], false);
+    if (S.en_on)
+      for (int I = 0; I < S.n_nuc; ++I) {
+        T ax = nx - S.nuc[I][0], ay = ny - S.nuc[I][1], az = nz - S.nuc[I][2];
+        T bx = ox - S.nuc[I][0], by = oy - S.nuc[I][1], bz = oz - S.nuc[I][2];
+        du += uen_val(S, M<T>::sq(ax * ax + ay * ay + az * az), I) -
+              uen_val(S, M<T>::sq(bx * bx + by * by + bz * bz), I);
+      }
+    const T p = R * R * M<T>::ex(T(2) * du);
+    if (U < p) {
+      // Sherman-Morrison: A'^-1 = A^-1 - (A^-1 e_0)(phi^T A^-1 - e_0^T)/R
+      const T iR = M<T>::rcp(R);
+      T c[Cap<NA>::v];
+#pragma unroll
+      for (int m = 0; m < NA; ++m) c[m] = ai[m][0];
+#pragma unroll
+      for (int j = 1; j < NA; ++j) {
+        T wj = 0;
+#pragma unroll
+        for (int m = 0; m < NA; ++m) wj += phi[m] * ai[m][j];
+        const T f = wj * iR;
+#pragma unroll
+        for (int m = 0; m < NA; ++m) ai[m][j] -= c[m] * f;
+      }
+#pragma unroll
+      for (int m = 0; m < NA; ++m) ai[m][0] = c[m] * iR;
+      ra[0][0] = nx;
+      ra[0][1] = ny;
+      ra[0][2] = nz;
+      logdet += (double)M<T>::lg(R < T(0) ? -R : R);
+      if (R < T(0)) sgn = -sgn;
+      ju += (double)du;
+      ++acc;
+    }
+    rotate<NA>(ra, ai);
+  }
+}
+
+// wave-wide sum (deterministic butterfly order)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+}  // namespace qmc

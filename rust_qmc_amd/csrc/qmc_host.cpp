@@ -1,0 +1,1220 @@
+// qmc_host.cpp -- host side of rust_qmc_amd: the C ABI of include/qmc.h.
+// Builds systems (the reference's constructors and a generic description),
+// uploads them as DevSys parameter blocks, owns walker ensembles in HBM and
+// drives the kernels of qmc_kernels.hip.
+#include "qmc.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "qmc_device.hpp"
+#include "qmc_kernels.hpp"
+
+using namespace qmc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) return fail(QMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 on the host (for seeded system data); same stream as the
+// device and the published method (Salmon et al., SC'11).
+// ---------------------------------------------------------------------------
+void philox_host(const uint32_t ctr[4], uint64_t seed, uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c0 = n0;
+    c1 = (uint32_t)p1;
+    c2 = n2;
+    c3 = (uint32_t)p0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+double u01(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; }
+
+// ---------------------------------------------------------------------------
+// Host copy of a system description
+// ---------------------------------------------------------------------------
+struct HostSys {
+  std::vector<int32_t> spins;
+  std::vector<double> nuc;  // [M][3]
+  std::vector<double> Z;
+  std::vector<qmc_ao> aos;
+  std::vector<double> pexp, pcoef;
+  std::vector<int32_t> ppow;
+  int32_t n_mo = 0;
+  std::vector<double> mo_up, mo_dn;  // [n_mo][n_ao]
+  int32_t jas_kind = QMC_JAS_NONE;
+  double F = 1.0, b_ee = 0, b_en = 0, a_anti = 0, a_para = 0;
+  int32_t k_ee = 0, k_en = 0, n_species = 0;
+  std::vector<double> c_anti, c_para, d_en;  // d_en [n_species][k_en]
+  std::vector<int32_t> species;
+  std::vector<int32_t> init_nuc;
+  double init_sigma = 1.0;
+  uint32_t flags = 0;
+
+  int n_elec() const { return (int)spins.size(); }
+  int n_nuc() const { return (int)Z.size(); }
+  int n_ao() const { return (int)aos.size(); }
+  int n_up() const { return (int)std::count(spins.begin(), spins.end(), 1); }
+  int n_dn() const { return n_elec() - n_up(); }
+  int n_params() const { return jas_kind == QMC_JAS_PADE ? 4 + 2 * k_ee + n_species * k_en : 0; }
+};
+
+}  // namespace
+
+struct qmc_system {
+  HostSys h;
+  int32_t precision = QMC_FP64;
+  DevSys<double>* d64 = nullptr;  // device parameter blocks
+  DevSys<float>* d32 = nullptr;
+  const EngineOps* ops = nullptr;
+  bool dirty = true;
+  ~qmc_system() {
+    if (d64) (void)hipFree(d64);
+    if (d32) (void)hipFree(d32);
+  }
+};
+
+namespace {
+
+int validate(const HostSys& h) {
+  if (h.n_elec() < 1 || h.n_elec() > MAXE) return fail(QMC_ERR_INVALID, "n_elec out of range");
+  for (int s : h.spins)
+    if (s != 1 && s != -1) return fail(QMC_ERR_INVALID, "spins must be +1 or -1");
+  if (h.n_nuc() < 1 || h.n_nuc() > MAXN) return fail(QMC_ERR_INVALID, "n_nuc out of range");
+  if (h.n_ao() < 1 || h.n_ao() > MAXAO) return fail(QMC_ERR_INVALID, "n_ao out of range");
+  if (h.n_mo < std::max(h.n_up(), h.n_dn()) || h.n_mo > MAXMO)
+    return fail(QMC_ERR_INVALID, "n_mo must cover each spin sector and be <= 32");
+  for (const auto& a : h.aos) {
+    if (a.center < 0 || a.center >= h.n_nuc()) return fail(QMC_ERR_INVALID, "AO centre out of range");
+    if (a.nprim < 1 || a.prim0 < 0 || a.prim0 + a.nprim > (int)h.pexp.size())
+      return fail(QMC_ERR_INVALID, "AO primitive range out of bounds");
+    if (a.kind == QMC_AO_GTO && (a.l < 0 || a.l > 1 || (a.l == 1 && (a.m < 0 || a.m > 2))))
+      return fail(QMC_ERR_INVALID, "GTO angular momentum must be s or p");
+    if (a.kind != QMC_AO_GTO && a.kind != QMC_AO_STO && a.kind != QMC_AO_SLATER1S)
+      return fail(QMC_ERR_INVALID, "unknown AO kind");
+  }
+  if (h.k_ee < 0 || h.k_ee > MAXK || h.k_en < 0 || h.k_en > MAXK) return fail(QMC_ERR_INVALID, "series length > 8");
+  if (h.k_en > 0 && (h.n_species < 1 || h.n_species > MAXSPEC)) return fail(QMC_ERR_INVALID, "bad n_species");
+  if ((int)h.init_nuc.size() != h.n_elec()) return fail(QMC_ERR_INVALID, "init_nuc size");
+  for (int c : h.init_nuc)
+    if (c < 0 || c >= h.n_nuc()) return fail(QMC_ERR_INVALID, "init_nuc out of range");
+  return QMC_OK;
+}
+
+bool same_prims(const HostSys& h, const qmc_ao& a, const qmc_ao& b, bool coefs) {
+  if (a.nprim != b.nprim || a.center != b.center) return false;
+  for (int k = 0; k < a.nprim; ++k) {
+    if (h.pexp[a.prim0 + k] != h.pexp[b.prim0 + k]) return false;
+    if (coefs && h.pcoef[a.prim0 + k] != h.pcoef[b.prim0 + k]) return false;
+  }
+  return true;
+}
+
+bool is_p(const qmc_ao& a, int m) { return a.kind == QMC_AO_GTO && a.l == 1 && a.m == m; }
+
+template <class T>
+int fill_devsys(const HostSys& h, DevSys<T>& D) {
+  std::memset(&D, 0, sizeof(D));
+  D.n_elec = h.n_elec();
+  D.n_up = h.n_up();
+  D.n_dn = h.n_dn();
+  D.n_nuc = h.n_nuc();
+  D.n_ao = h.n_ao();
+  D.n_mo = h.n_mo;
+  D.flags = h.flags;
+  for (int I = 0; I < h.n_nuc(); ++I) {
+    for (int c = 0; c < 3; ++c) D.nuc[I][c] = (T)h.nuc[3 * I + c];
+    D.Z[I] = (T)h.Z[I];
+  }
+  double vnn = 0.0;
+  for (int I = 0; I < h.n_nuc(); ++I)
+    for (int J = I + 1; J < h.n_nuc(); ++J) {
+      double dx = h.nuc[3 * I] - h.nuc[3 * J], dy = h.nuc[3 * I + 1] - h.nuc[3 * J + 1],
+             dz = h.nuc[3 * I + 2] - h.nuc[3 * J + 2];
+      vnn += h.Z[I] * h.Z[J] / std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  D.vnn = vnn;
+  // Jastrow
+  D.jas_kind = h.jas_kind;
+  if (h.jas_kind == QMC_JAS_PAIR) {
+    D.a_anti = D.a_para = (T)0.5;
+    D.b_ee = (T)(1.0 / h.F);
+    D.cst_ee = (T)(-0.5 * h.F);
+    D.en_on = 0;
+    D.grad_scale = (T)((h.flags & QMC_FLAG_J2_GRAD_QUIRK) ? h.F : 1.0);
+  } else if (h.jas_kind == QMC_JAS_PADE) {
+    D.a_anti = (T)h.a_anti;
+    D.a_para = (T)h.a_para;
+    D.b_ee = (T)h.b_ee;
+    D.b_en = (T)h.b_en;
+    D.cst_ee = 0;
+    D.en_on = 1;
+    D.grad_scale = 1;
+    D.k_ee = h.k_ee;
+    D.k_en = h.k_en;
+    D.n_species = h.n_species;
+    for (int k = 0; k < h.k_ee; ++k) {
+      D.c_anti[k] = (T)h.c_anti[k];
+      D.c_para[k] = (T)h.c_para[k];
+    }
+    for (int s = 0; s < h.n_species; ++s)
+      for (int k = 0; k < h.k_en; ++k) D.d_en[s][k] = (T)h.d_en[s * h.k_en + k];
+    for (int I = 0; I < h.n_nuc(); ++I) D.species[I] = h.species.empty() ? 0 : h.species[I];
+  } else {
+    D.grad_scale = 1;
+  }
+  // the quirky Laplacian only exists for the pair Jastrow
+  if (h.jas_kind != QMC_JAS_PAIR) D.flags &= ~(uint32_t)QMC_FLAG_J2_LAP_QUIRK;
+  // shells: consecutive AOs sharing a centre and primitives are evaluated once
+  int nsh = 0, np = 0;
+  const int A = h.n_ao();
+  for (int i = 0; i < A;) {
+    if (nsh >= MAXSH) return fail(QMC_ERR_UNSUPPORTED, "too many shells");
+    const qmc_ao& a = h.aos[i];
+    int kind, nao, m = 0;
+    const qmc_ao* pa = nullptr;
+    if (a.kind != QMC_AO_GTO) {
+      kind = a.kind == QMC_AO_SLATER1S ? SH_SLATER1S : SH_STO;
+      nao = 1;
+    } else if (a.l == 0 && i + 3 < A && is_p(h.aos[i + 1], 0) && is_p(h.aos[i + 2], 1) && is_p(h.aos[i + 3], 2) &&
+               same_prims(h, a, h.aos[i + 1], false) && same_prims(h, h.aos[i + 1], h.aos[i + 2], true) &&
+               same_prims(h, h.aos[i + 1], h.aos[i + 3], true)) {
+      kind = SH_SP;
+      nao = 4;
+      pa = &h.aos[i + 1];
+    } else if (is_p(a, 0) && i + 2 < A && is_p(h.aos[i + 1], 1) && is_p(h.aos[i + 2], 2) &&
+               same_prims(h, a, h.aos[i + 1], true) && same_prims(h, a, h.aos[i + 2], true)) {
+      kind = SH_P;
+      nao = 3;
+      pa = &a;
+    } else if (a.l == 0) {
+      kind = SH_S;
+      nao = 1;
+    } else {
+      kind = SH_P1;
+      nao = 1;
+      m = a.m;
+      pa = &a;
+    }
+    if (np + a.nprim > MAXPRIM) return fail(QMC_ERR_UNSUPPORTED, "too many primitives");
+    D.sh_kind[nsh] = kind;
+    D.sh_center[nsh] = a.center;
+    D.sh_nprim[nsh] = a.nprim;
+    D.sh_prim0[nsh] = np;
+    D.sh_ao0[nsh] = i;
+    D.sh_m[nsh] = m;
+    for (int k = 0; k < a.nprim; ++k) {
+      D.prim_exp[np + k] = (T)h.pexp[a.prim0 + k];
+      const bool s_part = (kind == SH_S || kind == SH_SP || kind == SH_STO || kind == SH_SLATER1S);
+      D.coef_s[np + k] = s_part ? (T)h.pcoef[a.prim0 + k] : (T)0;
+      D.coef_p[np + k] = pa ? (T)h.pcoef[pa->prim0 + k] : (T)0;
+      D.prim_pow[np + k] = (a.kind == QMC_AO_GTO || h.ppow.empty()) ? 0 : h.ppow[a.prim0 + k];
+    }
+    np += a.nprim;
+    ++nsh;
+    i += nao;
+  }
+  D.n_shell = nsh;
+  for (int mo = 0; mo < h.n_mo; ++mo)
+    for (int ao = 0; ao < A; ++ao) {
+      D.moT[0][ao][mo] = (T)h.mo_up[mo * A + ao];
+      D.moT[1][ao][mo] = (T)h.mo_dn[mo * A + ao];
+    }
+  int ku = 0, kd = 0;
+  for (int e = 0; e < h.n_elec(); ++e) {
+    if (h.spins[e] == 1)
+      D.sector_elec[0][ku++] = e;
+    else
+      D.sector_elec[1][kd++] = e;
+    D.init_nuc[e] = h.init_nuc[e];
+  }
+  D.init_sigma = (T)h.init_sigma;
+  return QMC_OK;
+}
+
+int upload(qmc_system* s) {
+  if (!s->dirty) return QMC_OK;
+  int rc = validate(s->h);
+  if (rc) return rc;
+  auto d64 = std::make_unique<DevSys<double>>();
+  auto d32 = std::make_unique<DevSys<float>>();
+  if ((rc = fill_devsys(s->h, *d64))) return rc;
+  if ((rc = fill_devsys(s->h, *d32))) return rc;
+  if (!s->d64) HIP_TRY(hipMalloc(&s->d64, sizeof(DevSys<double>)));
+  if (!s->d32) HIP_TRY(hipMalloc(&s->d32, sizeof(DevSys<float>)));
+  HIP_TRY(hipMemcpy(s->d64, d64.get(), sizeof(DevSys<double>), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d32, d32.get(), sizeof(DevSys<float>), hipMemcpyHostToDevice));
+  s->ops = find_engine(s->h.n_up(), s->h.n_dn());
+  s->dirty = false;
+  return QMC_OK;
+}
+
+int need_engine(const qmc_system* s) {
+  if (!s->ops)
+    return fail(QMC_ERR_UNSUPPORTED, "no engine compiled for " + std::to_string(s->h.n_up()) + " up / " +
+                                         std::to_string(s->h.n_dn()) + " down electrons");
+  return QMC_OK;
+}
+
+int make_system(HostSys&& h, int32_t precision, qmc_system** out) {
+  if (!out) return fail(QMC_ERR_INVALID, "out is null");
+  if (precision != QMC_FP64 && precision != QMC_MIXED) return fail(QMC_ERR_INVALID, "bad precision");
+  auto s = std::make_unique<qmc_system>();
+  s->h = std::move(h);
+  s->precision = precision;
+  int rc = upload(s.get());
+  if (rc) return rc;
+  *out = s.release();
+  return QMC_OK;
+}
+
+// --- basis data -------------------------------------------------------------
+// 6-31G as carried by the reference (methane.rs:701-770, benzene.rs:161-226)
+const double C631_1S_E[6] = {3047.5249, 457.36951, 103.94869, 29.210155, 9.2866630, 3.1639270};
+const double C631_1S_C[6] = {0.0018347, 0.0140373, 0.0688426, 0.2321844, 0.4679413, 0.3623120};
+const double C631_SP_E[3] = {7.8682724, 1.8812885, 0.5442493};
+const double C631_SP_S[3] = {-0.1193324, -0.1608542, 1.1434564};
+const double C631_SP_P[3] = {0.0689991, 0.3164240, 0.7443083};
+const double C631_OUT_E = 0.1687145;
+const double H631_IN_E[3] = {18.7311370, 2.8253937, 0.6401217};
+const double H631_IN_C[3] = {0.0334946, 0.2347270, 0.8137573};
+const double H631_OUT_E = 0.1612778;
+// STO-3G, Hehre, Stewart and Pople, J. Chem. Phys. 51, 2657 (1969)
+const double STO3G_1S_C[3] = {0.15432897, 0.53532814, 0.44463454};
+const double H_STO3G_E[3] = {3.42525091, 0.62391373, 0.16885540};
+const double C_STO3G_1S_E[3] = {71.6168370, 13.0450960, 3.5305122};
+const double C_STO3G_SP_E[3] = {2.9412494, 0.6834831, 0.2222899};
+const double C_STO3G_SP_S[3] = {-0.09996723, 0.39951283, 0.70011547};
+const double C_STO3G_SP_P[3] = {0.15591627, 0.60768372, 0.39195739};
+
+void add_ao(HostSys& h, int kind, int center, int l, int m, int np, const double* e, const double* c,
+            const int32_t* pw = nullptr) {
+  qmc_ao a;
+  a.kind = kind;
+  a.center = center;
+  a.l = l;
+  a.m = m;
+  a.nprim = np;
+  a.prim0 = (int32_t)h.pexp.size();
+  for (int k = 0; k < np; ++k) {
+    h.pexp.push_back(e[k]);
+    h.pcoef.push_back(c[k]);
+    h.ppow.push_back(pw ? pw[k] : 0);
+  }
+  h.aos.push_back(a);
+}
+
+void add_c631(HostSys& h, int nuc) {
+  const double one = 1.0;
+  add_ao(h, QMC_AO_GTO, nuc, 0, 0, 6, C631_1S_E, C631_1S_C);
+  add_ao(h, QMC_AO_GTO, nuc, 0, 0, 3, C631_SP_E, C631_SP_S);
+  for (int m = 0; m < 3; ++m) add_ao(h, QMC_AO_GTO, nuc, 1, m, 3, C631_SP_E, C631_SP_P);
+  add_ao(h, QMC_AO_GTO, nuc, 0, 0, 1, &C631_OUT_E, &one);
+  for (int m = 0; m < 3; ++m) add_ao(h, QMC_AO_GTO, nuc, 1, m, 1, &C631_OUT_E, &one);
+}
+
+// Jastrow3 defaults a_anti = 1/2, a_para = 1/4 (jastrow.rs:177-178); the e-n
+// series species are the distinct charges in order of first appearance.
+void jastrow3(HostSys& h, double b_ee, double b_en) {
+  h.jas_kind = QMC_JAS_PADE;
+  h.a_anti = 0.5;
+  h.a_para = 0.25;
+  h.b_ee = b_ee;
+  h.b_en = b_en;
+  std::vector<double> seen;
+  h.species.assign(h.n_nuc(), 0);
+  for (int I = 0; I < h.n_nuc(); ++I) {
+    auto it = std::find(seen.begin(), seen.end(), h.Z[I]);
+    if (it == seen.end()) {
+      if ((int)seen.size() < MAXSPEC) seen.push_back(h.Z[I]);
+      it = seen.end() - 1;
+    }
+    h.species[I] = (int32_t)(it - seen.begin());
+  }
+  h.n_species = (int32_t)seen.size();
+}
+
+void methane_geometry(HostSys& h, double ch) {
+  const double d = ch / std::sqrt(3.0);  // methane.rs:625-632
+  const double H[4][3] = {{d, d, d}, {d, -d, -d}, {-d, d, -d}, {-d, -d, d}};
+  h.nuc = {0.0, 0.0, 0.0};
+  h.Z = {6.0};
+  for (int k = 0; k < 4; ++k) {
+    h.nuc.insert(h.nuc.end(), H[k], H[k] + 3);
+    h.Z.push_back(1.0);
+  }
+  h.spins = {1, 1, 1, 1, 1, -1, -1, -1, -1, -1};  // methane.rs:640
+  h.init_nuc.resize(10);
+  for (int e = 0; e < 10; ++e) h.init_nuc[e] = e < 6 ? 0 : e - 5;  // methane.rs:1110-1119
+  h.init_sigma = 0.5;
+}
+
+// Seeded orthonormal MO rows: C = 2U-1 from Philox counter (mo*n_ao+ao, 0,
+// 0, SYS), then modified Gram-Schmidt over rows (Euclidean metric).
+void random_mo(uint64_t seed, int n_mo, int n_ao, std::vector<double>& mo) {
+  mo.assign((size_t)n_mo * n_ao, 0.0);
+  for (int m = 0; m < n_mo; ++m)
+    for (int a = 0; a < n_ao; ++a) {
+      uint32_t ctr[4] = {(uint32_t)(m * n_ao + a), 0u, 0u, RNG_SYS}, w[4];
+      philox_host(ctr, seed, w);
+      mo[m * n_ao + a] = 2.0 * u01(w[0]) - 1.0;
+    }
+  for (int m = 0; m < n_mo; ++m) {
+    for (int q = 0; q < m; ++q) {
+      double p = 0.0;
+      for (int a = 0; a < n_ao; ++a) p += mo[m * n_ao + a] * mo[q * n_ao + a];
+      for (int a = 0; a < n_ao; ++a) mo[m * n_ao + a] -= p * mo[q * n_ao + a];
+    }
+    double nn = 0.0;
+    for (int a = 0; a < n_ao; ++a) nn += mo[m * n_ao + a] * mo[m * n_ao + a];
+    const double inv = 1.0 / std::sqrt(nn);
+    for (int a = 0; a < n_ao; ++a) mo[m * n_ao + a] *= inv;
+  }
+}
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+// Device view of a caller buffer: the buffer itself when it is device
+// memory, otherwise a staging allocation copied in (in=true) and/or out.
+struct Staged {
+  void* user = nullptr;
+  void* dev = nullptr;
+  size_t bytes = 0;
+  bool owned = false;
+  hipStream_t st = nullptr;
+  ~Staged() {
+    if (owned && dev) (void)hipFree(dev);
+  }
+  int init(const void* p, size_t nbytes, bool copy_in, hipStream_t s) {
+    user = const_cast<void*>(p);
+    bytes = nbytes;
+    st = s;
+    if (!p) return QMC_OK;
+    if (is_device_ptr(p)) {
+      dev = user;
+      return QMC_OK;
+    }
+    HIP_TRY(hipMalloc(&dev, std::max<size_t>(nbytes, 8)));
+    owned = true;
+    if (copy_in) HIP_TRY(hipMemcpyAsync(dev, p, nbytes, hipMemcpyHostToDevice, s));
+    return QMC_OK;
+  }
+  int copy_out() {
+    if (owned && user) HIP_TRY(hipMemcpyAsync(user, dev, bytes, hipMemcpyDeviceToHost, st));
+    return QMC_OK;
+  }
+};
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* qmc_last_error(void) { return g_err.c_str(); }
+const char* qmc_version(void) { return "rust_qmc_amd 0.1 (gfx950)"; }
+
+int qmc_system_create(const qmc_system_desc* d, int32_t precision, qmc_system** out) {
+  if (!d || !d->spins || !d->nuc_pos || !d->nuc_charge || !d->aos || !d->prim_exp || !d->prim_coef || !d->mo_up)
+    return fail(QMC_ERR_INVALID, "null field in qmc_system_desc");
+  if (d->n_elec < 1 || d->n_elec > MAXE || d->n_nuc < 1 || d->n_nuc > MAXN || d->n_ao < 1 || d->n_ao > MAXAO ||
+      d->n_mo < 1 || d->n_mo > MAXMO)
+    return fail(QMC_ERR_INVALID, "size out of range in qmc_system_desc");
+  HostSys h;
+  h.spins.assign(d->spins, d->spins + d->n_elec);
+  h.nuc.assign(d->nuc_pos, d->nuc_pos + 3 * d->n_nuc);
+  h.Z.assign(d->nuc_charge, d->nuc_charge + d->n_nuc);
+  h.aos.assign(d->aos, d->aos + d->n_ao);
+  int nprim = 0;
+  for (int i = 0; i < d->n_ao; ++i) nprim = std::max(nprim, d->aos[i].prim0 + d->aos[i].nprim);
+  if (nprim > MAXPRIM) return fail(QMC_ERR_INVALID, "too many primitives");
+  h.pexp.assign(d->prim_exp, d->prim_exp + nprim);
+  h.pcoef.assign(d->prim_coef, d->prim_coef + nprim);
+  if (d->prim_pow)
+    h.ppow.assign(d->prim_pow, d->prim_pow + nprim);
+  else
+    h.ppow.assign(nprim, 0);
+  h.n_mo = d->n_mo;
+  h.mo_up.assign(d->mo_up, d->mo_up + (size_t)d->n_mo * d->n_ao);
+  const double* dn = d->mo_dn ? d->mo_dn : d->mo_up;
+  h.mo_dn.assign(dn, dn + (size_t)d->n_mo * d->n_ao);
+  h.jas_kind = d->jastrow_kind;
+  h.F = d->jas_F;
+  h.b_ee = d->b_ee;
+  h.b_en = d->b_en;
+  h.a_anti = d->a_anti;
+  h.a_para = d->a_para;
+  h.k_ee = d->k_ee;
+  h.k_en = d->k_en;
+  h.n_species = d->n_species;
+  if (h.k_ee > 0) {
+    if (!d->c_anti || !d->c_para) return fail(QMC_ERR_INVALID, "series coefficients missing");
+    h.c_anti.assign(d->c_anti, d->c_anti + h.k_ee);
+    h.c_para.assign(d->c_para, d->c_para + h.k_ee);
+  }
+  if (h.k_en > 0) {
+    if (!d->d_en || !d->species) return fail(QMC_ERR_INVALID, "e-n series coefficients missing");
+    h.d_en.assign(d->d_en, d->d_en + h.n_species * h.k_en);
+  }
+  if (d->species) h.species.assign(d->species, d->species + d->n_nuc);
+  if (d->init_nuc)
+    h.init_nuc.assign(d->init_nuc, d->init_nuc + d->n_elec);
+  else
+    h.init_nuc.assign(d->n_elec, 0);
+  h.init_sigma = d->init_sigma > 0 ? d->init_sigma : 1.0;
+  h.flags = d->flags;
+  return make_system(std::move(h), precision, out);
+}
+
+int qmc_system_destroy(qmc_system* sys) {
+  delete sys;
+  return QMC_OK;
+}
+
+int qmc_h2_molecule_mo_create(double alpha, const double c1[3], const double c2[3], double F, double init_sigma,
+                              int32_t precision, qmc_system** out) {
+  if (!c1 || !c2) return fail(QMC_ERR_INVALID, "null centre");
+  HostSys h;
+  h.spins = {1, -1};
+  h.nuc = {c1[0], c1[1], c1[2], c2[0], c2[1], c2[2]};
+  h.Z = {1.0, 1.0};
+  const double one = 1.0;
+  add_ao(h, QMC_AO_SLATER1S, 0, 0, 0, 1, &alpha, &one);
+  add_ao(h, QMC_AO_SLATER1S, 1, 0, 0, 1, &alpha, &one);
+  // psi = (phi1 + phi2)(r1) (phi1 + phi2)(r2) J (hydrogen.rs:120-124)
+  h.n_mo = 1;
+  h.mo_up = {1.0, 1.0};
+  h.mo_dn = {1.0, 1.0};
+  h.jas_kind = QMC_JAS_PAIR;
+  h.F = F;
+  h.init_nuc = {0, 1};
+  h.init_sigma = init_sigma;
+  h.flags = 0;  // Jastrow1 is exact (jastrow.rs:35-53)
+  return make_system(std::move(h), precision, out);
+}
+
+int qmc_sto_slater_det_create(int32_t n, const int32_t* nprim, const double* coef, const int32_t* pw,
+                              const double* zeta, const int32_t* spins, double F, double Z, uint32_t flags,
+                              int32_t precision, qmc_system** out) {
+  if (n < 1 || n > MAXE || !nprim || !coef || !pw || !zeta || !spins) return fail(QMC_ERR_INVALID, "bad arguments");
+  HostSys h;
+  h.spins.assign(spins, spins + n);
+  h.nuc = {0.0, 0.0, 0.0};
+  h.Z = {Z};
+  // STOSlaterDet (slater.rs:205-216): column j holds orbital j and couples
+  // only to electrons of spin_j, so the spin-up determinant uses the
+  // orbitals of the spin-up columns in order, likewise spin down.
+  int off = 0;
+  std::vector<int> up_orb, dn_orb;
+  for (int o = 0; o < n; ++o) {
+    add_ao(h, QMC_AO_STO, 0, 0, 0, nprim[o], zeta + off, coef + off, pw + off);
+    off += nprim[o];
+    (spins[o] == 1 ? up_orb : dn_orb).push_back(o);
+  }
+  h.n_mo = std::max((int)up_orb.size(), (int)dn_orb.size());
+  h.mo_up.assign((size_t)h.n_mo * n, 0.0);
+  h.mo_dn.assign((size_t)h.n_mo * n, 0.0);
+  for (size_t k = 0; k < up_orb.size(); ++k) h.mo_up[k * n + up_orb[k]] = 1.0;
+  for (size_t k = 0; k < dn_orb.size(); ++k) h.mo_dn[k * n + dn_orb[k]] = 1.0;
+  h.jas_kind = QMC_JAS_PAIR;
+  h.F = F;
+  h.init_nuc.assign(n, 0);
+  h.init_sigma = 1.0;  // slater.rs:198-204
+  h.flags = flags;
+  return make_system(std::move(h), precision, out);
+}
+
+int qmc_lithium_seeded_create(uint64_t seed, uint32_t flags, int32_t precision, qmc_system** out) {
+  const double xi[2] = {2.7, 0.65};
+  double coef[2][3];
+  for (int o = 0; o < 2; ++o) {
+    double c[3];
+    for (int nu = 0; nu < 3; ++nu) {
+      uint32_t ctr[4] = {(uint32_t)(3 * o + nu), 1u, 0u, RNG_SYS}, w[4];
+      philox_host(ctr, seed, w);
+      c[nu] = 2.0 * u01(w[0]) - 1.0;
+    }
+    // int |psi|^2 d^3r = sum_ab c_a c_b 4 pi (a+b+2)! / (2 xi)^(a+b+3)
+    double norm = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        double f = 1.0;
+        for (int i = 2; i <= a + b + 2; ++i) f *= i;
+        norm += c[a] * c[b] * 4.0 * M_PI * f / std::pow(2.0 * xi[o], a + b + 3);
+      }
+    const double A = 1.0 / std::sqrt(norm);
+    for (int nu = 0; nu < 3; ++nu) coef[o][nu] = A * c[nu];
+  }
+  const int32_t nprim[3] = {3, 3, 3};
+  const double cf[9] = {coef[0][0], coef[0][1], coef[0][2], coef[0][0], coef[0][1], coef[0][2],
+                        coef[1][0], coef[1][1], coef[1][2]};
+  const int32_t pw[9] = {0, 1, 2, 0, 1, 2, 0, 1, 2};
+  const double zt[9] = {xi[0], xi[0], xi[0], xi[0], xi[0], xi[0], xi[1], xi[1], xi[1]};
+  const int32_t spins[3] = {1, -1, 1};
+  return qmc_sto_slater_det_create(3, nprim, cf, pw, zt, spins, 1.0, 3.0, flags, precision, out);
+}
+
+int qmc_methane_gto_create(double b_ee, double b_en, double ch_bond, int32_t basis, uint64_t mo_seed,
+                           int32_t precision, qmc_system** out) {
+  HostSys h;
+  methane_geometry(h, ch_bond);
+  const double one = 1.0;
+  if (basis == QMC_BASIS_631G) {
+    // methane.rs:697-776 and the MO coefficients of methane.rs:662-674
+    add_c631(h, 0);
+    for (int k = 0; k < 4; ++k) add_ao(h, QMC_AO_GTO, 1 + k, 0, 0, 3, H631_IN_E, H631_IN_C);
+    for (int k = 0; k < 4; ++k) add_ao(h, QMC_AO_GTO, 1 + k, 0, 0, 1, &H631_OUT_E, &one);
+    static const double MO[5][17] = {
+        {0.9943, 0.0234, 0.0, 0.0, 0.0, 0.0020, 0.0, 0.0, 0.0, 0.0018, 0.0018, 0.0018, 0.0018, 0.0015, 0.0015, 0.0015, 0.0015},
+        {-0.0237, 0.2223, 0.0, 0.0, 0.0, 0.8340, 0.0, 0.0, 0.0, 0.0673, 0.0673, 0.0673, 0.0673, 0.0437, 0.0437, 0.0437, 0.0437},
+        {0.0, 0.0, 0.2090, 0.0, 0.0, 0.0, 0.5730, 0.0, 0.0, 0.1020, 0.1020, -0.1020, -0.1020, 0.0670, 0.0670, -0.0670, -0.0670},
+        {0.0, 0.0, 0.0, 0.2090, 0.0, 0.0, 0.0, 0.5730, 0.0, 0.1020, -0.1020, 0.1020, -0.1020, 0.0670, -0.0670, 0.0670, -0.0670},
+        {0.0, 0.0, 0.0, 0.0, 0.2090, 0.0, 0.0, 0.0, 0.5730, 0.1020, -0.1020, -0.1020, 0.1020, 0.0670, -0.0670, -0.0670, 0.0670},
+    };
+    h.n_mo = 5;
+    h.mo_up.assign(&MO[0][0], &MO[0][0] + 5 * 17);
+  } else if (basis == QMC_BASIS_STO3G) {
+    add_ao(h, QMC_AO_GTO, 0, 0, 0, 3, C_STO3G_1S_E, STO3G_1S_C);
+    add_ao(h, QMC_AO_GTO, 0, 0, 0, 3, C_STO3G_SP_E, C_STO3G_SP_S);
+    for (int m = 0; m < 3; ++m) add_ao(h, QMC_AO_GTO, 0, 1, m, 3, C_STO3G_SP_E, C_STO3G_SP_P);
+    for (int k = 0; k < 4; ++k) add_ao(h, QMC_AO_GTO, 1 + k, 0, 0, 3, H_STO3G_E, STO3G_1S_C);
+    h.n_mo = 5;
+    random_mo(mo_seed, 5, 9, h.mo_up);
+  } else {
+    return fail(QMC_ERR_INVALID, "unknown basis");
+  }
+  h.mo_dn = h.mo_up;  // both spins share the MOs (methane.rs:824)
+  jastrow3(h, b_ee, b_en);
+  return make_system(std::move(h), precision, out);
+}
+
+int qmc_benzene_gto_create(double b_ee, double b_en, double cc, double ch, int32_t precision, qmc_system** out) {
+  HostSys h;
+  // geometry, benzene.rs:119-140: C at (cc cos, cc sin, 0), H radially beyond
+  for (int i = 0; i < 6; ++i) {
+    const double ang = M_PI / 3.0 * i;
+    h.nuc.insert(h.nuc.end(), {cc * std::cos(ang), cc * std::sin(ang), 0.0});
+    h.Z.push_back(6.0);
+  }
+  for (int i = 0; i < 6; ++i) {
+    const double ang = M_PI / 3.0 * i, rh = cc + ch;
+    h.nuc.insert(h.nuc.end(), {rh * std::cos(ang), rh * std::sin(ang), 0.0});
+    h.Z.push_back(1.0);
+  }
+  h.spins.assign(42, 1);
+  for (int e = 21; e < 42; ++e) h.spins[e] = -1;
+  h.init_nuc.resize(42);
+  for (int e = 0; e < 42; ++e) h.init_nuc[e] = e < 36 ? e / 6 : 6 + (e - 36);  // benzene.rs:642-655
+  h.init_sigma = 0.5;
+  for (int c = 0; c < 6; ++c) add_c631(h, c);
+  const double one = 1.0;
+  for (int k = 0; k < 6; ++k) {
+    add_ao(h, QMC_AO_GTO, 6 + k, 0, 0, 3, H631_IN_E, H631_IN_C);
+    add_ao(h, QMC_AO_GTO, 6 + k, 0, 0, 1, &H631_OUT_E, &one);
+  }
+  // MO coefficients, benzene.rs:262-470
+  const int NA = 66;
+  std::vector<double> mo(21 * NA, 0.0);
+  auto C = [&](int m, int n, int k) -> double& { return mo[m * NA + 9 * n + k]; };
+  auto Hh = [&](int m, int n, int k) -> double& { return mo[m * NA + 54 + 2 * n + k]; };
+  int m = 0;
+  for (int n = 0; n < 6; ++n, ++m) {
+    C(m, n, 0) = 0.9943;
+    C(m, n, 1) = 0.0234;
+    C(m, n, 5) = 0.0020;
+  }
+  const double s6 = 1.0 / std::sqrt(6.0), s3 = std::sqrt(3.0);
+  for (int n = 0; n < 6; ++n) {
+    C(m, n, 1) = 0.20 * s6;
+    C(m, n, 5) = 0.80 * s6;
+  }
+  ++m;
+  for (int pass = 0; pass < 2; ++pass, ++m)
+    for (int n = 0; n < 6; ++n) {
+      const double ang = M_PI / 3.0 * n;
+      const double w = (pass == 0 ? std::cos(ang) : std::sin(ang)) / s3;
+      C(m, n, 1) = 0.20 * w;
+      C(m, n, 5) = 0.80 * w;
+      C(m, n, pass == 0 ? 2 : 3) = 0.50 * w;
+      C(m, n, pass == 0 ? 6 : 7) = 0.55 * w;
+    }
+  for (int n = 0; n < 6; ++n, ++m) {
+    double dir[3];
+    for (int c = 0; c < 3; ++c) dir[c] = h.nuc[3 * (6 + n) + c] - h.nuc[3 * n + c];
+    const double dn = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    for (double& v : dir) v /= dn;
+    C(m, n, 1) = 0.15;
+    C(m, n, 5) = 0.30;
+    for (int c = 0; c < 3; ++c) {
+      C(m, n, 2 + c) = 0.20 * dir[c];
+      C(m, n, 6 + c) = 0.55 * dir[c];
+    }
+    Hh(m, n, 0) = 0.25;
+    Hh(m, n, 1) = 0.15;
+  }
+  for (int n = 0; n < 6; ++n) {
+    C(m, n, 4) = 0.30 * s6;
+    C(m, n, 8) = 0.70 * s6;
+  }
+  ++m;
+  for (int pass = 0; pass < 2; ++pass, ++m)
+    for (int n = 0; n < 6; ++n) {
+      const double ang = M_PI / 3.0 * n;
+      const double ph = (pass == 0 ? std::cos(ang) : std::sin(ang)) / s3;
+      C(m, n, 4) = 0.30 * ph;
+      C(m, n, 8) = 0.70 * ph;
+    }
+  for (int n = 0; n < 6; ++n) {
+    const double ang = M_PI / 3.0 * n, tx = -std::sin(ang), ty = std::cos(ang);
+    C(m, n, 2) = 0.25 * tx * s6;
+    C(m, n, 3) = 0.25 * ty * s6;
+    C(m, n, 6) = 0.60 * tx * s6;
+    C(m, n, 7) = 0.60 * ty * s6;
+  }
+  ++m;
+  for (int pass = 0; pass < 2; ++pass, ++m)
+    for (int n = 0; n < 6; ++n) {
+      const double ang = M_PI / 3.0 * n, tx = -std::sin(ang), ty = std::cos(ang);
+      const double ph = (pass == 0 ? std::cos(2.0 * ang) : std::sin(2.0 * ang)) / s3;
+      C(m, n, 2) = 0.25 * tx * ph;
+      C(m, n, 3) = 0.25 * ty * ph;
+      C(m, n, 6) = 0.60 * tx * ph;
+      C(m, n, 7) = 0.60 * ty * ph;
+    }
+  h.n_mo = 21;
+  h.mo_up = mo;
+  h.mo_dn = mo;
+  jastrow3(h, b_ee, b_en);
+  return make_system(std::move(h), precision, out);
+}
+
+int qmc_set_precision(qmc_system* s, int32_t precision) {
+  if (!s || (precision != QMC_FP64 && precision != QMC_MIXED)) return fail(QMC_ERR_INVALID, "bad precision");
+  s->precision = precision;
+  return QMC_OK;
+}
+
+int qmc_set_flags(qmc_system* s, uint32_t flags) {
+  if (!s) return fail(QMC_ERR_INVALID, "null system");
+  s->h.flags = flags;
+  s->dirty = true;
+  return upload(s);
+}
+
+int qmc_system_info(const qmc_system* s, int32_t* n_elec, int32_t* n_up, int32_t* n_nuc, int32_t* n_ao) {
+  if (!s) return fail(QMC_ERR_INVALID, "null system");
+  if (n_elec) *n_elec = s->h.n_elec();
+  if (n_up) *n_up = s->h.n_up();
+  if (n_nuc) *n_nuc = s->h.n_nuc();
+  if (n_ao) *n_ao = s->h.n_ao();
+  return QMC_OK;
+}
+
+int qmc_get_mo(const qmc_system* s, int32_t spin, double* mo) {
+  if (!s || !mo || (spin != 0 && spin != 1)) return fail(QMC_ERR_INVALID, "bad arguments");
+  const auto& v = spin == 0 ? s->h.mo_up : s->h.mo_dn;
+  std::copy(v.begin(), v.end(), mo);
+  return QMC_OK;
+}
+
+int qmc_nuclear_repulsion(const qmc_system* s, double* vnn) {
+  if (!s || !vnn) return fail(QMC_ERR_INVALID, "bad arguments");
+  DevSys<double>* D = new DevSys<double>();
+  int rc = fill_devsys(s->h, *D);
+  *vnn = D->vnn;
+  delete D;
+  return rc;
+}
+
+int qmc_num_params(const qmc_system* s, int32_t* n) {
+  if (!s || !n) return fail(QMC_ERR_INVALID, "bad arguments");
+  *n = s->h.n_params();
+  return QMC_OK;
+}
+
+int qmc_get_params(const qmc_system* s, double* p) {
+  if (!s || !p) return fail(QMC_ERR_INVALID, "bad arguments");
+  const HostSys& h = s->h;
+  if (h.jas_kind != QMC_JAS_PADE) return QMC_OK;
+  int q = 0;
+  p[q++] = h.b_ee;
+  p[q++] = h.b_en;
+  p[q++] = h.a_anti;
+  p[q++] = h.a_para;
+  for (int k = 0; k < h.k_ee; ++k) p[q++] = h.c_anti[k];
+  for (int k = 0; k < h.k_ee; ++k) p[q++] = h.c_para[k];
+  for (int k = 0; k < h.n_species * h.k_en; ++k) p[q++] = h.d_en[k];
+  return QMC_OK;
+}
+
+int qmc_set_params(qmc_system* s, const double* p, int32_t n) {
+  if (!s || !p) return fail(QMC_ERR_INVALID, "bad arguments");
+  HostSys& h = s->h;
+  // methane.rs:1200-1206 asserts the length; here it is an error code
+  if (n != h.n_params()) return fail(QMC_ERR_INVALID, "parameter count mismatch");
+  int q = 0;
+  h.b_ee = p[q++];
+  h.b_en = p[q++];
+  h.a_anti = p[q++];
+  h.a_para = p[q++];
+  for (int k = 0; k < h.k_ee; ++k) h.c_anti[k] = p[q++];
+  for (int k = 0; k < h.k_ee; ++k) h.c_para[k] = p[q++];
+  for (int k = 0; k < h.n_species * h.k_en; ++k) h.d_en[k] = p[q++];
+  s->dirty = true;
+  return upload(s);
+}
+
+int qmc_set_jastrow_series(qmc_system* s, int32_t k_ee, int32_t k_en) {
+  if (!s || k_ee < 0 || k_ee > MAXK || k_en < 0 || k_en > MAXK) return fail(QMC_ERR_INVALID, "bad arguments");
+  HostSys& h = s->h;
+  if (h.jas_kind != QMC_JAS_PADE) return fail(QMC_ERR_INVALID, "series needs the Pade Jastrow");
+  h.k_ee = k_ee;
+  h.k_en = k_en;
+  h.c_anti.assign(k_ee, 0.0);
+  h.c_para.assign(k_ee, 0.0);
+  h.d_en.assign((size_t)h.n_species * k_en, 0.0);
+  s->dirty = true;
+  return upload(s);
+}
+
+int qmc_get_nuclei(const qmc_system* s, double* R, double* Z) {
+  if (!s) return fail(QMC_ERR_INVALID, "null system");
+  if (R) std::copy(s->h.nuc.begin(), s->h.nuc.end(), R);
+  if (Z) std::copy(s->h.Z.begin(), s->h.Z.end(), Z);
+  return QMC_OK;
+}
+
+int qmc_set_nuclei(qmc_system* s, const double* R, int32_t n) {
+  if (!s || !R) return fail(QMC_ERR_INVALID, "bad arguments");
+  // methane.rs:965-978 asserts the count and rebuilds basis centres; AO
+  // centres here are nucleus indices, so only the coordinates change.
+  if (n != s->h.n_nuc()) return fail(QMC_ERR_INVALID, "nucleus count mismatch");
+  s->h.nuc.assign(R, R + 3 * n);
+  s->dirty = true;
+  return upload(s);
+}
+
+int qmc_evaluate(const qmc_system* sys, int64_t W, const double* pos, double* logabs, double* sign, double* grad,
+                 double* lap, double* eloc, void* stream) {
+  if (!sys || !pos || W < 0) return fail(QMC_ERR_INVALID, "bad arguments");
+  if (W == 0) return QMC_OK;
+  int rc = need_engine(sys);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int N = sys->h.n_elec();
+  Staged P, L, S, G, Lp, E;
+  if ((rc = P.init(pos, sizeof(double) * W * N * 3, true, st))) return rc;
+  if ((rc = L.init(logabs, sizeof(double) * W, false, st))) return rc;
+  if ((rc = S.init(sign, sizeof(double) * W, false, st))) return rc;
+  if ((rc = G.init(grad, sizeof(double) * W * N * 3, false, st))) return rc;
+  if ((rc = Lp.init(lap, sizeof(double) * W * N, false, st))) return rc;
+  if ((rc = E.init(eloc, sizeof(double) * W, false, st))) return rc;
+  int* sing = nullptr;
+  HIP_TRY(hipMalloc(&sing, sizeof(int)));
+  std::unique_ptr<int, void (*)(int*)> sing_guard(sing, [](int* p) { (void)hipFree(p); });
+  HIP_TRY(hipMemsetAsync(sing, 0, sizeof(int), st));
+  EvalOut O{(double*)L.dev, (double*)S.dev, (double*)G.dev, (double*)Lp.dev, (double*)E.dev};
+  if (sys->precision == QMC_FP64)
+    HIP_TRY(sys->ops->eval64(sys->d64, sys->d64, W, (const double*)P.dev, O, sing, st));
+  else
+    HIP_TRY(sys->ops->eval32(sys->d64, sys->d32, W, (const double*)P.dev, O, sing, st));
+  for (Staged* s : {&L, &S, &G, &Lp, &E})
+    if ((rc = s->copy_out())) return rc;
+  int nsing = 0;
+  HIP_TRY(hipMemcpyAsync(&nsing, sing, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (nsing > 0 && !(sys->h.flags & QMC_FLAG_SINGULAR_IDENTITY))
+    return fail(QMC_ERR_SINGULAR, std::to_string(nsing) + " configurations with a singular Slater matrix");
+  return QMC_OK;
+}
+
+int qmc_eval_log_psi(const qmc_system* s, int64_t W, const double* pos, double* logabs, double* sign, void* st) {
+  return qmc_evaluate(s, W, pos, logabs, sign, nullptr, nullptr, nullptr, st);
+}
+int qmc_grad_log_psi(const qmc_system* s, int64_t W, const double* pos, double* g, void* st) {
+  return qmc_evaluate(s, W, pos, nullptr, nullptr, g, nullptr, nullptr, st);
+}
+int qmc_lap_over_psi(const qmc_system* s, int64_t W, const double* pos, double* l, void* st) {
+  return qmc_evaluate(s, W, pos, nullptr, nullptr, nullptr, l, nullptr, st);
+}
+int qmc_local_energy(const qmc_system* s, int64_t W, const double* pos, double* el, void* st) {
+  return qmc_evaluate(s, W, pos, nullptr, nullptr, nullptr, nullptr, el, st);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Ensembles
+// ===========================================================================
+struct qmc_ensemble {
+  const qmc_system* sys = nullptr;
+  int64_t W = 0, ld = 0;
+  int prec = QMC_FP64;
+  void* pos = nullptr;
+  void* ainv = nullptr;
+  double* logdet = nullptr;
+  double* sign = nullptr;
+  double* ju = nullptr;
+  double* eloc = nullptr;
+  uint64_t seed = 0;
+  uint32_t step = 0;
+  bool inverse_stale = false;
+  int since_refresh = 0;
+  double* partial = nullptr;
+  size_t partial_cap = 0;
+  double* sums = nullptr;
+  size_t sums_cap = 0;
+  hipStream_t st = nullptr;
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // start/stop pairs, one pair per launch
+  double kernel_ms = 0.0;
+  int64_t launches = 0;
+  ~qmc_ensemble() {
+    for (void* p : {pos, ainv, (void*)logdet, (void*)sign, (void*)ju, (void*)eloc, (void*)partial, (void*)sums})
+      if (p) (void)hipFree(p);
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  size_t tsize() const { return prec == QMC_FP64 ? sizeof(double) : sizeof(float); }
+  int nainv() const {
+    const int nu = sys->h.n_up(), nd = sys->h.n_dn();
+    return nu * nu + nd * nd;
+  }
+  template <class T>
+  EnsView<T> view() const {
+    return EnsView<T>{W, ld, (T*)pos, (T*)ainv, logdet, sign, ju, eloc};
+  }
+};
+
+namespace {
+
+int ens_refresh(qmc_ensemble* e, int want_eloc) {
+  const qmc_system* s = e->sys;
+  if (e->prec == QMC_FP64)
+    HIP_TRY(s->ops->build64(s->d64, s->d64, e->view<double>(), want_eloc, nullptr, e->st));
+  else
+    HIP_TRY(s->ops->build32(s->d64, s->d32, e->view<float>(), want_eloc, nullptr, e->st));
+  e->inverse_stale = false;
+  e->since_refresh = 0;
+  return QMC_OK;
+}
+
+int ensure_cap(double** p, size_t* cap, size_t n) {
+  if (*cap >= n) return QMC_OK;
+  if (*p) HIP_TRY(hipFree(*p));
+  *p = nullptr;
+  HIP_TRY(hipMalloc(p, n * sizeof(double)));
+  *cap = n;
+  return QMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qmc_ensemble_create(const qmc_system* sys, int64_t W, uint64_t seed, qmc_ensemble** out) {
+  if (!sys || !out || W < 1 || W > (int64_t)1 << 31) return fail(QMC_ERR_INVALID, "bad arguments");
+  int rc = need_engine(sys);
+  if (rc) return rc;
+  auto e = std::make_unique<qmc_ensemble>();
+  e->sys = sys;
+  e->W = W;
+  e->ld = (W + 63) / 64 * 64;
+  e->prec = sys->precision;
+  e->seed = seed;
+  const int N = sys->h.n_elec();
+  HIP_TRY(hipMalloc(&e->pos, e->tsize() * 3 * N * e->ld));
+  HIP_TRY(hipMalloc(&e->ainv, e->tsize() * std::max(1, e->nainv()) * e->ld));
+  for (double** p : {&e->logdet, &e->sign, &e->ju, &e->eloc}) HIP_TRY(hipMalloc(p, sizeof(double) * e->ld));
+  if (e->prec == QMC_FP64)
+    HIP_TRY(launch_init64(sys->d64, e->view<double>(), seed, 0, e->st));
+  else
+    HIP_TRY(launch_init32(sys->d64, e->view<float>(), seed, 0, e->st));
+  if ((rc = ens_refresh(e.get(), 1))) return rc;
+  HIP_TRY(hipStreamSynchronize(e->st));
+  *out = e.release();
+  return QMC_OK;
+}
+
+int qmc_ensemble_destroy(qmc_ensemble* e) {
+  delete e;
+  return QMC_OK;
+}
+
+int qmc_ensemble_get_positions(const qmc_ensemble* e, double* pos) {
+  if (!e || !pos) return fail(QMC_ERR_INVALID, "bad arguments");
+  const int N = e->sys->h.n_elec();
+  std::vector<char> buf(e->tsize() * 3 * N * e->ld);
+  HIP_TRY(hipMemcpy(buf.data(), e->pos, buf.size(), hipMemcpyDeviceToHost));
+  std::vector<double> tmp((size_t)e->W * N * 3);
+  for (int q = 0; q < 3 * N; ++q)
+    for (int64_t w = 0; w < e->W; ++w) {
+      const size_t i = (size_t)q * e->ld + w;
+      tmp[(size_t)w * 3 * N + q] = e->prec == QMC_FP64 ? ((double*)buf.data())[i] : ((float*)buf.data())[i];
+    }
+  if (is_device_ptr(pos))
+    HIP_TRY(hipMemcpy(pos, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+  else
+    std::copy(tmp.begin(), tmp.end(), pos);
+  return QMC_OK;
+}
+
+int qmc_ensemble_set_positions(qmc_ensemble* e, const double* pos) {
+  if (!e || !pos) return fail(QMC_ERR_INVALID, "bad arguments");
+  const int N = e->sys->h.n_elec();
+  std::vector<double> tmp((size_t)e->W * N * 3);
+  if (is_device_ptr(pos))
+    HIP_TRY(hipMemcpy(tmp.data(), pos, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  else
+    std::copy(pos, pos + tmp.size(), tmp.begin());
+  std::vector<char> buf(e->tsize() * 3 * N * e->ld, 0);
+  for (int q = 0; q < 3 * N; ++q)
+    for (int64_t w = 0; w < e->W; ++w) {
+      const size_t i = (size_t)q * e->ld + w;
+      const double v = tmp[(size_t)w * 3 * N + q];
+      if (e->prec == QMC_FP64)
+        ((double*)buf.data())[i] = v;
+      else
+        ((float*)buf.data())[i] = (float)v;
+    }
+  HIP_TRY(hipMemcpy(e->pos, buf.data(), buf.size(), hipMemcpyHostToDevice));
+  int rc = ens_refresh(e, 1);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(e->st));
+  return QMC_OK;
+}
+
+int qmc_ensemble_get_state(const qmc_ensemble* e, double* logabs, double* sign, double* eloc) {
+  if (!e) return fail(QMC_ERR_INVALID, "null ensemble");
+  std::vector<double> a(e->W), b(e->W);
+  if (logabs) {
+    HIP_TRY(hipMemcpy(a.data(), e->logdet, sizeof(double) * e->W, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b.data(), e->ju, sizeof(double) * e->W, hipMemcpyDeviceToHost));
+    for (int64_t w = 0; w < e->W; ++w) logabs[w] = a[w] + b[w];
+  }
+  if (sign) HIP_TRY(hipMemcpy(sign, e->sign, sizeof(double) * e->W, hipMemcpyDeviceToHost));
+  if (eloc) HIP_TRY(hipMemcpy(eloc, e->eloc, sizeof(double) * e->W, hipMemcpyDeviceToHost));
+  return QMC_OK;
+}
+
+int qmc_ensemble_refresh(qmc_ensemble* e) {
+  if (!e) return fail(QMC_ERR_INVALID, "null ensemble");
+  int rc = ens_refresh(e, 1);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(e->st));
+  return QMC_OK;
+}
+
+int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_sum, double* e2_sum,
+                        double* accepted, double* eloc_trace) {
+  if (!e || !p || p->n_steps < 0 || !(p->step_size >= 0.0)) return fail(QMC_ERR_INVALID, "bad arguments");
+  if (p->move != QMC_MOVE_SINGLE && p->move != QMC_MOVE_ALL) return fail(QMC_ERR_INVALID, "bad move kind");
+  if (p->n_steps == 0) return QMC_OK;
+  const qmc_system* s = e->sys;
+  int rc;
+  if (p->move == QMC_MOVE_SINGLE && e->inverse_stale && (rc = ens_refresh(e, 0))) return rc;
+  const int nblk = (int)((e->W + 63) / 64);
+  const int refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
+  int chunk = p->steps_per_launch > 0 ? p->steps_per_launch : 16;
+  if ((rc = ensure_cap(&e->partial, &e->partial_cap, (size_t)chunk * nblk * 3))) return rc;
+  if ((rc = ensure_cap(&e->sums, &e->sums_cap, (size_t)p->n_steps * 3))) return rc;
+  Staged tr;
+  if ((rc = tr.init(eloc_trace, sizeof(double) * (size_t)p->n_steps * e->W, false, e->st))) return rc;
+  int done = 0;
+  int64_t nl = 0;
+  while (done < p->n_steps) {
+    int ns = std::min(chunk, p->n_steps - done);
+    if (p->move == QMC_MOVE_SINGLE && refresh > 0) ns = std::min(ns, refresh - e->since_refresh);
+    SweepArgs A{ns, e->step, e->seed, 0, p->step_size, e->W};
+    double* trace = tr.dev ? (double*)tr.dev + (size_t)done * e->W : nullptr;
+    if (e->timing) {
+      while ((int64_t)e->ev.size() < 2 * (nl + 1)) {
+        hipEvent_t ev;
+        HIP_TRY(hipEventCreate(&ev));
+        e->ev.push_back(ev);
+      }
+      HIP_TRY(hipEventRecord(e->ev[2 * nl], e->st));
+    }
+    if (p->move == QMC_MOVE_SINGLE) {
+      if (e->prec == QMC_FP64)
+        HIP_TRY(s->ops->sweep64(s->d64, e->view<double>(), A, e->partial, trace, e->st));
+      else
+        HIP_TRY(s->ops->sweep32(s->d32, e->view<float>(), A, e->partial, trace, e->st));
+    } else {
+      if (e->prec == QMC_FP64)
+        HIP_TRY(s->ops->ae64(s->d64, s->d64, e->view<double>(), A, e->partial, trace, e->st));
+      else
+        HIP_TRY(s->ops->ae32(s->d64, s->d32, e->view<float>(), A, e->partial, trace, e->st));
+      e->inverse_stale = true;
+    }
+    if (e->timing) HIP_TRY(hipEventRecord(e->ev[2 * nl + 1], e->st));
+    ++nl;
+    HIP_TRY(launch_reduce(e->partial, ns, nblk, 3, e->sums + (size_t)done * 3, e->st));
+    e->step += (uint32_t)ns;
+    done += ns;
+    if (p->move == QMC_MOVE_SINGLE) {
+      e->since_refresh += ns;
+      if (refresh > 0 && e->since_refresh >= refresh && (rc = ens_refresh(e, 0))) return rc;
+    }
+  }
+  std::vector<double> h((size_t)p->n_steps * 3);
+  HIP_TRY(hipMemcpyAsync(h.data(), e->sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
+  if ((rc = tr.copy_out())) return rc;
+  HIP_TRY(hipStreamSynchronize(e->st));
+  e->launches = nl;
+  e->kernel_ms = 0.0;
+  if (e->timing)
+    for (int64_t i = 0; i < nl; ++i) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, e->ev[2 * i], e->ev[2 * i + 1]));
+      e->kernel_ms += ms;
+    }
+  for (int q = 0; q < 3; ++q) {
+    double* dst = q == 0 ? e_sum : (q == 1 ? e2_sum : accepted);
+    if (!dst) continue;
+    std::vector<double> col(p->n_steps);
+    for (int i = 0; i < p->n_steps; ++i) col[i] = h[(size_t)i * 3 + q];
+    if (is_device_ptr(dst))
+      HIP_TRY(hipMemcpy(dst, col.data(), col.size() * sizeof(double), hipMemcpyHostToDevice));
+    else
+      std::copy(col.begin(), col.end(), dst);
+  }
+  return QMC_OK;
+}
+
+int qmc_ensemble_set_timing(qmc_ensemble* e, int32_t on) {
+  if (!e) return fail(QMC_ERR_INVALID, "null ensemble");
+  e->timing = on != 0;
+  return QMC_OK;
+}
+
+int qmc_ensemble_timing(const qmc_ensemble* e, double* ms, int64_t* launches) {
+  if (!e) return fail(QMC_ERR_INVALID, "null ensemble");
+  if (ms) *ms = e->kernel_ms;
+  if (launches) *launches = e->launches;
+  return QMC_OK;
+}
+
+// vmc.rs:141-163
+double qmc_autocorrelation_time(const double* e, int64_t n) {
+  if (!e || n < 1) return 1.0;
+  double mean = 0.0;
+  for (int64_t i = 0; i < n; ++i) mean += e[i];
+  mean /= (double)n;
+  double var = 0.0;
+  for (int64_t i = 0; i < n; ++i) var += (e[i] - mean) * (e[i] - mean);
+  var /= (double)n;
+  if (var == 0.0) return 1.0;
+  double ac = 1.0;
+  for (int64_t t = 1; t < n / 2; ++t) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n - t; ++i) s += (e[i] - mean) * (e[i + t] - mean);
+    const double at = s / ((double)(n - t) * var);
+    if (at < 0.0) break;
+    ac += 2.0 * at;
+  }
+  return ac;
+}
+
+// vmc.rs:166-188
+double qmc_blocking_error(const double* e, int64_t n, double tau) {
+  const int64_t bs = (int64_t)std::ceil(2.0 * tau);
+  if (!e || bs < 1) return 0.0;
+  const int64_t nb = n / bs;
+  if (nb < 2) return 0.0;
+  std::vector<double> bm(nb);
+  for (int64_t b = 0; b < nb; ++b) {
+    double s = 0.0;
+    for (int64_t i = b * bs; i < b * bs + bs; ++i) s += e[i];
+    bm[b] = s / (double)bs;
+  }
+  double mean = 0.0;
+  for (double v : bm) mean += v;
+  mean /= (double)nb;
+  double var = 0.0;
+  for (double v : bm) var += (v - mean) * (v - mean);
+  var /= (double)(nb - 1);
+  return std::sqrt(var / (double)nb);
+}
+
+// MCMCSimulation::run (vmc.rs:69-92) + compute_results (vmc.rs:131-138)
+int qmc_vmc_run(const qmc_system* sys, const qmc_mcmc_params* p, uint64_t seed, qmc_vmc_results* res,
+                double* energy_trace) {
+  if (!sys || !p || !res || p->n_walkers < 1 || p->n_steps < 1 || p->adaptation_interval < 1 ||
+      p->n_burnin < 0 || p->n_burnin >= p->n_steps)
+    return fail(QMC_ERR_INVALID, "bad arguments");
+  qmc_ensemble* e = nullptr;
+  int rc = qmc_ensemble_create(sys, p->n_walkers, seed, &e);
+  if (rc) return rc;
+  std::unique_ptr<qmc_ensemble> guard(e);
+  const int N = sys->h.n_elec();
+  const double per_step_moves = p->move == QMC_MOVE_ALL ? (double)p->n_walkers : (double)p->n_walkers * N;
+  std::vector<double> energies(p->n_steps), es(p->adaptation_interval), acc(p->adaptation_interval);
+  double step = p->initial_step_size, acc_total = 0.0;
+  for (int64_t done = 0; done < p->n_steps;) {
+    const int ns = (int)std::min<int64_t>(p->adaptation_interval, p->n_steps - done);
+    qmc_sweep_params sp{p->move, ns, step, p->refresh_interval, 0};
+    if ((rc = qmc_ensemble_sweeps(e, &sp, es.data(), nullptr, acc.data(), nullptr))) return rc;
+    double a = 0.0;
+    for (int i = 0; i < ns; ++i) {
+      energies[done + i] = es[i] / (double)p->n_walkers;
+      a += acc[i];
+    }
+    acc_total += a;
+    done += ns;
+    if (ns == p->adaptation_interval) {  // vmc.rs:85-88, 122-128
+      const double rate = a / (per_step_moves * (double)p->adaptation_interval);
+      step = std::min(std::max(step * std::sqrt(rate / p->target_acceptance), p->min_step_size), p->max_step_size);
+    }
+  }
+  const double* e0 = energies.data() + p->n_burnin;
+  const int64_t n = p->n_steps - p->n_burnin;
+  double mean = 0.0;
+  for (int64_t i = 0; i < n; ++i) mean += e0[i];
+  res->energy = mean / (double)n;
+  res->autocorrelation_time = qmc_autocorrelation_time(e0, n);
+  res->error = qmc_blocking_error(e0, n, res->autocorrelation_time);
+  res->acceptance = acc_total / (per_step_moves * (double)p->n_steps);
+  res->final_step_size = step;
+  if (energy_trace) std::copy(energies.begin(), energies.end(), energy_trace);
+  return QMC_OK;
+}
+
+}  // extern "C"

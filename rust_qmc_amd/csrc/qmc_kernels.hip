@@ -1,0 +1,396 @@
+// qmc_kernels.hip -- gfx950 kernels of the walker engine and their
+// launchers.  Walker state lives in HBM between launches as
+// structure-of-arrays with the walker index fastest (coalesced 64-lane
+// loads), and in VGPRs inside a launch.
+#include "qmc_kernels.hpp"
+
+#include "qmc_engine.hpp"
+
+namespace qmc {
+
+template <int NU, int ND, class T>
+struct WalkerRegs {
+  T ru[Cap<NU>::v][3], rd[Cap<ND>::v][3];
+  T au[Cap<NU>::v][Cap<NU>::v], ad[Cap<ND>::v][Cap<ND>::v];
+};
+
+template <int NU, int ND, class T>
+__device__ __forceinline__ void load_positions(const DevSys<T>& S, const T* pos, int64_t ld, int64_t w,
+                                               WalkerRegs<NU, ND, T>& W) {
+#pragma unroll
+  for (int k = 0; k < NU; ++k) {
+    const int e = S.sector_elec[0][k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) W.ru[k][c] = pos[(e * 3 + c) * ld + w];
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const int e = S.sector_elec[1][k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) W.rd[k][c] = pos[(e * 3 + c) * ld + w];
+  }
+}
+
+template <int NU, int ND, class T>
+__device__ __forceinline__ void store_positions(const DevSys<T>& S, T* pos, int64_t ld, int64_t w,
+                                                const WalkerRegs<NU, ND, T>& W) {
+#pragma unroll
+  for (int k = 0; k < NU; ++k) {
+    const int e = S.sector_elec[0][k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pos[(e * 3 + c) * ld + w] = W.ru[k][c];
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const int e = S.sector_elec[1][k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pos[(e * 3 + c) * ld + w] = W.rd[k][c];
+  }
+}
+
+template <int NU, int ND, class T>
+__device__ __forceinline__ void load_inverse(const T* ainv, int64_t ld, int64_t w, WalkerRegs<NU, ND, T>& W) {
+#pragma unroll
+  for (int m = 0; m < NU; ++m)
+#pragma unroll
+    for (int k = 0; k < NU; ++k) W.au[m][k] = ainv[(m * NU + k) * ld + w];
+#pragma unroll
+  for (int m = 0; m < ND; ++m)
+#pragma unroll
+    for (int k = 0; k < ND; ++k) W.ad[m][k] = ainv[(NU * NU + m * ND + k) * ld + w];
+}
+
+template <int NU, int ND, class T>
+__device__ __forceinline__ void store_inverse(T* ainv, int64_t ld, int64_t w, const WalkerRegs<NU, ND, T>& W) {
+#pragma unroll
+  for (int m = 0; m < NU; ++m)
+#pragma unroll
+    for (int k = 0; k < NU; ++k) ainv[(m * NU + k) * ld + w] = W.au[m][k];
+#pragma unroll
+  for (int m = 0; m < ND; ++m)
+#pragma unroll
+    for (int k = 0; k < ND; ++k) ainv[(NU * NU + m * ND + k) * ld + w] = W.ad[m][k];
+}
+
+// ---------------------------------------------------------------------------
+// initial positions: electron e of walker w = nuc[init_nuc[e]] + sigma z,
+// z from Philox counter (w, 0, e, INIT), Box-Muller in FP64
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(64) init_positions_kernel(const DevSys<double>* __restrict__ Sp, T* pos,
+                                                            int64_t W, int64_t ld, uint64_t seed,
+                                                            int64_t w0) {
+  const DevSys<double>& S = *Sp;
+  const int64_t w = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (w >= W) return;
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  for (int e = 0; e < S.n_elec; ++e) {
+    double z[3];
+    Rng<double>::normals3(philox((uint32_t)(w0 + w), 0u, (uint32_t)e, RNG_INIT, k0, k1), z);
+    const int c = S.init_nuc[e];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) pos[(e * 3 + q) * ld + w] = (T)(S.nuc[c][q] + S.init_sigma * z[q]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// state build / refresh: Slater matrices and inverses in FP64 from the
+// current positions, ln|D|, sign, Jastrow exponent; optionally E_L.
+// ---------------------------------------------------------------------------
+template <int NU, int ND, class T>
+__global__ void __launch_bounds__(64) build_state_kernel(const DevSys<double>* __restrict__ Sd,
+                                                         const DevSys<T>* __restrict__ St, EnsView<T> E,
+                                                         int want_eloc, int* singular_count) {
+  const int64_t w = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (w >= E.W) return;
+  WalkerRegs<NU, ND, T> R;
+  load_positions<NU, ND>(*St, E.pos, E.ld, w, R);
+  double lu, su, ldn, sd;
+  bool s1 = sector_inverse<NU>(*Sd, 0, R.ru, R.au, lu, su);
+  bool s2 = sector_inverse<ND>(*Sd, 1, R.rd, R.ad, ldn, sd);
+  if ((s1 || s2) && singular_count) atomicAdd(singular_count, 1);
+  store_inverse<NU, ND>(E.ainv, E.ld, w, R);
+  E.logdet[w] = lu + ldn;
+  E.sign[w] = su * sd;
+  E.ju[w] = jastrow_u<NU, ND>(*Sd, R.ru, R.rd);
+  if (want_eloc) E.eloc[w] = local_energy<NU, ND>(*St, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
+}
+
+// ---------------------------------------------------------------------------
+// persistent single-electron sweeps: n_steps sweeps, E_L after each, and
+// per-(step, block) partial sums {sum E_L, sum E_L^2, accepted moves}
+// ---------------------------------------------------------------------------
+template <int NU, int ND, class T>
+__global__ void __launch_bounds__(64) sweep_kernel(const DevSys<T>* __restrict__ Sp, EnsView<T> E,
+                                                   SweepArgs A, double* __restrict__ partial,
+                                                   double* __restrict__ el_trace) {
+  const DevSys<T>& S = *Sp;
+  const int64_t w = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const bool live = w < E.W;
+  const int64_t wl = live ? w : E.W - 1;  // dead lanes shadow the last walker and write nothing
+  WalkerRegs<NU, ND, T> R;
+  load_positions<NU, ND>(S, E.pos, E.ld, wl, R);
+  load_inverse<NU, ND>(E.ainv, E.ld, wl, R);
+  double logdet = E.logdet[wl], sgn = E.sign[wl], ju = E.ju[wl];
+  double el = 0.0;
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  const uint32_t wid = (uint32_t)(A.walker0 + wl);
+  const T sigma = (T)A.step_size;
+  for (int s = 0; s < A.n_steps; ++s) {
+    const uint32_t step = A.step0 + (uint32_t)s;
+    int acc = 0;
+    sector_moves<NU, ND>(S, 0, R.ru, R.au, R.rd, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
+    sector_moves<ND, NU>(S, 1, R.rd, R.ad, R.ru, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
+    el = local_energy<NU, ND>(S, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
+    if (el_trace && live) el_trace[(int64_t)s * A.trace_ld + w] = el;
+    double e1 = live ? el : 0.0, e2 = live ? el * el : 0.0, a = live ? (double)acc : 0.0;
+    e1 = wave_sum(e1);
+    e2 = wave_sum(e2);
+    a = wave_sum(a);
+    if (threadIdx.x == 0) {
+      double* p = partial + ((int64_t)s * gridDim.x + blockIdx.x) * 3;
+      p[0] = e1;
+      p[1] = e2;
+      p[2] = a;
+    }
+  }
+  if (live) {
+    store_positions<NU, ND>(S, E.pos, E.ld, w, R);
+    store_inverse<NU, ND>(E.ainv, E.ld, w, R);
+    E.logdet[w] = logdet;
+    E.sign[w] = sgn;
+    E.ju[w] = ju;
+    E.eloc[w] = el;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// all-electron Metropolis (MCMCSimulation::metropolis_step, vmc.rs:95-119):
+// every electron moved at once, full recomputation of ln|psi|, E_L
+// recomputed on acceptance only and cached otherwise.
+// ---------------------------------------------------------------------------
+template <int NU, int ND, class T>
+__global__ void __launch_bounds__(64) all_electron_kernel(const DevSys<double>* __restrict__ Sd,
+                                                          const DevSys<T>* __restrict__ Sp, EnsView<T> E,
+                                                          SweepArgs A, double* __restrict__ partial,
+                                                          double* __restrict__ el_trace) {
+  const DevSys<T>& S = *Sp;
+  const int64_t w = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const bool live = w < E.W;
+  const int64_t wl = live ? w : E.W - 1;
+  WalkerRegs<NU, ND, T> R, Q;
+  load_positions<NU, ND>(S, E.pos, E.ld, wl, R);
+  double ldet = E.logdet[wl], ju = E.ju[wl], sgn = E.sign[wl], el = E.eloc[wl];
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  const uint32_t wid = (uint32_t)(A.walker0 + wl);
+  const T sigma = (T)A.step_size;
+  const uint32_t N = (uint32_t)S.n_elec;
+  for (int s = 0; s < A.n_steps; ++s) {
+    const uint32_t step = A.step0 + (uint32_t)s;
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+      T z[3];
+      Rng<T>::normals3(philox(wid, step, (uint32_t)S.sector_elec[0][k], RNG_ALL, k0, k1), z);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Q.ru[k][c] = R.ru[k][c] + sigma * z[c];
+    }
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+      T z[3];
+      Rng<T>::normals3(philox(wid, step, (uint32_t)S.sector_elec[1][k], RNG_ALL, k0, k1), z);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Q.rd[k][c] = R.rd[k][c] + sigma * z[c];
+    }
+    double lu, su, ldn, sd;
+    sector_inverse<NU>(*Sd, 0, Q.ru, Q.au, lu, su);
+    sector_inverse<ND>(*Sd, 1, Q.rd, Q.ad, ldn, sd);
+    const double jun = jastrow_u<NU, ND>(*Sd, Q.ru, Q.rd);
+    const double U = u01d(philox(wid, step, N, RNG_ALL, k0, k1).x);
+    int acc = 0;
+    if (U < exp(2.0 * ((lu + ldn + jun) - (ldet + ju)))) {
+      R = Q;
+      ldet = lu + ldn;
+      ju = jun;
+      sgn = su * sd;
+      el = local_energy<NU, ND>(S, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
+      acc = 1;
+    }
+    if (el_trace && live) el_trace[(int64_t)s * A.trace_ld + w] = el;
+    double e1 = live ? el : 0.0, e2 = live ? el * el : 0.0, a = live ? (double)acc : 0.0;
+    e1 = wave_sum(e1);
+    e2 = wave_sum(e2);
+    a = wave_sum(a);
+    if (threadIdx.x == 0) {
+      double* p = partial + ((int64_t)s * gridDim.x + blockIdx.x) * 3;
+      p[0] = e1;
+      p[1] = e2;
+      p[2] = a;
+    }
+  }
+  if (live) {
+    store_positions<NU, ND>(S, E.pos, E.ld, w, R);
+    E.logdet[w] = ldet;
+    E.ju[w] = ju;
+    E.sign[w] = sgn;
+    E.eloc[w] = el;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// batched per-configuration evaluation of the trait methods.  Input
+// positions are [W][N][3] FP64 in global electron order (the reference's
+// Vec<Vector3<f64>> layout); outputs are written walker-major.
+// ---------------------------------------------------------------------------
+template <int NU, int ND, class T>
+__global__ void __launch_bounds__(64) eval_kernel(const DevSys<double>* __restrict__ Sd,
+                                                  const DevSys<T>* __restrict__ St, int64_t W,
+                                                  const double* __restrict__ pos, EvalOut O, int* singular_count) {
+  const int64_t w = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (w >= W) return;
+  const DevSys<T>& S = *St;
+  const int N = NU + ND;
+  // the determinants, inverses and ln J come from the FP64 positions; the
+  // local-energy terms then run in T
+  WalkerRegs<NU, ND, double> D;
+#pragma unroll
+  for (int k = 0; k < NU; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) D.ru[k][c] = pos[(w * N + S.sector_elec[0][k]) * 3 + c];
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) D.rd[k][c] = pos[(w * N + S.sector_elec[1][k]) * 3 + c];
+  double lu, su, ldn, sd;
+  bool s1 = sector_inverse<NU>(*Sd, 0, D.ru, D.au, lu, su);
+  bool s2 = sector_inverse<ND>(*Sd, 1, D.rd, D.ad, ldn, sd);
+  if ((s1 || s2) && singular_count) atomicAdd(singular_count, 1);
+  if (O.logabs) O.logabs[w] = lu + ldn + jastrow_u<NU, ND>(*Sd, D.ru, D.rd);
+  if (O.sign) O.sign[w] = su * sd;
+  WalkerRegs<NU, ND, T> R;
+#pragma unroll
+  for (int k = 0; k < Cap<NU>::v; ++k) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R.ru[k][c] = (T)D.ru[k][c];
+#pragma unroll
+    for (int m = 0; m < Cap<NU>::v; ++m) R.au[k][m] = (T)D.au[k][m];
+  }
+#pragma unroll
+  for (int k = 0; k < Cap<ND>::v; ++k) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R.rd[k][c] = (T)D.rd[k][c];
+#pragma unroll
+    for (int m = 0; m < Cap<ND>::v; ++m) R.ad[k][m] = (T)D.ad[k][m];
+  }
+  if (O.eloc || O.grad || O.lap) {
+    // per-electron terms land in private arrays in global electron order
+    const bool per_e = O.grad || O.lap;
+    double g[3 * (N > 0 ? N : 1)], l[N > 0 ? N : 1];
+    double el = local_energy<NU, ND>(S, R.ru, R.au, R.rd, R.ad, per_e ? g : nullptr, per_e ? l : nullptr, 1);
+    if (O.eloc) O.eloc[w] = el;
+    if (O.grad)
+#pragma unroll
+      for (int i = 0; i < 3 * N; ++i) O.grad[w * 3 * N + i] = g[i];
+    if (O.lap)
+#pragma unroll
+      for (int i = 0; i < N; ++i) O.lap[w * N + i] = l[i];
+  }
+}
+
+// deterministic reduction of per-block partials: out[s][q] = sum_b partial[s][b][q]
+__global__ void __launch_bounds__(256) reduce_partials_kernel(const double* __restrict__ partial, int nblk,
+                                                              int nq, double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int s = blockIdx.x;
+  for (int q = 0; q < nq; ++q) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) v += partial[((int64_t)s * nblk + b) * nq + q];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(int64_t)s * nq + q] = sh[0];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned nblocks(int64_t W) { return (unsigned)((W + 63) / 64); }
+
+template <class T>
+static hipError_t launch_init(const void* Sd, EnsView<T> E, uint64_t seed, int64_t w0, hipStream_t st) {
+  init_positions_kernel<T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<double>*)Sd, E.pos, E.W, E.ld, seed, w0);
+  return hipGetLastError();
+}
+
+template <int NU, int ND, class T>
+struct Launch {
+  static hipError_t build(const void* Sd, const void* St, EnsView<T> E, int want_eloc, int* sing,
+                          hipStream_t st) {
+    build_state_kernel<NU, ND, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<double>*)Sd,
+                                                              (const DevSys<T>*)St, E, want_eloc, sing);
+    return hipGetLastError();
+  }
+  static hipError_t sweep(const void* St, EnsView<T> E, SweepArgs A, double* partial, double* trace,
+                          hipStream_t st) {
+    sweep_kernel<NU, ND, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<T>*)St, E, A, partial, trace);
+    return hipGetLastError();
+  }
+  static hipError_t all_electron(const void* Sd, const void* St, EnsView<T> E, SweepArgs A, double* partial,
+                                 double* trace, hipStream_t st) {
+    all_electron_kernel<NU, ND, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<double>*)Sd,
+                                                               (const DevSys<T>*)St, E, A, partial, trace);
+    return hipGetLastError();
+  }
+  static hipError_t eval(const void* Sd, const void* St, int64_t W, const double* pos, EvalOut O, int* sing,
+                         hipStream_t st) {
+    eval_kernel<NU, ND, T><<<nblocks(W), 64, 0, st>>>((const DevSys<double>*)Sd, (const DevSys<T>*)St, W,
+                                                     pos, O, sing);
+    return hipGetLastError();
+  }
+};
+
+template <int NU, int ND>
+static EngineOps make_ops() {
+  EngineOps o;
+  o.nu = NU;
+  o.nd = ND;
+  o.build64 = &Launch<NU, ND, double>::build;
+  o.build32 = &Launch<NU, ND, float>::build;
+  o.sweep64 = &Launch<NU, ND, double>::sweep;
+  o.sweep32 = &Launch<NU, ND, float>::sweep;
+  o.ae64 = &Launch<NU, ND, double>::all_electron;
+  o.ae32 = &Launch<NU, ND, float>::all_electron;
+  o.eval64 = &Launch<NU, ND, double>::eval;
+  o.eval32 = &Launch<NU, ND, float>::eval;
+  return o;
+}
+
+// electron-count shapes compiled for the register engine
+static const EngineOps kOps[] = {
+    make_ops<1, 0>(), make_ops<1, 1>(), make_ops<2, 1>(), make_ops<2, 2>(), make_ops<3, 2>(),
+    make_ops<3, 3>(), make_ops<4, 4>(), make_ops<5, 5>(),
+};
+
+const EngineOps* find_engine(int nu, int nd) {
+  for (const auto& o : kOps)
+    if (o.nu == nu && o.nd == nd) return &o;
+  return nullptr;
+}
+
+hipError_t launch_init64(const void* Sd, EnsView<double> E, uint64_t seed, int64_t w0, hipStream_t st) {
+  return launch_init<double>(Sd, E, seed, w0, st);
+}
+hipError_t launch_init32(const void* Sd, EnsView<float> E, uint64_t seed, int64_t w0, hipStream_t st) {
+  return launch_init<float>(Sd, E, seed, w0, st);
+}
+
+hipError_t launch_reduce(const double* partial, int nsteps, int nblk, int nq, double* out, hipStream_t st) {
+  reduce_partials_kernel<<<nsteps, 256, 0, st>>>(partial, nblk, nq, out);
+  return hipGetLastError();
+}
+
+}  // namespace qmc

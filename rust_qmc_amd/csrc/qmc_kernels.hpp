@@ -1,0 +1,58 @@
+// qmc_kernels.hpp -- host-visible views of device buffers and the launcher
+// table of the register engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qmc {
+
+// Walker ensemble resident in HBM, structure-of-arrays with the walker
+// index fastest; ld is the padded walker stride.
+template <class T>
+struct EnsView {
+  int64_t W, ld;
+  T* pos;          // [N*3][ld]
+  T* ainv;         // [NU*NU + ND*ND][ld], inverse Slater matrices [mo][row]
+  double* logdet;  // [ld] ln|D_up D_dn|
+  double* sign;    // [ld]
+  double* ju;      // [ld] Jastrow exponent ln J
+  double* eloc;    // [ld] last local energy
+};
+
+struct SweepArgs {
+  int n_steps;
+  uint32_t step0;
+  uint64_t seed;
+  int64_t walker0;  // Philox walker index of lane 0
+  double step_size;
+  int64_t trace_ld;  // walker stride of the optional E_L trace
+};
+
+// Batched evaluation outputs; any pointer may be null.
+struct EvalOut {
+  double* logabs;  // [W]
+  double* sign;    // [W]
+  double* grad;    // [W][N][3] grad ln psi
+  double* lap;     // [W][N] lap psi / psi
+  double* eloc;    // [W]
+};
+
+struct EngineOps {
+  int nu, nd;
+  hipError_t (*build64)(const void*, const void*, EnsView<double>, int, int*, hipStream_t);
+  hipError_t (*build32)(const void*, const void*, EnsView<float>, int, int*, hipStream_t);
+  hipError_t (*sweep64)(const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
+  hipError_t (*sweep32)(const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
+  hipError_t (*ae64)(const void*, const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
+  hipError_t (*ae32)(const void*, const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
+  hipError_t (*eval64)(const void*, const void*, int64_t, const double*, EvalOut, int*, hipStream_t);
+  hipError_t (*eval32)(const void*, const void*, int64_t, const double*, EvalOut, int*, hipStream_t);
+};
+
+const EngineOps* find_engine(int nu, int nd);
+hipError_t launch_init64(const void* Sd, EnsView<double> E, uint64_t seed, int64_t w0, hipStream_t st);
+hipError_t launch_init32(const void* Sd, EnsView<float> E, uint64_t seed, int64_t w0, hipStream_t st);
+hipError_t launch_reduce(const double* partial, int nsteps, int nblk, int nq, double* out, hipStream_t st);
+
+}  // namespace qmc

@@ -1,0 +1,225 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
+
+FP64 mode must reproduce the oracle to 1e-10 relative on identical
+configurations (log psi, grad ln psi = drift/2, lap psi / psi, E_L), and
+follow the oracle's Metropolis trajectories step for step from the same
+Philox seed.  Mixed mode is held to the FP32 tolerances stated per test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL64 = 1e-10
+
+
+@pytest.fixture(scope="module")
+def Q():
+    import rust_qmc_amd as Q
+    Q.lib()
+    return Q
+
+
+def _pair(Q, oracle, name, precision="fp64"):
+    """(GPU system, oracle system) built from the same reference constructor."""
+    O = oracle.System
+    if name == "h2":
+        return Q.H2MoleculeMO(1.3, (0, 0, 0.7), (0, 0, -0.7), 3.0, precision=precision), O.h2mo(1.3, 0.7, 3.0)
+    if name == "li_ref":
+        return Q.Lithium(None, precision=precision), O.lithium_ref(exact=False)
+    if name == "li_ref_exact":
+        return Q.Lithium(None, flags=0, precision=precision), O.lithium_ref(exact=True)
+    if name == "li_seeded":
+        return Q.Lithium(0, precision=precision), O.lithium_config(0, exact=False)
+    if name == "ch4_631g":
+        return Q.MethaneGTO(1.5, 2.0, 2.05, "6-31G", precision=precision), O.methane_gto(1.5, 2.0, 2.05)
+    if name == "ch4_sto3g":
+        return (Q.MethaneGTO(1.5, 2.0, 2.06, "STO-3G", 0, precision=precision),
+                O.methane_sto3g(1.5, 2.0, 2.06, 0))
+    raise KeyError(name)
+
+
+SYSTEMS = ["h2", "li_ref", "li_ref_exact", "li_seeded", "ch4_631g", "ch4_sto3g"]
+
+
+def _configs(oracle_sys, W, seed=3, equil=4):
+    """Seeded configurations, partly equilibrated with oracle sweeps."""
+    pos = oracle_sys.initialize_batch(seed, W)
+    if equil:
+        pos, _, _, _ = oracle_sys.single_electron_steps(pos, seed, 1000, equil, 0.4)
+    return pos
+
+
+def _close(a, b, rtol, scale=None):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    s = np.abs(b) if scale is None else scale
+    return np.all(np.abs(a - b) <= rtol * np.maximum(s, 1e-300) + rtol * 1e-3)
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_eval_parity_fp64(Q, oracle, name):
+    g, o = _pair(Q, oracle, name)
+    pos = _configs(o, 97)  # ragged: not a multiple of the 64-lane block
+    ref = o.batch_eval(pos)
+    out = g.evaluate_batch(pos)
+    assert np.array_equal(out["sign"], ref["sign"])
+    assert _close(out["logabs"], ref["logabs"], RTOL64)
+    gs = np.abs(ref["grad"]).max(axis=(1, 2))[:, None, None]
+    assert _close(out["grad"], ref["grad"], RTOL64, scale=gs)
+    ls = np.abs(ref["lap"]).max(axis=1)[:, None]
+    assert _close(out["lap"], ref["lap"], RTOL64, scale=ls)
+    assert _close(out["eloc"], ref["eloc"], RTOL64)
+
+
+@pytest.mark.parametrize("name", ["h2", "li_seeded", "ch4_631g", "ch4_sto3g"])
+def test_eval_mixed_precision_error(Q, oracle, name):
+    """FP32 orbitals/Jastrow with an FP64 inverse: per-configuration E_L
+    within 2e-4 relative (plus 1e-3 Ha absolute), ln|psi| to 1e-12."""
+    g, o = _pair(Q, oracle, name, precision="mixed")
+    pos = _configs(o, 64)
+    ref = o.batch_eval(pos)
+    out = g.evaluate_batch(pos)
+    assert _close(out["logabs"], ref["logabs"], 1e-12)
+    err = np.abs(out["eloc"] - ref["eloc"])
+    assert np.all(err <= 2e-4 * np.abs(ref["eloc"]) + 1e-3), err.max()
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_single_electron_trajectory_parity(Q, oracle, name):
+    """Same seed -> the GPU's Sherman-Morrison sweeps follow the oracle's
+    brute-force single-electron Metropolis chain step for step."""
+    g, o = _pair(Q, oracle, name)
+    W, steps, seed, step_size = 70, 12, 11, 0.45
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    es, e2, acc, tr = ens.sweeps(steps, step_size, Q.QMC_MOVE_SINGLE, refresh_interval=5, steps_per_launch=3,
+                                 trace=True)
+    pos_o, lp_o, el_o, acc_o = o.single_electron_steps(p0, seed, 0, steps, step_size)
+    assert acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=1e-9)
+    assert _close(tr, el_o, 1e-9)
+    la, sg, el = ens.state()
+    assert _close(la, lp_o, 1e-9)
+    assert np.allclose(es, el_o.sum(axis=1), rtol=1e-9)
+
+
+@pytest.mark.parametrize("name", ["h2", "li_ref", "ch4_sto3g"])
+def test_all_electron_trajectory_parity(Q, oracle, name):
+    """MCMCSimulation::metropolis_step semantics (vmc.rs:95-119): all
+    electrons moved, E_L recomputed on acceptance only."""
+    g, o = _pair(Q, oracle, name)
+    W, steps, seed, step_size = 66, 15, 5, 0.3
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    la0, _, el0 = ens.state()
+    es, e2, acc, tr = ens.sweeps(steps, step_size, Q.QMC_MOVE_ALL, steps_per_launch=4, trace=True)
+    ref = o.batch_eval(p0)
+    assert _close(el0, ref["eloc"], 1e-10)
+    pos_o, lp_o, el_o, acc_o = o.all_electron_steps(p0, ref["logabs"], ref["eloc"], seed, 0, steps, step_size)
+    assert acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=1e-9)
+    assert _close(tr[-1], el_o, 1e-9)
+
+
+def test_vmc_run_reproduces_oracle_mcmc(Q, oracle):
+    """qmc_vmc_run (MCMCSimulation::run) vs the oracle's literal run:
+    identical per-step energy series, tau and blocking error."""
+    g, o = _pair(Q, oracle, "h2")
+    sim = Q.MCMCSimulation(g, Q.MCMCParams(64, 300, 0.5, 2.0, 0.05, 0.5, 100, 0, Q.QMC_MOVE_ALL), seed=2)
+    res = sim.run()
+    ores, oe = o.mcmc_run(64, 300, 0.5, seed=2, max_step=2.0, min_step=0.05, target=0.5, interval=100)
+    assert np.allclose(sim.energies, oe, rtol=1e-9)
+    assert res.energy == pytest.approx(ores.energy, rel=1e-10)
+    assert res.autocorrelation_time == pytest.approx(ores.autocorrelation_time, rel=1e-6)
+    assert res.acceptance == pytest.approx(ores.acceptance, rel=1e-12)
+    assert res.final_step_size == pytest.approx(ores.final_step, rel=1e-12)
+
+
+def test_sweeps_reproducible(Q, oracle):
+    g, _ = _pair(Q, oracle, "ch4_sto3g", precision="mixed")
+    runs = []
+    for _ in range(2):
+        ens = Q.Ensemble(g, 200, 9)
+        runs.append(ens.sweeps(10, 0.3))
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
+def test_mixed_vs_fp64_ensemble_energy(Q, oracle):
+    """Mixed precision samples the same |psi|^2: ensemble energies of FP64
+    and mixed runs of CH4/STO-3G agree within 2.5 sigma."""
+    res = {}
+    for prec in ("fp64", "mixed"):
+        g, _ = _pair(Q, oracle, "ch4_sto3g", precision=prec)
+        ens = Q.Ensemble(g, 4096, 21)
+        ens.sweeps(40, 0.3)
+        es, e2, acc = ens.sweeps(160, 0.3, refresh_interval=64)
+        tr = es / 4096
+        tau = Q.autocorrelation_time(tr)
+        res[prec] = (tr.mean(), Q.blocking_error(tr, tau))
+    (a, sa), (b, sb) = res["fp64"], res["mixed"]
+    assert abs(a - b) < 2.5 * math.hypot(sa, sb) + 1e-4
+
+
+def test_device_pointer_inputs(Q, oracle):
+    """Device-resident positions and outputs (buffers from the library's own
+    allocator) give the same results as host buffers."""
+    g, o = _pair(Q, oracle, "ch4_631g")
+    pos = _configs(o, 33)
+    out_h = g.evaluate_batch(pos)
+    with Q.DeviceBuffer(pos.nbytes) as dpos, Q.DeviceBuffer(33 * 8) as del_:
+        dpos.upload(pos)
+        Q.check(Q.lib().qmc_local_energy(g._h, 33, dpos.ptr, del_.ptr, None))
+        el = del_.download(np.float64, 33)
+    assert np.array_equal(el, out_h["eloc"])
+
+
+def test_singular_configuration_reported(Q, oracle):
+    g, _ = _pair(Q, oracle, "ch4_631g")
+    pos = _configs(oracle.System.methane_gto(), 4)
+    # a spin-up electron 170 bohr out: every Gaussian underflows to exactly 0,
+    # the Slater matrix has a zero row and its last pivot is exactly 0
+    pos[1, 2] = [100.0, 100.0, 100.0]
+    with pytest.raises(Q.QmcError) as ei:
+        g.evaluate_batch(pos)
+    assert ei.value.code == -4
+    g.set_flags(Q.QMC_FLAGS_REFERENCE)  # methane.rs:829 identity fallback
+    out = g.evaluate_batch(pos, ("logabs", "sign"))
+    assert out["logabs"][1] == -np.inf and out["sign"][1] == 0.0
+    assert np.isfinite(out["logabs"][[0, 2, 3]]).all()
+
+
+def test_empty_and_single_walker(Q, oracle):
+    g, o = _pair(Q, oracle, "h2")
+    assert g.evaluate_batch(np.zeros((0, 2, 3)))["eloc"].shape == (0,)
+    pos = _configs(o, 1)
+    assert g.local_energy(pos[0]) == pytest.approx(o.local_energy(pos[0]), rel=RTOL64)
+
+
+def test_hydrogenic_zero_variance(Q):
+    """One electron in STO{c=1,p=0,zeta=3} about Z=3: E_L = -4.5 everywhere
+    (lithium.rs:84-99)."""
+    s = Q.STOSlaterDet([([1.0], [0], [3.0])], [1], Z=3.0, flags=0)
+    rng = np.random.default_rng(0)
+    pos = rng.normal(size=(50, 1, 3)) * 0.7
+    el = s.evaluate_batch(pos, ("eloc",))["eloc"]
+    assert np.allclose(el, -4.5, rtol=0, atol=1e-12)
+
+
+def test_nuclei_and_params_roundtrip(Q, oracle):
+    g, o = _pair(Q, oracle, "ch4_631g")
+    assert g.nuclear_repulsion() == pytest.approx(13.499626641060864, rel=1e-14)
+    R = g.get_nuclei()
+    R2 = R + 0.01 * np.arange(R.size).reshape(R.shape)
+    g.set_nuclei(R2)
+    o.set_nuclei(R2)
+    p = g.get_params()
+    assert np.allclose(p, [1.5, 2.0, 0.5, 0.25])
+    p2 = p * 1.1
+    g.set_params(p2)
+    o.set_params(p2)
+    pos = _configs(o, 20)
+    assert _close(g.evaluate_batch(pos)["eloc"], o.batch_eval(pos)["eloc"], RTOL64)
