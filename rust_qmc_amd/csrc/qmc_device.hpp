@@ -287,6 +287,112 @@ __device__ __forceinline__ void mo_grad_lap(const DevSys<T>& S, int spin, T x, T
   }
 }
 
+// Determinant terms of one electron from AO derivatives contracted with the
+// inverse column: w_ao = sum_m C[spin][m][ao] ainv[m][0], then
+// grad D / D = sum_ao w_ao grad chi_ao and lap D / D = sum_ao w_ao lap chi_ao.
+// on_shell(center, sx, sy, sz) receives each shell's sum_ao w_ao grad chi_ao
+// (minus d ln|D| / dR_center for this electron).
+template <int NS, class T, class OnShell>
+__device__ __forceinline__ void det_terms(const DevSys<T>& S, int spin, T x, T y, T z,
+                                          const T ai[Cap<NS>::v][Cap<NS>::v], T& gdx, T& gdy, T& gdz, T& ld,
+                                          OnShell on_shell) {
+  gdx = gdy = gdz = ld = T(0);
+  auto wgt = [&](int ao) {
+    T w = 0;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) w += S.moT[spin][ao][m] * ai[m][0];
+    return w;
+  };
+  for (int sh = 0; sh < S.n_shell; ++sh) {
+    const int c = S.sh_center[sh], kind = S.sh_kind[sh], p0 = S.sh_prim0[sh], np = S.sh_nprim[sh];
+    const int a0 = S.sh_ao0[sh];
+    T dx = x - S.nuc[c][0], dy = y - S.nuc[c][1], dz = z - S.nuc[c][2];
+    T r2 = dx * dx + dy * dy + dz * dz;
+    T sx = 0, sy = 0, sz = 0, sl = 0;
+    if (kind <= SH_P1) {
+      T rs = 0, rsa = 0, rsaa = 0, rp = 0, rpa = 0, rpaa = 0;
+      for (int k = 0; k < np; ++k) {
+        const T a = S.prim_exp[p0 + k];
+        T e = M<T>::ex(-a * r2);
+        T cs = S.coef_s[p0 + k] * e, cp = S.coef_p[p0 + k] * e;
+        rs += cs;
+        rsa += cs * a;
+        rsaa += cs * a * a;
+        rp += cp;
+        rpa += cp * a;
+        rpaa += cp * a * a;
+      }
+      if (kind == SH_S || kind == SH_SP) {
+        const T w = wgt(a0);
+        const T g = T(-2) * rsa * w;
+        sx += g * dx;
+        sy += g * dy;
+        sz += g * dz;
+        sl += w * (T(-6) * rsa + T(4) * r2 * rsaa);
+      }
+      if (kind != SH_S) {
+        // p_m = d_m R: grad_j = delta_jm R - 2 d_m Ra d_j; lap = d_m (-10 Ra + 4 r^2 Raa)
+        const T lr = T(-10) * rpa + T(4) * r2 * rpaa;
+        T wx, wy, wz;
+        if (kind == SH_P1) {
+          const int m = S.sh_m[sh];
+          const T w = wgt(a0);
+          wx = m == 0 ? w : T(0);
+          wy = m == 1 ? w : T(0);
+          wz = m == 2 ? w : T(0);
+        } else {
+          const int b = (kind == SH_SP) ? a0 + 1 : a0;
+          wx = wgt(b);
+          wy = wgt(b + 1);
+          wz = wgt(b + 2);
+        }
+        const T wd = wx * dx + wy * dy + wz * dz;
+        const T g = T(-2) * rpa * wd;
+        sx += g * dx + wx * rp;
+        sy += g * dy + wy * rp;
+        sz += g * dz + wz * rp;
+        sl += wd * lr;
+      }
+    } else {
+      T s = M<T>::sq(r2);
+      T d1 = 0, d2 = 0;
+      for (int k = 0; k < np; ++k) {
+        const T zt = S.prim_exp[p0 + k];
+        const int p = S.prim_pow[p0 + k];
+        const T pf = T(p);
+        T sp = T(1);
+        for (int q = 0; q < p; ++q) sp *= s;
+        T spm1 = p >= 1 ? sp / s : T(0);
+        T spm2 = p >= 2 ? spm1 / s : T(0);
+        T e = S.coef_s[p0 + k] * M<T>::ex(-zt * s);
+        d1 += e * (-zt * sp + pf * spm1);
+        d2 += e * (zt * zt * sp - T(2) * zt * pf * spm1 + pf * (pf - T(1)) * spm2);
+      }
+      const T w = wgt(a0);
+      if (s == T(0)) {
+        sl = (kind == SH_SLATER1S) ? (w == T(0) ? T(0) : -INFINITY * w) : T(0);
+      } else {
+        T is = T(1) / s;
+        T g = w * d1 * is;
+        sx = g * dx;
+        sy = g * dy;
+        sz = g * dz;
+        sl = w * (d2 + T(2) * d1 * is);
+      }
+    }
+    gdx += sx;
+    gdy += sy;
+    gdz += sz;
+    ld += sl;
+    on_shell(c, sx, sy, sz);
+  }
+}
+
+struct NoShell {
+  template <class T>
+  __device__ __forceinline__ void operator()(int, T, T, T) const {}
+};
+
 // ---------------------------------------------------------------------------
 // Jastrow pair functions (jastrow.rs:218-321 and the Pade series):
 //   u_ee = a x + cst + sum_k c_k x^(k+2),  x = r/(1+b r)

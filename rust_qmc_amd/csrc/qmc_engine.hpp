@@ -40,17 +40,8 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
                                                double& pot) {
   const T x = ra[0][0], y = ra[0][1], z = ra[0][2];
   // determinant part: grad D / D and lap D / D from the inverse column
-  T g[Cap<NA>::v][3], l[Cap<NA>::v];
-  mo_grad_lap<NA>(S, spin, x, y, z, g, l);
-  T gdx = 0, gdy = 0, gdz = 0, ld = 0;
-#pragma unroll
-  for (int m = 0; m < NA; ++m) {
-    const T c = ai[m][0];
-    gdx += g[m][0] * c;
-    gdy += g[m][1] * c;
-    gdz += g[m][2] * c;
-    ld += l[m] * c;
-  }
+  T gdx, gdy, gdz, ld;
+  det_terms<NA>(S, spin, x, y, z, ai, gdx, gdy, gdz, ld, NoShell{});
   // Jastrow part
   T gux = 0, guy = 0, guz = 0, lu = 0, pairsq = 0;
   double pe = 0.0;
