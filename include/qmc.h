@@ -173,6 +173,24 @@ int qmc_local_energy(const qmc_system* sys, int64_t W, const double* pos, double
 int qmc_evaluate(const qmc_system* sys, int64_t W, const double* pos, double* logabs, double* sign,
                  double* grad, double* lap, double* eloc, void* stream);
 
+/* ---- per-configuration derivatives (any output may be NULL) ----
+ * O   [W][P]    OptimizableWfn::log_derivatives (traits.rs:114; methane.rs:1212-1219)
+ * dE  [W][P]    d E_L / d p_k (no reference counterpart; pinned by finite differences)
+ * F   [W][M][3] ForceCalculator::hellmann_feynman_force (sampling/traits.rs:38; methane.rs:1241-1272)
+ * Om  [W][M][3] ForceCalculator::wfn_nuclear_gradient (sampling/traits.rs:50; methane.rs:1096-1101)
+ * P = qmc_num_params, M = number of nuclei. */
+int qmc_derivatives(const qmc_system* sys, int64_t W, const double* pos, double* eloc, double* O, double* dE,
+                    double* F, double* Om, void* stream);
+int qmc_log_derivs(const qmc_system* sys, int64_t W, const double* pos, double* O, void* stream);
+int qmc_dEL_dp(const qmc_system* sys, int64_t W, const double* pos, double* dE, void* stream);
+int qmc_hf_force(const qmc_system* sys, int64_t W, const double* pos, double* F, void* stream);
+int qmc_nuc_grad_log(const qmc_system* sys, int64_t W, const double* pos, double* Om, void* stream);
+
+/* (S + eps I) dp = -f by LU in FP64, falling back to dp = -f when the
+ * matrix is singular (SROptimizer::compute_sr_update, sr_optimize.rs:219-242).
+ * Returns QMC_OK, or QMC_ERR_SINGULAR when the fallback was taken. */
+int qmc_sr_solve(int32_t n, const double* S, const double* f, double eps, double* dp);
+
 /* ---- walker ensembles resident in HBM ---- */
 typedef struct qmc_ensemble qmc_ensemble;
 
@@ -211,6 +229,67 @@ int qmc_ensemble_sweeps(qmc_ensemble* ens, const qmc_sweep_params* p, double* e_
  * last qmc_ensemble_sweeps call. */
 int qmc_ensemble_set_timing(qmc_ensemble* ens, int32_t on);
 int qmc_ensemble_timing(const qmc_ensemble* ens, double* sweep_kernel_ms, int64_t* launches);
+
+/* ---- SR-grad samples and force moments over an ensemble ----
+ * qmc_ensemble_measure evaluates, for every walker of the ensemble, E_L,
+ * O = [O_J (P), Omega (3M)], dE_L/dp (P) and the HF force (3M), and adds
+ * their first and second moments (one FP64 matrix-core Gram product) to
+ * the ensemble's accumulator; reset != 0 clears it first.
+ * qmc_ensemble_moments derives from the accumulator: */
+typedef struct {
+  int64_t n_samples;
+  int32_t P, M;      /* Jastrow parameters, nuclei; O has Q = P + 3M entries */
+  double energy;     /* <E_L> */
+  double variance;   /* <E_L^2> - <E_L>^2 */
+  double* o_mean;    /* [Q]   <O>          (caller buffers; NULL to skip) */
+  double* S;         /* [Q*Q] <O O^T> - <O><O>^T   (sr_optimize.rs:203-217) */
+  double* f;         /* [Q]   <E_L O> - <E_L><O>    (sr_optimize.rs:224-234) */
+  double* de_mean;   /* [P]   <dE_L/dp> */
+  double* f_hf;      /* [M*3] <F_HF> with the nucleus-nucleus term (sign per QMC_FLAG_HF_NN_REFSIGN) */
+  double* f_zv;      /* [M*3] <F_HF> - 2(<E_L Omega> - <E_L><Omega>)  (geometry_opt.rs:204-213) */
+  double* f_var;     /* [M]   <|F_HF|^2> - |<F_HF>|^2  (geometry_opt.rs:220-224) */
+} qmc_moments;
+int qmc_ensemble_measure(qmc_ensemble* ens, int32_t reset);
+int qmc_ensemble_moments(const qmc_ensemble* ens, qmc_moments* out);
+
+/* SROptimizer (sr_optimize.rs:21-40) */
+typedef struct {
+  int64_t n_samples;       /* samples per iteration (walkers x measurement rounds) */
+  int64_t n_walkers;
+  int64_t n_equilibrate;   /* sweeps before the first iteration */
+  int64_t max_iterations;
+  double learning_rate;
+  double sr_epsilon;
+  double tolerance;
+  double step_size;        /* Metropolis step (bohr) */
+  int32_t decorrelation;   /* sweeps between measurements (reference: 5, sr_optimize.rs:145) */
+  int32_t move;            /* QMC_MOVE_ALL as the reference, or QMC_MOVE_SINGLE */
+  int32_t include_nuclear; /* 1: solve the (P+3M) system with Omega appended to O */
+  int32_t refresh_interval;
+} qmc_sr_params;
+/* SRResult (sr_optimize.rs:351-366) */
+typedef struct {
+  double final_energy;
+  double final_variance;
+  int32_t iterations;
+} qmc_sr_results;
+/* SROptimizer::optimize (sr_optimize.rs:252-348); histories (host, may be
+ * NULL) hold max_iterations entries, param_history (max_iterations+1)*P. */
+int qmc_sr_optimize(qmc_system* sys, const qmc_sr_params* p, uint64_t seed, qmc_sr_results* res,
+                    double* energy_history, double* variance_history, double* param_history);
+
+/* GeometryOptimizer::sample_forces (geometry_opt.rs:105-227): forces [M*3]
+ * (bare HF or zero-variance), mean energy, its error, force variance [M]. */
+typedef struct {
+  int64_t n_samples, n_walkers, n_equilibrate;
+  double mc_step_size;
+  int32_t zero_variance; /* ForceEstimator::ZeroVariance (force_variance.rs:27-39) */
+  int32_t decorrelation;
+  int32_t move;
+  int32_t refresh_interval;
+} qmc_force_params;
+int qmc_sample_forces(const qmc_system* sys, const qmc_force_params* p, uint64_t seed, double* forces,
+                      double* mean_energy, double* error, double* force_variance);
 
 /* MCMCParams (vmc.rs:12-21) plus the options the reference hard-wires */
 typedef struct {

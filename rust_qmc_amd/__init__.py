@@ -197,6 +197,27 @@ class System:
     def local_energy(self, r):
         return self.evaluate_batch(_f64(r)[None], ("eloc",))["eloc"][0]
 
+    # derivatives (OptimizableWfn::log_derivatives, ForceCalculator) -----------
+    def derivatives_batch(self, pos):
+        """Per configuration: eloc [W], O [W][P], dE/dp [W][P], HF force
+        [W][M][3] and d ln psi / dR [W][M][3]."""
+        pos = _f64(pos)
+        W = pos.shape[0]
+        P, M = self.num_params(), self.n_nuc
+        el, O, dE = np.zeros(W), np.zeros((W, P)), np.zeros((W, P))
+        F, Om = np.zeros((W, M, 3)), np.zeros((W, M, 3))
+        check(lib().qmc_derivatives(self._h, W, _ptr(pos), _ptr(el), _ptr(O), _ptr(dE), _ptr(F), _ptr(Om), None))
+        return dict(eloc=el, O=O, dE=dE, F=F, Om=Om)
+
+    def log_derivatives(self, r):
+        return self.derivatives_batch(_f64(r)[None])["O"][0]
+
+    def hellmann_feynman_force(self, r):
+        return self.derivatives_batch(_f64(r)[None])["F"][0]
+
+    def wfn_nuclear_gradient(self, r):
+        return self.derivatives_batch(_f64(r)[None])["Om"][0]
+
     def initialize(self, seed=0, walker=0):
         """MultiWfn::initialize with a Philox stream (walker `walker`)."""
         ens = Ensemble(self, walker + 1, seed)
@@ -360,6 +381,22 @@ class Ensemble:
     def refresh(self):
         check(lib().qmc_ensemble_refresh(self._h))
 
+    def measure(self, reset=False):
+        """Accumulate one SR-grad sample of every walker."""
+        check(lib().qmc_ensemble_measure(self._h, int(reset)))
+
+    def moments(self):
+        P, M = self.system.num_params(), self.system.n_nuc
+        Q = P + 3 * M
+        out = dict(o_mean=np.zeros(Q), S=np.zeros((Q, Q)), f=np.zeros(Q), de_mean=np.zeros(max(P, 1)),
+                   f_hf=np.zeros((M, 3)), f_zv=np.zeros((M, 3)), f_var=np.zeros(M))
+        m = _lib.QmcMoments(0, 0, 0, 0.0, 0.0, *[_ptr(out[k]) for k in
+                                                 ("o_mean", "S", "f", "de_mean", "f_hf", "f_zv", "f_var")])
+        check(lib().qmc_ensemble_moments(self._h, C.byref(m)))
+        out["de_mean"] = out["de_mean"][:P]
+        out.update(n_samples=m.n_samples, energy=m.energy, variance=m.variance, P=P, M=M)
+        return out
+
     def set_timing(self, on=True):
         check(lib().qmc_ensemble_set_timing(self._h, int(on)))
 
@@ -422,6 +459,83 @@ class MCMCSimulation:
         check(lib().qmc_vmc_run(self.wavefunction._h, C.byref(cp), self.seed, C.byref(res), _ptr(tr)))
         self.energies = tr
         return MCMCResults(res.energy, res.error, res.autocorrelation_time, res.acceptance, res.final_step_size)
+
+
+def sr_solve(S, f, eps):
+    """(S + eps I) dp = -f (sr_optimize.rs:219-242); returns (dp, singular)."""
+    S = _f64(S)
+    f = _f64(f)
+    dp = np.zeros(f.size)
+    rc = lib().qmc_sr_solve(f.size, _ptr(S), _ptr(f), eps, _ptr(dp))
+    return dp, rc == _lib.QMC_ERR_SINGULAR
+
+
+@dataclass
+class SRResult:
+    """sr_optimize.rs:351-366"""
+    final_params: np.ndarray
+    final_energy: float
+    final_variance: float
+    energy_history: np.ndarray
+    variance_history: np.ndarray
+    param_history: np.ndarray
+
+
+@dataclass
+class SROptimizer:
+    """sr_optimize.rs:21-56 (defaults as SROptimizer::default)"""
+    n_samples: int = 5000
+    n_walkers: int = 20
+    n_equilibrate: int = 500
+    max_iterations: int = 50
+    learning_rate: float = 0.05
+    sr_epsilon: float = 0.001
+    tolerance: float = 1e-5
+    step_size: float = 0.5
+    decorrelation: int = 5
+    move: int = QMC_MOVE_ALL
+    include_nuclear: bool = False
+    refresh_interval: int = 64
+    seed: int = 0
+
+    def optimize(self, wfn: System) -> SRResult:
+        """sr_optimize.rs:252-348"""
+        P = wfn.num_params()
+        p = _lib.QmcSrParams(self.n_samples, self.n_walkers, self.n_equilibrate, self.max_iterations,
+                             self.learning_rate, self.sr_epsilon, self.tolerance, self.step_size, self.decorrelation,
+                             self.move, int(self.include_nuclear), self.refresh_interval)
+        res = _lib.QmcSrResults()
+        eh = np.zeros(self.max_iterations)
+        vh = np.zeros(self.max_iterations)
+        ph = np.zeros((self.max_iterations + 1, P))
+        check(lib().qmc_sr_optimize(wfn._h, C.byref(p), self.seed, C.byref(res), _ptr(eh), _ptr(vh), _ptr(ph)))
+        n = res.iterations
+        return SRResult(wfn.get_params(), res.final_energy, res.final_variance, eh[:n], vh[:n], ph[: n + 1])
+
+
+@dataclass
+class GeometryOptimizer:
+    """Sampling settings of geometry_opt.rs:16-53 (sample_forces only)."""
+    n_samples: int = 5000
+    n_walkers: int = 20
+    n_equilibrate: int = 500
+    mc_step_size: float = 0.5
+    zero_variance: bool = False
+    decorrelation: int = 5
+    move: int = QMC_MOVE_ALL
+    refresh_interval: int = 64
+    seed: int = 0
+
+    def sample_forces(self, wfn: System):
+        """geometry_opt.rs:105-227: (mean_forces [M][3], mean_energy, error, force_variance [M])."""
+        p = _lib.QmcForceParams(self.n_samples, self.n_walkers, self.n_equilibrate, self.mc_step_size,
+                                int(self.zero_variance), self.decorrelation, self.move, self.refresh_interval)
+        M = wfn.n_nuc
+        F = np.zeros((M, 3))
+        fv = np.zeros(M)
+        me, err = C.c_double(), C.c_double()
+        check(lib().qmc_sample_forces(wfn._h, C.byref(p), self.seed, _ptr(F), C.byref(me), C.byref(err), _ptr(fv)))
+        return F, me.value, err.value, fv
 
 
 def autocorrelation_time(e):

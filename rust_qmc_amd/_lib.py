@@ -40,7 +40,9 @@ EXPORTED = [
     "qmc_eval_log_psi", "qmc_grad_log_psi", "qmc_lap_over_psi", "qmc_local_energy", "qmc_evaluate",
     "qmc_ensemble_create", "qmc_ensemble_destroy", "qmc_ensemble_get_positions", "qmc_ensemble_set_positions",
     "qmc_ensemble_get_state", "qmc_ensemble_refresh", "qmc_ensemble_sweeps", "qmc_ensemble_set_timing",
-    "qmc_ensemble_timing", "qmc_vmc_run",
+    "qmc_ensemble_timing", "qmc_vmc_run", "qmc_derivatives", "qmc_log_derivs", "qmc_dEL_dp", "qmc_hf_force",
+    "qmc_nuc_grad_log", "qmc_sr_solve", "qmc_ensemble_measure", "qmc_ensemble_moments", "qmc_sr_optimize",
+    "qmc_sample_forces",
     "qmc_autocorrelation_time", "qmc_blocking_error", "qmc_last_error", "qmc_version",
 ]
 
@@ -77,6 +79,27 @@ class QmcMcmcParams(C.Structure):
     _fields_ = [("n_walkers", _i64), ("n_steps", _i64), ("initial_step_size", _d), ("max_step_size", _d),
                 ("min_step_size", _d), ("target_acceptance", _d), ("adaptation_interval", _i64),
                 ("n_burnin", _i64), ("move", _i32), ("refresh_interval", _i32)]
+
+
+class QmcMoments(C.Structure):
+    _fields_ = [("n_samples", _i64), ("P", _i32), ("M", _i32), ("energy", _d), ("variance", _d),
+                ("o_mean", _p), ("S", _p), ("f", _p), ("de_mean", _p), ("f_hf", _p), ("f_zv", _p),
+                ("f_var", _p)]
+
+
+class QmcSrParams(C.Structure):
+    _fields_ = [("n_samples", _i64), ("n_walkers", _i64), ("n_equilibrate", _i64), ("max_iterations", _i64),
+                ("learning_rate", _d), ("sr_epsilon", _d), ("tolerance", _d), ("step_size", _d),
+                ("decorrelation", _i32), ("move", _i32), ("include_nuclear", _i32), ("refresh_interval", _i32)]
+
+
+class QmcSrResults(C.Structure):
+    _fields_ = [("final_energy", _d), ("final_variance", _d), ("iterations", _i32)]
+
+
+class QmcForceParams(C.Structure):
+    _fields_ = [("n_samples", _i64), ("n_walkers", _i64), ("n_equilibrate", _i64), ("mc_step_size", _d),
+                ("zero_variance", _i32), ("decorrelation", _i32), ("move", _i32), ("refresh_interval", _i32)]
 
 
 class QmcVmcResults(C.Structure):
@@ -118,6 +141,16 @@ _SIGS = {
     "qmc_ensemble_set_timing": (_i32, [_p, _i32]),
     "qmc_ensemble_timing": (_i32, [_p, _p, _p]),
     "qmc_vmc_run": (_i32, [_p, _p, _u64, _p, _p]),
+    "qmc_derivatives": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p, _p]),
+    "qmc_log_derivs": (_i32, [_p, _i64, _p, _p, _p]),
+    "qmc_dEL_dp": (_i32, [_p, _i64, _p, _p, _p]),
+    "qmc_hf_force": (_i32, [_p, _i64, _p, _p, _p]),
+    "qmc_nuc_grad_log": (_i32, [_p, _i64, _p, _p, _p]),
+    "qmc_sr_solve": (_i32, [_i32, _p, _p, _d, _p]),
+    "qmc_ensemble_measure": (_i32, [_p, _i32]),
+    "qmc_ensemble_moments": (_i32, [_p, _p]),
+    "qmc_sr_optimize": (_i32, [_p, _p, _u64, _p, _p, _p, _p]),
+    "qmc_sample_forces": (_i32, [_p, _p, _u64, _p, _p, _p, _p]),
     "qmc_autocorrelation_time": (_d, [_p, _i64]),
     "qmc_blocking_error": (_d, [_p, _i64, _d]),
     "qmc_last_error": (C.c_char_p, []),

@@ -1,0 +1,379 @@
+// qmc_derivs.hip -- per-walker derivatives for stochastic reconfiguration
+// and forces, and their ensemble moments.
+//
+// derivs_kernel writes, for every walker, one column of the feature matrix
+// Y [Qp][ld] (feature-major, walker fastest):
+//   O_k = d ln psi / d p_k        (Jastrow parameters, methane.rs:1212-1219
+//                                  plus the Pade series)          cols [0, P)
+//   Omega_I = d ln psi / d R_I    (methane.rs:990-1101)             [P, P+3M)
+//   E_L                                                              P+3M
+//   1                                                                P+3M+1
+//   d E_L / d p_k                                                    [P+3M+2, 2P+3M+2)
+//   F_I electron part of hellmann_feynman_force (methane.rs:1247-1257)
+//                                                                    [2P+3M+2, 2P+6M+2)
+// gram_kernel then forms G = Y Y^T with FP64 MFMA, which holds every mean,
+// covariance and cross moment SR (sr_optimize.rs:176-246) and the ZV force
+// estimator (geometry_opt.rs:174-226) need.
+#include "qmc_derivs.hpp"
+
+#include "qmc_engine.hpp"
+
+namespace qmc {
+
+struct DerivLayout {
+  int P, M, N, Qp;
+  __device__ __forceinline__ int o(int k) const { return k; }
+  __device__ __forceinline__ int om(int I, int c) const { return P + 3 * I + c; }
+  __device__ __forceinline__ int e() const { return P + 3 * M; }
+  __device__ __forceinline__ int one() const { return P + 3 * M + 1; }
+  __device__ __forceinline__ int de(int k) const { return P + 3 * M + 2 + k; }
+  __device__ __forceinline__ int f(int I, int c) const { return 2 * P + 3 * M + 2 + 3 * I + c; }
+  __device__ __forceinline__ int g(int q, int c) const { return Qp + 3 * q + c; }
+};
+
+// per-thread accumulator rows in LDS: element q of lane t at lds[q*64 + t]
+struct Row {
+  double* base;
+  __device__ __forceinline__ double& operator[](int q) const { return base[q * 64]; }
+};
+
+// Pass 1 for the electron in slot 0 of sector A (combined index q):
+// determinant and Jastrow gradients, E_L terms, Omega, HF force.
+template <int NA, int NB, class T>
+__device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, int q, const T ra[Cap<NA>::v][3],
+                                                const T ai[Cap<NA>::v][Cap<NA>::v], const T rb[Cap<NB>::v][3],
+                                                const DerivLayout& L, Row row, double& pot) {
+  const T x = ra[0][0], y = ra[0][1], z = ra[0][2];
+  T gdx, gdy, gdz, ld;
+  det_terms<NA>(S, spin, x, y, z, ai, gdx, gdy, gdz, ld, [&](int c, T sx, T sy, T sz) {
+    row[L.om(c, 0)] -= (double)sx;
+    row[L.om(c, 1)] -= (double)sy;
+    row[L.om(c, 2)] -= (double)sz;
+  });
+  T gux = 0, guy = 0, guz = 0, lu = 0, pairsq = 0;
+  double pe = 0.0;
+  auto ee = [&](T px, T py, T pz, bool same) {
+    T dx = x - px, dy = y - py, dz = z - pz;
+    T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+    T ir = M<T>::rcp(r);
+    T u1, lp;
+    uee_d(S, r, ir, same, u1, lp);
+    if (r >= T(1e-10)) {
+      T f = u1 * ir;
+      gux += f * dx;
+      guy += f * dy;
+      guz += f * dz;
+      lu += lp;
+      pairsq += u1 * u1;
+    }
+    pe += 0.5 * (double)ir;
+  };
+#pragma unroll
+  for (int j = 1; j < NA; ++j) ee(ra[j][0], ra[j][1], ra[j][2], true);
+#pragma unroll
+  for (int j = 0; j < NB; ++j) ee(rb[j][0], rb[j][1], rb[j][2], false);
+  for (int I = 0; I < S.n_nuc; ++I) {
+    T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+    T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+    T ir = M<T>::rcp(r);
+    pe -= (double)(S.Z[I] * ir);
+    if (r > T(1e-10)) {  // methane.rs:1253
+      const double f3 = (double)(S.Z[I] * ir * ir * ir);
+      row[L.f(I, 0)] += f3 * (double)dx;
+      row[L.f(I, 1)] += f3 * (double)dy;
+      row[L.f(I, 2)] += f3 * (double)dz;
+    }
+    if (S.en_on) {
+      T u1, lp;
+      uen_d(S, r, ir, I, u1, lp);
+      if (r >= T(1e-10)) {
+        T f = u1 * ir;
+        gux += f * dx;
+        guy += f * dy;
+        guz += f * dz;
+        lu += lp;
+        // d u_en / d R_I = -u1 (r_i - R_I)/r   (methane.rs:1071-1090)
+        row[L.om(I, 0)] -= (double)(f * dx);
+        row[L.om(I, 1)] -= (double)(f * dy);
+        row[L.om(I, 2)] -= (double)(f * dz);
+      }
+    }
+  }
+  const T gs = S.grad_scale;
+  T gjx = gs * gux, gjy = gs * guy, gjz = gs * guz;
+  T lj = (S.flags & F_J2_LAP_QUIRK) ? (lu + pairsq) : (lu + gux * gux + guy * guy + guz * guz);
+  const T lap = ld + lj + T(2) * (gdx * gjx + gdy * gjy + gdz * gjz);
+  row[L.g(q, 0)] = (double)(gdx + gux);
+  row[L.g(q, 1)] = (double)(gdy + guy);
+  row[L.g(q, 2)] = (double)(gdz + guz);
+  pot += pe;
+  return -0.5 * (double)lap;
+}
+
+// Pass 2: unique pairs -> O_k and d E_L / d p_k (Pade Jastrow only)
+template <int NU, int ND, class T>
+__device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>::v][3],
+                                          const T rd[Cap<ND>::v][3], const DerivLayout& L, Row row) {
+  constexpr int N = NU + ND;
+  auto pos = [&](int q, int c) -> T { return q < NU ? ru[q][c] : rd[q - NU][c]; };
+  const int K = S.k_ee;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) {
+      const bool same = (i < NU) == (j < NU);
+      T dx = pos(i, 0) - pos(j, 0), dy = pos(i, 1) - pos(j, 1), dz = pos(i, 2) - pos(j, 2);
+      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+      const T a = same ? S.a_para : S.a_anti;
+      const T* c = same ? S.c_para : S.c_anti;
+      PairDerivs<T> pd;
+      pd.init(r, S.b_ee, a, K, c);
+      const int ia = same ? 3 : 2, c0 = 4 + (same ? K : 0);
+      // O: d u / d b_ee = -(a + s') x^2, d u / d a = x, d u / d c_k = x^(k+2)
+      row[L.o(0)] += (double)(-pd.A1 * pd.x * pd.x);
+      row[L.o(ia)] += (double)pd.x;
+      if (r < T(1e-10)) continue;
+      const T ir = T(1) / r;
+      // Delta = (G_i - G_j) . rhat_ij
+      const T dlt = (T)((row[L.g(i, 0)] - row[L.g(j, 0)]) * (double)(dx * ir) +
+                        (row[L.g(i, 1)] - row[L.g(j, 1)]) * (double)(dy * ir) +
+                        (row[L.g(i, 2)] - row[L.g(j, 2)]) * (double)(dz * ir));
+      T du1, dl;
+      pd.d_b(r, S.b_ee, du1, dl);
+      row[L.de(0)] += -(double)(dl + dlt * du1);
+      pd.d_a(r, S.b_ee, du1, dl);
+      row[L.de(ia)] += -(double)(dl + dlt * du1);
+      T xk = T(1);
+      for (int k = 0; k < K; ++k) {
+        row[L.o(c0 + k)] += (double)(xk * pd.x * pd.x);
+        pd.d_c(k, r, S.b_ee, xk, du1, dl);
+        row[L.de(c0 + k)] += -(double)(dl + dlt * du1);
+        xk *= pd.x;
+      }
+    }
+  const int Ke = S.k_en;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    for (int I = 0; I < S.n_nuc; ++I) {
+      T dx = pos(i, 0) - S.nuc[I][0], dy = pos(i, 1) - S.nuc[I][1], dz = pos(i, 2) - S.nuc[I][2];
+      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+      const int sp = S.species[I];
+      const T* c = S.d_en[sp];
+      PairDerivs<T> pd;
+      pd.init(r, S.b_en, -S.Z[I], Ke, c);
+      // d u_en / d b_en = (Z - s') y^2 = -A1 y^2
+      row[L.o(1)] += (double)(-pd.A1 * pd.x * pd.x);
+      const int c0 = 4 + 2 * S.k_ee + sp * Ke;
+      if (r < T(1e-10)) {
+        T xk = T(1);
+        for (int k = 0; k < Ke; ++k) {
+          row[L.o(c0 + k)] += (double)(xk * pd.x * pd.x);
+          xk *= pd.x;
+        }
+        continue;
+      }
+      const T ir = T(1) / r;
+      const T dlt = (T)(row[L.g(i, 0)] * (double)(dx * ir) + row[L.g(i, 1)] * (double)(dy * ir) +
+                        row[L.g(i, 2)] * (double)(dz * ir));
+      T du1, dl;
+      pd.d_b(r, S.b_en, du1, dl);
+      row[L.de(1)] += -(double)(T(0.5) * dl + dlt * du1);
+      T xk = T(1);
+      for (int k = 0; k < Ke; ++k) {
+        row[L.o(c0 + k)] += (double)(xk * pd.x * pd.x);
+        pd.d_c(k, r, S.b_en, xk, du1, dl);
+        row[L.de(c0 + k)] += -(double)(T(0.5) * dl + dlt * du1);
+        xk *= pd.x;
+      }
+    }
+}
+
+template <int NU, int ND, class T>
+__global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __restrict__ Sd,
+                                                    const DevSys<T>* __restrict__ St, DerivArgs A) {
+  extern __shared__ double lds[];
+  const DevSys<T>& S = *St;
+  const int64_t w = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const bool live = w < A.W;
+  const int64_t wl = live ? w : A.W - 1;
+  DerivLayout L{A.P, A.M, NU + ND, A.Qp};
+  Row row{lds + threadIdx.x};
+  const int nq = A.Qp + 3 * (NU + ND);
+  for (int q = 0; q < nq; ++q) row[q] = 0.0;
+  // positions in FP64, from AoS input or from the ensemble's SoA state
+  WalkerRegs<NU, ND, double> D;
+  constexpr int N = NU + ND;
+#pragma unroll
+  for (int k = 0; k < NU; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int e = Sd->sector_elec[0][k];
+      D.ru[k][c] = A.pos_aos ? A.pos_aos[(wl * N + e) * 3 + c]
+                             : (A.pos64 ? A.pos64[(e * 3 + c) * A.ld + wl] : (double)A.pos32[(e * 3 + c) * A.ld + wl]);
+    }
+#pragma unroll
+  for (int k = 0; k < ND; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int e = Sd->sector_elec[1][k];
+      D.rd[k][c] = A.pos_aos ? A.pos_aos[(wl * N + e) * 3 + c]
+                             : (A.pos64 ? A.pos64[(e * 3 + c) * A.ld + wl] : (double)A.pos32[(e * 3 + c) * A.ld + wl]);
+    }
+  double lu, su, ldn, sd;
+  sector_inverse<NU>(*Sd, 0, D.ru, D.au, lu, su);
+  sector_inverse<ND>(*Sd, 1, D.rd, D.ad, ldn, sd);
+  WalkerRegs<NU, ND, T> R;
+#pragma unroll
+  for (int k = 0; k < Cap<NU>::v; ++k) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R.ru[k][c] = (T)D.ru[k][c];
+#pragma unroll
+    for (int m = 0; m < Cap<NU>::v; ++m) R.au[k][m] = (T)D.au[k][m];
+  }
+#pragma unroll
+  for (int k = 0; k < Cap<ND>::v; ++k) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R.rd[k][c] = (T)D.rd[k][c];
+#pragma unroll
+    for (int m = 0; m < Cap<ND>::v; ++m) R.ad[k][m] = (T)D.ad[k][m];
+  }
+  double kin = 0.0, pot = 0.0;
+#pragma unroll 1
+  for (int k = 0; k < NU; ++k) {
+    kin += electron_pass<NU, ND>(S, 0, k, R.ru, R.au, R.rd, L, row, pot);
+    rotate<NU>(R.ru, R.au);
+  }
+#pragma unroll 1
+  for (int k = 0; k < ND; ++k) {
+    kin += electron_pass<ND, NU>(S, 1, NU + k, R.rd, R.ad, R.ru, L, row, pot);
+    rotate<ND>(R.rd, R.ad);
+  }
+  row[L.e()] = kin + pot + S.vnn;
+  row[L.one()] = 1.0;
+  if (S.jas_kind == JAS_PADE) pair_pass<NU, ND>(S, R.ru, R.rd, L, row);
+  if (live)
+    for (int q = 0; q < A.Qp; ++q) A.Y[(int64_t)q * A.ldy + w] = row[q];
+}
+
+// ---------------------------------------------------------------------------
+// G = Y Y^T over a walker chunk per wave with v_mfma_f64_16x16x4_f64.
+// Fragment convention (16x16x4 f64): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator register r of lane l is D[4(l/16)+r][l%16].
+// A tile (I, J) pairs feature block I (rows) with block J (columns); both
+// fragments are "feature (l%16) of walker k0 + l/16", so one load per block
+// serves every tile.  Only I <= J is formed; the host mirrors.
+// ---------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int NB>
+__global__ void __launch_bounds__(64) gram_kernel(const double* __restrict__ Y, int64_t ldy, int64_t W, int chunk,
+                                                  double* __restrict__ partial) {
+  const int lane = threadIdx.x;
+  const int64_t w0 = (int64_t)blockIdx.x * chunk;
+  const int64_t w1 = w0 + chunk < W ? w0 + chunk : W;
+  constexpr int NT = NB * (NB + 1) / 2;
+  dbl4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+  const int fr = lane & 15, kk = lane >> 4;
+  for (int64_t k0 = w0; k0 < w1; k0 += 4) {
+    const int64_t wk = k0 + kk;
+    double f[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) f[b] = wk < w1 ? Y[(int64_t)(16 * b + fr) * ldy + wk] : 0.0;
+    int t = 0;
+#pragma unroll
+    for (int I = 0; I < NB; ++I)
+#pragma unroll
+      for (int J = I; J < NB; ++J) {
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(f[I], f[J], acc[t], 0, 0, 0);
+        ++t;
+      }
+  }
+  // partial [block][Qp][Qp] (upper tiles only)
+  const int Qp = 16 * NB;
+  double* out = partial + (int64_t)blockIdx.x * Qp * Qp;
+  int t = 0;
+#pragma unroll
+  for (int I = 0; I < NB; ++I)
+#pragma unroll
+    for (int J = I; J < NB; ++J) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[(16 * I + 4 * kk + r) * Qp + 16 * J + fr] = acc[t][r];
+      ++t;
+    }
+}
+
+// deterministic sum of the per-chunk partial Grams (upper triangle)
+__global__ void __launch_bounds__(256) gram_reduce_kernel(const double* __restrict__ partial, int nblk, int Qp,
+                                                          double* __restrict__ G) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)Qp * Qp) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * Qp * Qp + i];
+  G[i] = s;
+}
+
+template <int NU, int ND, class T>
+static hipError_t launch_derivs(const void* Sd, const void* St, DerivArgs A, hipStream_t st) {
+  const size_t shm = sizeof(double) * 64 * (A.Qp + 3 * (NU + ND));
+  if (shm > 160 * 1024) return hipErrorInvalidValue;
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)derivs_kernel<NU, ND, T>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  derivs_kernel<NU, ND, T><<<(unsigned)((A.W + 63) / 64), 64, shm, st>>>((const DevSys<double>*)Sd,
+                                                                          (const DevSys<T>*)St, A);
+  return hipGetLastError();
+}
+
+template <int NU, int ND>
+static DerivOps make_dops() {
+  DerivOps o;
+  o.nu = NU;
+  o.nd = ND;
+  o.derivs64 = &launch_derivs<NU, ND, double>;
+  o.derivs32 = &launch_derivs<NU, ND, float>;
+  return o;
+}
+
+static const DerivOps kDOps[] = {
+    make_dops<1, 0>(), make_dops<1, 1>(), make_dops<2, 1>(), make_dops<2, 2>(), make_dops<3, 2>(),
+    make_dops<3, 3>(), make_dops<4, 4>(), make_dops<5, 5>(),
+};
+
+const DerivOps* find_derivs(int nu, int nd) {
+  for (const auto& o : kDOps)
+    if (o.nu == nu && o.nd == nd) return &o;
+  return nullptr;
+}
+
+hipError_t launch_gram(const double* Y, int64_t ldy, int64_t W, int Qp, double* partial, int chunk, double* G,
+                       hipStream_t st) {
+  const int nblk = (int)((W + chunk - 1) / chunk);
+  switch (Qp / 16) {
+#define QMC_GRAM_CASE(nb) \
+  case nb: gram_kernel<nb><<<nblk, 64, 0, st>>>(Y, ldy, W, chunk, partial); break;
+    QMC_GRAM_CASE(1)
+    QMC_GRAM_CASE(2)
+    QMC_GRAM_CASE(3)
+    QMC_GRAM_CASE(4)
+    QMC_GRAM_CASE(5)
+    QMC_GRAM_CASE(6)
+    QMC_GRAM_CASE(7)
+    QMC_GRAM_CASE(8)
+#undef QMC_GRAM_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t n = (int64_t)Qp * Qp;
+  gram_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(partial, nblk, Qp, G);
+  return hipGetLastError();
+}
+
+}  // namespace qmc

@@ -415,6 +415,63 @@ __device__ __forceinline__ void series(int K, const T* c, T x, T& s, T& s1, T& s
   s2 = v2;
 }
 
+// s'''(x) of the series, needed by d/db of the pair Laplacian
+template <class T>
+__device__ __forceinline__ T series3(int K, const T* c, T x) {
+  T v = 0, xp = T(1);  // x^(k-1) for k >= 1
+  for (int k = 1; k < K; ++k) {
+    const T n = T(k + 2);
+    v += c[k] * n * (n - T(1)) * (n - T(2)) * xp;
+    xp *= x;
+  }
+  return v;
+}
+
+// Parameter derivatives of one Pade pair term u = A x + s(x), x = r/(1+b r),
+// where A is a (e-e) or -Z (e-n) and s the power series with coefficients c.
+// Gives du'/dq and d(u''+2u'/r)/dq for q = b, A and each c_k; callers
+// combine them into d E_L / dq.
+template <class T>
+struct PairDerivs {
+  T x, iD, A1, s2, s3;
+  __device__ __forceinline__ void init(T r, T b, T A, int K, const T* c) {
+    T D = T(1) + b * r;
+    iD = T(1) / D;
+    x = r * iD;
+    T s, s1;
+    s1 = 0;
+    s2 = 0;
+    s3 = 0;
+    if (K > 0) {
+      series(K, c, x, s, s1, s2);
+      s3 = series3(K, c, x);
+    }
+    A1 = A + s1;
+  }
+  // d/db of u' and of lap = u'' + 2u'/r
+  __device__ __forceinline__ void d_b(T r, T b, T& du1, T& dlap) const {
+    const T iD2 = iD * iD, iD3 = iD2 * iD, iD4 = iD2 * iD2, x2 = x * x;
+    du1 = -x2 * s2 * iD2 - T(2) * r * A1 * iD3;
+    const T t1 = -x2 * s3 * iD4 - T(4) * r * s2 * iD4 * iD;
+    const T t2 = -T(2) * A1 * iD3 + T(2) * b * x2 * s2 * iD3 + T(6) * b * r * A1 * iD4;
+    dlap = t1 + t2 + T(2) / r * du1;
+  }
+  // d/dA (A enters linearly)
+  __device__ __forceinline__ void d_a(T r, T b, T& du1, T& dlap) const {
+    du1 = iD * iD;
+    dlap = -T(2) * b * iD * iD * iD + T(2) * du1 / r;
+  }
+  // d/dc_k for the term c_k x^(k+2)
+  __device__ __forceinline__ void d_c(int k, T r, T b, T xk, T& du1, T& dlap) const {
+    // xk = x^k
+    const T n = T(k + 2);
+    const T p1 = n * xk * x, p2 = n * (n - T(1)) * xk;
+    const T iD2 = iD * iD;
+    du1 = p1 * iD2;
+    dlap = p2 * iD2 * iD2 - T(2) * b * p1 * iD2 * iD + T(2) * du1 / r;
+  }
+};
+
 // value only (cst excluded: it cancels in every ratio and is added to ln J separately)
 template <class T>
 __device__ __forceinline__ T uee_val(const DevSys<T>& S, T r, bool same) {

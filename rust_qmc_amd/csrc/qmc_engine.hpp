@@ -10,6 +10,13 @@
 
 namespace qmc {
 
+// one walker's register-resident state
+template <int NU, int ND, class T>
+struct WalkerRegs {
+  T ru[Cap<NU>::v][3], rd[Cap<ND>::v][3];
+  T au[Cap<NU>::v][Cap<NU>::v], ad[Cap<ND>::v][Cap<ND>::v];
+};
+
 // rotate sector rows: slot k <- slot k+1, slot N-1 <- old slot 0
 template <int NA, class T>
 __device__ __forceinline__ void rotate(T ra[Cap<NA>::v][3], T ai[Cap<NA>::v][Cap<NA>::v]) {

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -14,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "qmc_derivs.hpp"
 #include "qmc_device.hpp"
 #include "qmc_kernels.hpp"
 
@@ -95,7 +97,9 @@ struct qmc_system {
   DevSys<double>* d64 = nullptr;  // device parameter blocks
   DevSys<float>* d32 = nullptr;
   const EngineOps* ops = nullptr;
+  const DerivOps* dops = nullptr;
   bool dirty = true;
+  uint64_t version = 0;  // bumped on every upload; ensembles refresh when it moves
   ~qmc_system() {
     if (d64) (void)hipFree(d64);
     if (d32) (void)hipFree(d32);
@@ -273,8 +277,78 @@ int upload(qmc_system* s) {
   HIP_TRY(hipMemcpy(s->d64, d64.get(), sizeof(DevSys<double>), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->d32, d32.get(), sizeof(DevSys<float>), hipMemcpyHostToDevice));
   s->ops = find_engine(s->h.n_up(), s->h.n_dn());
+  s->dops = find_derivs(s->h.n_up(), s->h.n_dn());
   s->dirty = false;
+  s->version++;
   return QMC_OK;
+}
+
+// feature layout of the derivative matrix Y (mirrors DerivLayout in qmc_derivs.hip)
+struct Cols {
+  int P, M, Qp;
+  explicit Cols(const HostSys& h) : P(h.n_params()), M(h.n_nuc()) { Qp = (2 * P + 6 * M + 2 + 15) / 16 * 16; }
+  int e() const { return P + 3 * M; }
+  int one() const { return P + 3 * M + 1; }
+  int de(int k) const { return P + 3 * M + 2 + k; }
+  int f(int I, int c) const { return 2 * P + 3 * M + 2 + 3 * I + c; }
+};
+
+// nucleus-nucleus part of the Hellmann-Feynman force: the physical
+// repulsion is +Z_I Z_J (R_I - R_J)/|R_I - R_J|^3; the reference subtracts
+// it (methane.rs:1259-1269), reproduced under QMC_FLAG_HF_NN_REFSIGN.
+std::vector<double> hf_nn(const HostSys& h) {
+  const int M = h.n_nuc();
+  std::vector<double> F(3 * M, 0.0);
+  const double sg = (h.flags & QMC_FLAG_HF_NN_REFSIGN) ? -1.0 : 1.0;
+  for (int I = 0; I < M; ++I)
+    for (int J = 0; J < M; ++J) {
+      if (I == J) continue;
+      double d[3], r2 = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        d[c] = h.nuc[3 * I + c] - h.nuc[3 * J + c];
+        r2 += d[c] * d[c];
+      }
+      const double r = std::sqrt(r2);
+      if (r > 1e-10)
+        for (int c = 0; c < 3; ++c) F[3 * I + c] += sg * h.Z[I] * h.Z[J] * d[c] / (r2 * r);
+    }
+  return F;
+}
+
+// copy a host vector to a caller buffer that may be host or device memory
+int put(double* dst, const double* src, size_t n);
+
+// LU with partial pivoting on a dense FP64 system (host)
+bool lu_solve(int n, std::vector<double> a, std::vector<double> b, std::vector<double>& x) {
+  std::vector<int> piv(n);
+  for (int i = 0; i < n; ++i) piv[i] = i;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = std::fabs(a[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(a[i * n + k]) > best) {
+        best = std::fabs(a[i * n + k]);
+        p = i;
+      }
+    if (best == 0.0) return false;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) std::swap(a[k * n + j], a[p * n + j]);
+      std::swap(b[k], b[p]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = a[i * n + k] / a[k * n + k];
+      a[i * n + k] = f;
+      for (int j = k + 1; j < n; ++j) a[i * n + j] -= f * a[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  x.assign(n, 0.0);
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= a[i * n + j] * x[j];
+    x[i] = s / a[i * n + i];
+  }
+  return true;
 }
 
 int need_engine(const qmc_system* s) {
@@ -409,6 +483,15 @@ bool is_device_ptr(const void* p) {
     return false;
   }
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+int put(double* dst, const double* src, size_t n) {
+  if (!dst || n == 0) return QMC_OK;
+  if (is_device_ptr(dst))
+    HIP_TRY(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+  else
+    std::copy(src, src + n, dst);
+  return QMC_OK;
 }
 
 // Device view of a caller buffer: the buffer itself when it is device
@@ -879,6 +962,76 @@ int qmc_local_energy(const qmc_system* s, int64_t W, const double* pos, double* 
   return qmc_evaluate(s, W, pos, nullptr, nullptr, nullptr, nullptr, el, st);
 }
 
+int qmc_derivatives(const qmc_system* sys, int64_t W, const double* pos, double* eloc, double* O, double* dE,
+                    double* F, double* Om, void* stream) {
+  if (!sys || !pos || W < 0) return fail(QMC_ERR_INVALID, "bad arguments");
+  if (W == 0) return QMC_OK;
+  if (!sys->dops) return need_engine(sys) ? need_engine(sys) : fail(QMC_ERR_UNSUPPORTED, "no derivative kernel");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = sys->h.n_elec();
+  const Cols C(sys->h);
+  int rc;
+  Staged P;
+  if ((rc = P.init(pos, sizeof(double) * W * N * 3, true, st))) return rc;
+  double* Y = nullptr;
+  HIP_TRY(hipMalloc(&Y, sizeof(double) * (size_t)C.Qp * W));
+  std::unique_ptr<double, void (*)(double*)> yg(Y, [](double* p) { (void)hipFree(p); });
+  DerivArgs A{W, C.P, C.M, C.Qp, (const double*)P.dev, nullptr, nullptr, 0, Y, W};
+  if (sys->precision == QMC_FP64)
+    HIP_TRY(sys->dops->derivs64(sys->d64, sys->d64, A, st));
+  else
+    HIP_TRY(sys->dops->derivs32(sys->d64, sys->d32, A, st));
+  std::vector<double> h((size_t)C.Qp * W);
+  HIP_TRY(hipMemcpyAsync(h.data(), Y, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  const int Pn = C.P, M = C.M;
+  const std::vector<double> fnn = hf_nn(sys->h);
+  std::vector<double> o((size_t)W * Pn), d((size_t)W * Pn), f((size_t)W * 3 * M), om((size_t)W * 3 * M), e(W);
+  for (int64_t w = 0; w < W; ++w) {
+    e[w] = h[(size_t)C.e() * W + w];
+    for (int k = 0; k < Pn; ++k) {
+      o[w * Pn + k] = h[(size_t)k * W + w];
+      d[w * Pn + k] = h[(size_t)C.de(k) * W + w];
+    }
+    for (int q = 0; q < 3 * M; ++q) {
+      om[w * 3 * M + q] = h[(size_t)(Pn + q) * W + w];
+      f[w * 3 * M + q] = h[(size_t)C.f(q / 3, q % 3) * W + w] + fnn[q];
+    }
+  }
+  if ((rc = put(eloc, e.data(), e.size())) || (rc = put(O, o.data(), o.size())) || (rc = put(dE, d.data(), d.size())) ||
+      (rc = put(F, f.data(), f.size())) || (rc = put(Om, om.data(), om.size())))
+    return rc;
+  return QMC_OK;
+}
+
+int qmc_log_derivs(const qmc_system* s, int64_t W, const double* pos, double* O, void* st) {
+  return qmc_derivatives(s, W, pos, nullptr, O, nullptr, nullptr, nullptr, st);
+}
+int qmc_dEL_dp(const qmc_system* s, int64_t W, const double* pos, double* dE, void* st) {
+  return qmc_derivatives(s, W, pos, nullptr, nullptr, dE, nullptr, nullptr, st);
+}
+int qmc_hf_force(const qmc_system* s, int64_t W, const double* pos, double* F, void* st) {
+  return qmc_derivatives(s, W, pos, nullptr, nullptr, nullptr, F, nullptr, st);
+}
+int qmc_nuc_grad_log(const qmc_system* s, int64_t W, const double* pos, double* Om, void* st) {
+  return qmc_derivatives(s, W, pos, nullptr, nullptr, nullptr, nullptr, Om, st);
+}
+
+int qmc_sr_solve(int32_t n, const double* S, const double* f, double eps, double* dp) {
+  if (n < 1 || !S || !f || !dp) return fail(QMC_ERR_INVALID, "bad arguments");
+  std::vector<double> a(S, S + (size_t)n * n), b(n), x;
+  for (int i = 0; i < n; ++i) {
+    a[(size_t)i * n + i] += eps;
+    b[i] = -f[i];
+  }
+  if (!lu_solve(n, a, b, x)) {
+    std::copy(b.begin(), b.end(), dp);  // sr_optimize.rs:239-242
+    return fail(QMC_ERR_SINGULAR, "SR matrix singular; fell back to -f");
+  }
+  std::copy(x.begin(), x.end(), dp);
+  return QMC_OK;
+}
+
 }  // extern "C"
 
 // ===========================================================================
@@ -907,8 +1060,19 @@ struct qmc_ensemble {
   std::vector<hipEvent_t> ev;  // start/stop pairs, one pair per launch
   double kernel_ms = 0.0;
   int64_t launches = 0;
+  uint64_t sys_version = 0;  // system parameters the cached state was built with
+  // moment accumulator of qmc_ensemble_measure
+  std::vector<double> gacc;
+  int gq = 0;
+  double* Y = nullptr;
+  size_t Y_cap = 0;
+  double* gpart = nullptr;
+  size_t gpart_cap = 0;
+  double* G = nullptr;
+  size_t G_cap = 0;
   ~qmc_ensemble() {
-    for (void* p : {pos, ainv, (void*)logdet, (void*)sign, (void*)ju, (void*)eloc, (void*)partial, (void*)sums})
+    for (void* p : {pos, ainv, (void*)logdet, (void*)sign, (void*)ju, (void*)eloc, (void*)partial, (void*)sums,
+                    (void*)Y, (void*)gpart, (void*)G})
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   }
@@ -933,6 +1097,7 @@ int ens_refresh(qmc_ensemble* e, int want_eloc) {
     HIP_TRY(s->ops->build32(s->d64, s->d32, e->view<float>(), want_eloc, nullptr, e->st));
   e->inverse_stale = false;
   e->since_refresh = 0;
+  e->sys_version = s->version;
   return QMC_OK;
 }
 
@@ -1049,6 +1214,7 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
   if (p->n_steps == 0) return QMC_OK;
   const qmc_system* s = e->sys;
   int rc;
+  if (e->sys_version != s->version && (rc = ens_refresh(e, 1))) return rc;  // parameters or nuclei moved
   if (p->move == QMC_MOVE_SINGLE && e->inverse_stale && (rc = ens_refresh(e, 0))) return rc;
   const int nblk = (int)((e->W + 63) / 64);
   const int refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
@@ -1116,6 +1282,189 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
     else
       std::copy(col.begin(), col.end(), dst);
   }
+  return QMC_OK;
+}
+
+int qmc_ensemble_measure(qmc_ensemble* e, int32_t reset) {
+  if (!e) return fail(QMC_ERR_INVALID, "null ensemble");
+  const qmc_system* s = e->sys;
+  if (!s->dops) return fail(QMC_ERR_UNSUPPORTED, "no derivative kernel for this shape");
+  int rc;
+  if (e->sys_version != s->version && (rc = ens_refresh(e, 1))) return rc;
+  const Cols C(s->h);
+  const int chunk = 256;
+  const int nblk = (int)((e->W + chunk - 1) / chunk);
+  if ((rc = ensure_cap(&e->Y, &e->Y_cap, (size_t)C.Qp * e->ld))) return rc;
+  if ((rc = ensure_cap(&e->gpart, &e->gpart_cap, (size_t)nblk * C.Qp * C.Qp))) return rc;
+  if ((rc = ensure_cap(&e->G, &e->G_cap, (size_t)C.Qp * C.Qp))) return rc;
+  DerivArgs A{e->W, C.P, C.M, C.Qp, nullptr, nullptr, nullptr, e->ld, e->Y, e->ld};
+  if (e->prec == QMC_FP64) {
+    A.pos64 = (const double*)e->pos;
+    HIP_TRY(s->dops->derivs64(s->d64, s->d64, A, e->st));
+  } else {
+    A.pos32 = (const float*)e->pos;
+    HIP_TRY(s->dops->derivs32(s->d64, s->d32, A, e->st));
+  }
+  HIP_TRY(launch_gram(e->Y, e->ld, e->W, C.Qp, e->gpart, chunk, e->G, e->st));
+  std::vector<double> g((size_t)C.Qp * C.Qp);
+  HIP_TRY(hipMemcpyAsync(g.data(), e->G, g.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
+  HIP_TRY(hipStreamSynchronize(e->st));
+  if (reset || e->gq != C.Qp) {
+    e->gacc.assign(g.size(), 0.0);
+    e->gq = C.Qp;
+  }
+  for (int i = 0; i < C.Qp; ++i)  // upper triangle is valid; mirror it
+    for (int j = i; j < C.Qp; ++j) {
+      e->gacc[(size_t)i * C.Qp + j] += g[(size_t)i * C.Qp + j];
+      if (j != i) e->gacc[(size_t)j * C.Qp + i] += g[(size_t)i * C.Qp + j];
+    }
+  return QMC_OK;
+}
+
+int qmc_ensemble_moments(const qmc_ensemble* e, qmc_moments* out) {
+  if (!e || !out) return fail(QMC_ERR_INVALID, "bad arguments");
+  const Cols C(e->sys->h);
+  if (e->gq != C.Qp || e->gacc.empty()) return fail(QMC_ERR_INVALID, "no measurements accumulated");
+  auto g = [&](int a, int b) { return e->gacc[(size_t)a * C.Qp + b]; };
+  const double n = g(C.one(), C.one());
+  const int P = C.P, M = C.M, Q = P + 3 * M;
+  out->n_samples = (int64_t)std::llround(n);
+  out->P = P;
+  out->M = M;
+  const double em = g(C.e(), C.one()) / n;
+  out->energy = em;
+  out->variance = g(C.e(), C.e()) / n - em * em;
+  std::vector<double> om(Q), S((size_t)Q * Q), f(Q), de(P), fhf(3 * M), fzv(3 * M), fv(M, 0.0);
+  for (int q = 0; q < Q; ++q) om[q] = g(q, C.one()) / n;
+  for (int q = 0; q < Q; ++q) {
+    for (int r = 0; r < Q; ++r) S[(size_t)q * Q + r] = g(q, r) / n - om[q] * om[r];
+    f[q] = g(q, C.e()) / n - em * om[q];
+  }
+  for (int k = 0; k < P; ++k) de[k] = g(C.de(k), C.one()) / n;
+  const std::vector<double> fnn = hf_nn(e->sys->h);
+  for (int I = 0; I < M; ++I)
+    for (int c = 0; c < 3; ++c) {
+      const int col = C.f(I, c);
+      const double mean_el = g(col, C.one()) / n;
+      fhf[3 * I + c] = mean_el + fnn[3 * I + c];
+      fzv[3 * I + c] = fhf[3 * I + c] - 2.0 * f[P + 3 * I + c];
+      fv[I] += g(col, col) / n - mean_el * mean_el;
+    }
+  for (double& v : fv) v = std::max(v, 0.0);
+  int rc;
+  if ((rc = put(out->o_mean, om.data(), om.size())) || (rc = put(out->S, S.data(), S.size())) ||
+      (rc = put(out->f, f.data(), f.size())) || (rc = put(out->de_mean, de.data(), de.size())) ||
+      (rc = put(out->f_hf, fhf.data(), fhf.size())) || (rc = put(out->f_zv, fzv.data(), fzv.size())) ||
+      (rc = put(out->f_var, fv.data(), fv.size())))
+    return rc;
+  return QMC_OK;
+}
+
+// SROptimizer::optimize, sr_optimize.rs:252-348.  Walkers persist across
+// iterations (the reference draws fresh ones and re-equilibrates each
+// iteration); each iteration takes max(1, n_samples / n_walkers)
+// measurement rounds separated by `decorrelation` sweeps (:141-168).
+int qmc_sr_optimize(qmc_system* sys, const qmc_sr_params* p, uint64_t seed, qmc_sr_results* res,
+                    double* energy_history, double* variance_history, double* param_history) {
+  if (!sys || !p || !res || p->n_walkers < 1 || p->max_iterations < 1 || sys->h.jas_kind != QMC_JAS_PADE)
+    return fail(QMC_ERR_INVALID, "bad arguments (SR needs the Pade Jastrow)");
+  qmc_ensemble* e = nullptr;
+  int rc = qmc_ensemble_create(sys, p->n_walkers, seed, &e);
+  if (rc) return rc;
+  std::unique_ptr<qmc_ensemble> guard(e);
+  const int P = sys->h.n_params(), M = sys->h.n_nuc();
+  const int Q = p->include_nuclear ? P + 3 * M : P;
+  const int rounds = (int)std::max<int64_t>(1, p->n_samples / p->n_walkers);
+  const int dec = p->decorrelation > 0 ? p->decorrelation : 5;
+  auto sweeps = [&](int n) {
+    qmc_sweep_params sp{p->move, n, p->step_size, p->refresh_interval, 0};
+    return n > 0 ? qmc_ensemble_sweeps(e, &sp, nullptr, nullptr, nullptr, nullptr) : QMC_OK;
+  };
+  auto sample = [&](qmc_moments* m, std::vector<double>* S, std::vector<double>* f) {
+    int r;
+    for (int k = 0; k < rounds; ++k) {
+      if ((r = sweeps(dec)) || (r = qmc_ensemble_measure(e, k == 0))) return r;
+    }
+    const int Qa = P + 3 * M;
+    std::vector<double> Sa((size_t)Qa * Qa), fa(Qa);
+    *m = qmc_moments{};
+    m->S = Sa.data();
+    m->f = fa.data();
+    if ((r = qmc_ensemble_moments(e, m))) return r;
+    m->S = nullptr;
+    m->f = nullptr;
+    if (S) {
+      S->assign((size_t)Q * Q, 0.0);
+      f->assign(Q, 0.0);
+      for (int i = 0; i < Q; ++i) {
+        (*f)[i] = fa[i];
+        for (int j = 0; j < Q; ++j) (*S)[(size_t)i * Q + j] = Sa[(size_t)i * Qa + j];
+      }
+    }
+    return QMC_OK;
+  };
+  if ((rc = sweeps((int)p->n_equilibrate))) return rc;
+  std::vector<double> params(P);
+  qmc_get_params(sys, params.data());
+  if (param_history) std::copy(params.begin(), params.end(), param_history);
+  double prev = std::numeric_limits<double>::max();
+  int it = 0;
+  for (; it < p->max_iterations; ++it) {
+    qmc_moments m;
+    std::vector<double> S, f, dp(Q);
+    if ((rc = sample(&m, &S, &f))) return rc;
+    if (energy_history) energy_history[it] = m.energy;
+    if (variance_history) variance_history[it] = m.variance;
+    qmc_sr_solve(Q, S.data(), f.data(), p->sr_epsilon, dp.data());
+    // sr_optimize.rs:297-307: per-step clamp +-0.3, then the reference's
+    // parameters to [0.05, 10]; series coefficients to [-10, 10]
+    for (int k = 0; k < P; ++k) {
+      params[k] += std::min(std::max(p->learning_rate * dp[k], -0.3), 0.3);
+      params[k] = k < 4 ? std::min(std::max(params[k], 0.05), 10.0) : std::min(std::max(params[k], -10.0), 10.0);
+    }
+    if ((rc = qmc_set_params(sys, params.data(), P))) return rc;
+    if (param_history) std::copy(params.begin(), params.end(), param_history + (size_t)(it + 1) * P);
+    const double change = std::fabs(m.energy - prev);  // sr_optimize.rs:313-320
+    if (it > 5 && change < p->tolerance) {
+      ++it;
+      break;
+    }
+    prev = m.energy;
+  }
+  qmc_moments m;
+  if ((rc = sample(&m, nullptr, nullptr))) return rc;  // sr_optimize.rs:325-330
+  res->final_energy = m.energy;
+  res->final_variance = m.variance;
+  res->iterations = it;
+  return QMC_OK;
+}
+
+int qmc_sample_forces(const qmc_system* sys, const qmc_force_params* p, uint64_t seed, double* forces,
+                      double* mean_energy, double* error, double* force_variance) {
+  if (!sys || !p || p->n_walkers < 1) return fail(QMC_ERR_INVALID, "bad arguments");
+  qmc_ensemble* e = nullptr;
+  int rc = qmc_ensemble_create(sys, p->n_walkers, seed, &e);
+  if (rc) return rc;
+  std::unique_ptr<qmc_ensemble> guard(e);
+  const int rounds = (int)std::max<int64_t>(1, p->n_samples / p->n_walkers);
+  const int dec = p->decorrelation > 0 ? p->decorrelation : 5;
+  qmc_sweep_params eq{p->move, (int32_t)p->n_equilibrate, p->mc_step_size, p->refresh_interval, 0};
+  if (p->n_equilibrate > 0 && (rc = qmc_ensemble_sweeps(e, &eq, nullptr, nullptr, nullptr, nullptr))) return rc;
+  qmc_sweep_params sp{p->move, dec, p->mc_step_size, p->refresh_interval, 0};
+  for (int k = 0; k < rounds; ++k)
+    if ((rc = qmc_ensemble_sweeps(e, &sp, nullptr, nullptr, nullptr, nullptr)) || (rc = qmc_ensemble_measure(e, k == 0)))
+      return rc;
+  const int M = sys->h.n_nuc();
+  std::vector<double> fhf(3 * M), fzv(3 * M), fv(M);
+  qmc_moments m{};
+  m.f_hf = fhf.data();
+  m.f_zv = fzv.data();
+  m.f_var = fv.data();
+  if ((rc = qmc_ensemble_moments(e, &m))) return rc;
+  if (forces) std::copy_n(p->zero_variance ? fzv.data() : fhf.data(), 3 * M, forces);
+  if (mean_energy) *mean_energy = m.energy;
+  if (error) *error = std::sqrt(m.variance / (double)m.n_samples);  // geometry_opt.rs:198-201
+  if (force_variance) std::copy(fv.begin(), fv.end(), force_variance);
   return QMC_OK;
 }
 

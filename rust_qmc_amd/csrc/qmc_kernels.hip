@@ -9,12 +9,6 @@
 namespace qmc {
 
 template <int NU, int ND, class T>
-struct WalkerRegs {
-  T ru[Cap<NU>::v][3], rd[Cap<ND>::v][3];
-  T au[Cap<NU>::v][Cap<NU>::v], ad[Cap<ND>::v][Cap<ND>::v];
-};
-
-template <int NU, int ND, class T>
 __device__ __forceinline__ void load_positions(const DevSys<T>& S, const T* pos, int64_t ld, int64_t w,
                                                WalkerRegs<NU, ND, T>& W) {
 #pragma unroll
