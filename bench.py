@@ -95,7 +95,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=256)
-    ap.add_argument("--warmup", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--walkers", type=int, default=65536)
     ap.add_argument("--steps-per-launch", type=int, default=16)
     ap.add_argument("--refresh", type=int, default=64)

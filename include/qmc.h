@@ -140,6 +140,14 @@ int qmc_benzene_gto_create(double b_ee, double b_en, double cc_bond, double ch_b
                            qmc_system** out);
 
 int qmc_set_precision(qmc_system* sys, int32_t precision);
+/* Engine selection.  The register engine keeps one walker per lane in
+ * VGPRs (compiled electron counts up to 5 + 5); the wave engine keeps one
+ * walker per wavefront in LDS (any count up to 32 per spin).  AUTO picks
+ * the register engine when it exists for the system. */
+#define QMC_ENGINE_AUTO 0
+#define QMC_ENGINE_REGISTER 1
+#define QMC_ENGINE_WAVE 2
+int qmc_set_engine(qmc_system* sys, int32_t engine);
 int qmc_set_flags(qmc_system* sys, uint32_t flags);
 int qmc_system_info(const qmc_system* sys, int32_t* n_elec, int32_t* n_up, int32_t* n_nuc, int32_t* n_ao);
 /* MO matrix of one spin (0 up, 1 down) as [n_mo][n_ao] */

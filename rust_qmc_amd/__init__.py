@@ -112,6 +112,10 @@ class System:
     def set_flags(self, flags):
         check(lib().qmc_set_flags(self._h, flags))
 
+    def set_engine(self, engine):
+        """'auto', 'register' (one walker per lane) or 'wave' (one walker per wavefront)."""
+        check(lib().qmc_set_engine(self._h, {"auto": 0, "register": 1, "wave": 2}[engine]))
+
     def nuclear_repulsion(self):
         v = C.c_double()
         check(lib().qmc_nuclear_repulsion(self._h, C.byref(v)))

@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __rest
 // ---------------------------------------------------------------------------
 // G = Y Y^T over a walker chunk per wave with v_mfma_f64_16x16x4_f64.
 // Fragment convention (16x16x4 f64): lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; accumulator register r of lane l is D[4(l/16)+r][l%16].
+// B[l/16][l%16]; accumulator register r of lane l is D[4r + l/16][l%16].
 // A tile (I, J) pairs feature block I (rows) with block J (columns); both
 // fragments are "feature (l%16) of walker k0 + l/16", so one load per block
 // serves every tile.  Only I <= J is formed; the host mirrors.
@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(64) gram_kernel(const double* __restrict__ Y, 
 #pragma unroll
     for (int J = I; J < NB; ++J) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) out[(16 * I + 4 * kk + r) * Qp + 16 * J + fr] = acc[t][r];
+      for (int r = 0; r < 4; ++r) out[(16 * I + 4 * r + kk) * Qp + 16 * J + fr] = acc[t][r];
       ++t;
     }
 }

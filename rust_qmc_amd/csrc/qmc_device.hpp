@@ -52,6 +52,8 @@ struct DevSys {
   T prim_exp[MAXPRIM], coef_s[MAXPRIM], coef_p[MAXPRIM];
   int prim_pow[MAXPRIM];
   int sector_elec[2][MAXE];   // global index of the k-th electron of each spin sector
+  int elec_spin[MAXE];        // 0 up, 1 down
+  int elec_slot[MAXE];        // row of the electron in its sector
   int init_nuc[MAXE];
   T init_sigma;
   T moT[2][MAXAO][MAXMO];     // [spin][ao][mo], mo fastest
@@ -295,14 +297,13 @@ __device__ __forceinline__ void mo_grad_lap(const DevSys<T>& S, int spin, T x, T
 template <int NS, class T, class OnShell>
 __device__ __forceinline__ void det_terms(const DevSys<T>& S, int spin, T x, T y, T z,
                                           const T ai[Cap<NS>::v][Cap<NS>::v], T& gdx, T& gdy, T& gdz, T& ld,
-                                          OnShell on_shell) {
+                                          OnShell on_shell);
+
+// the same with the AO weights w_ao supplied by wgt(ao)
+template <class T, class Wgt, class OnShell>
+__device__ __forceinline__ void det_terms_w(const DevSys<T>& S, T x, T y, T z, Wgt wgt, T& gdx, T& gdy, T& gdz,
+                                            T& ld, OnShell on_shell) {
   gdx = gdy = gdz = ld = T(0);
-  auto wgt = [&](int ao) {
-    T w = 0;
-#pragma unroll
-    for (int m = 0; m < NS; ++m) w += S.moT[spin][ao][m] * ai[m][0];
-    return w;
-  };
   for (int sh = 0; sh < S.n_shell; ++sh) {
     const int c = S.sh_center[sh], kind = S.sh_kind[sh], p0 = S.sh_prim0[sh], np = S.sh_nprim[sh];
     const int a0 = S.sh_ao0[sh];
@@ -385,6 +386,61 @@ __device__ __forceinline__ void det_terms(const DevSys<T>& S, int spin, T x, T y
     gdz += sz;
     ld += sl;
     on_shell(c, sx, sy, sz);
+  }
+}
+
+template <int NS, class T, class OnShell>
+__device__ __forceinline__ void det_terms(const DevSys<T>& S, int spin, T x, T y, T z,
+                                          const T ai[Cap<NS>::v][Cap<NS>::v], T& gdx, T& gdy, T& gdz, T& ld,
+                                          OnShell on_shell) {
+  auto wgt = [&](int ao) {
+    T w = 0;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) w += S.moT[spin][ao][m] * ai[m][0];
+    return w;
+  };
+  det_terms_w(S, x, y, z, wgt, gdx, gdy, gdz, ld, on_shell);
+}
+
+// AO values of one shell at (x, y, z), written to chi[ao - base] for the
+// shell's AOs (base = 0: absolute AO index; base = first AO: shell-local)
+template <class T>
+__device__ __forceinline__ void shell_values(const DevSys<T>& S, int sh, T x, T y, T z, T* chi, bool local = false) {
+  const int c = S.sh_center[sh], kind = S.sh_kind[sh], p0 = S.sh_prim0[sh], np = S.sh_nprim[sh];
+  const int a0 = local ? 0 : S.sh_ao0[sh];
+  T dx = x - S.nuc[c][0], dy = y - S.nuc[c][1], dz = z - S.nuc[c][2];
+  T r2 = dx * dx + dy * dy + dz * dz;
+  if (kind <= SH_P1) {
+    T rs = T(0), rp = T(0);
+    for (int k = 0; k < np; ++k) {
+      T e = M<T>::ex(-S.prim_exp[p0 + k] * r2);
+      rs += S.coef_s[p0 + k] * e;
+      rp += S.coef_p[p0 + k] * e;
+    }
+    if (kind == SH_S) {
+      chi[a0] = rs;
+    } else if (kind == SH_P1) {
+      const int m = S.sh_m[sh];
+      chi[a0] = (m == 0 ? dx : (m == 1 ? dy : dz)) * rp;
+    } else {
+      int b = a0;
+      if (kind == SH_SP) {
+        chi[a0] = rs;
+        b = a0 + 1;
+      }
+      chi[b] = dx * rp;
+      chi[b + 1] = dy * rp;
+      chi[b + 2] = dz * rp;
+    }
+  } else {
+    T s = M<T>::sq(r2);
+    T v = T(0);
+    for (int k = 0; k < np; ++k) {
+      T sp = T(1);
+      for (int q = 0; q < S.prim_pow[p0 + k]; ++q) sp *= s;
+      v += S.coef_s[p0 + k] * sp * M<T>::ex(-S.prim_exp[p0 + k] * s);
+    }
+    chi[a0] = v;
   }
 }
 

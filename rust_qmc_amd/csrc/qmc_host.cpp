@@ -18,6 +18,7 @@
 #include "qmc_derivs.hpp"
 #include "qmc_device.hpp"
 #include "qmc_kernels.hpp"
+#include "qmc_wave.hpp"
 
 using namespace qmc;
 
@@ -98,8 +99,13 @@ struct qmc_system {
   DevSys<float>* d32 = nullptr;
   const EngineOps* ops = nullptr;
   const DerivOps* dops = nullptr;
+  int32_t engine = QMC_ENGINE_AUTO;
   bool dirty = true;
   uint64_t version = 0;  // bumped on every upload; ensembles refresh when it moves
+  // the wave engine runs when selected, or when no register engine was
+  // compiled for this electron count
+  bool wave() const { return engine == QMC_ENGINE_WAVE || (engine == QMC_ENGINE_AUTO && !ops); }
+  WaveShape shape() const { return WaveShape{h.n_elec(), h.n_up(), h.n_dn(), h.n_ao()}; }
   ~qmc_system() {
     if (d64) (void)hipFree(d64);
     if (d32) (void)hipFree(d32);
@@ -254,10 +260,15 @@ int fill_devsys(const HostSys& h, DevSys<T>& D) {
     }
   int ku = 0, kd = 0;
   for (int e = 0; e < h.n_elec(); ++e) {
-    if (h.spins[e] == 1)
+    if (h.spins[e] == 1) {
+      D.elec_spin[e] = 0;
+      D.elec_slot[e] = ku;
       D.sector_elec[0][ku++] = e;
-    else
+    } else {
+      D.elec_spin[e] = 1;
+      D.elec_slot[e] = kd;
       D.sector_elec[1][kd++] = e;
+    }
     D.init_nuc[e] = h.init_nuc[e];
   }
   D.init_sigma = (T)h.init_sigma;
@@ -352,8 +363,13 @@ bool lu_solve(int n, std::vector<double> a, std::vector<double> b, std::vector<d
 }
 
 int need_engine(const qmc_system* s) {
+  if (s->wave()) {
+    if (std::max(s->h.n_up(), s->h.n_dn()) > MAXMO)
+      return fail(QMC_ERR_UNSUPPORTED, "more than 32 electrons of one spin");
+    return QMC_OK;
+  }
   if (!s->ops)
-    return fail(QMC_ERR_UNSUPPORTED, "no engine compiled for " + std::to_string(s->h.n_up()) + " up / " +
+    return fail(QMC_ERR_UNSUPPORTED, "no register engine compiled for " + std::to_string(s->h.n_up()) + " up / " +
                                          std::to_string(s->h.n_dn()) + " down electrons");
   return QMC_OK;
 }
@@ -610,7 +626,9 @@ int qmc_h2_molecule_mo_create(double alpha, const double c1[3], const double c2[
   h.F = F;
   h.init_nuc = {0, 1};
   h.init_sigma = init_sigma;
-  h.flags = 0;  // Jastrow1 is exact (jastrow.rs:35-53)
+  // Jastrow1 is exact (jastrow.rs:35-53); the HF force keeps the
+  // reference's nucleus-nucleus sign (hydrogen.rs:245-255)
+  h.flags = QMC_FLAG_HF_NN_REFSIGN;
   return make_system(std::move(h), precision, out);
 }
 
@@ -706,6 +724,7 @@ int qmc_methane_gto_create(double b_ee, double b_en, double ch_bond, int32_t bas
   }
   h.mo_dn = h.mo_up;  // both spins share the MOs (methane.rs:824)
   jastrow3(h, b_ee, b_en);
+  h.flags = QMC_FLAG_HF_NN_REFSIGN;  // methane.rs:1259-1269
   return make_system(std::move(h), precision, out);
 }
 
@@ -806,12 +825,22 @@ int qmc_benzene_gto_create(double b_ee, double b_en, double cc, double ch, int32
   h.mo_up = mo;
   h.mo_dn = mo;
   jastrow3(h, b_ee, b_en);
+  h.flags = QMC_FLAG_HF_NN_REFSIGN;  // benzene.rs:846-856
   return make_system(std::move(h), precision, out);
 }
 
 int qmc_set_precision(qmc_system* s, int32_t precision) {
   if (!s || (precision != QMC_FP64 && precision != QMC_MIXED)) return fail(QMC_ERR_INVALID, "bad precision");
   s->precision = precision;
+  return QMC_OK;
+}
+
+int qmc_set_engine(qmc_system* s, int32_t engine) {
+  if (!s || engine < QMC_ENGINE_AUTO || engine > QMC_ENGINE_WAVE) return fail(QMC_ERR_INVALID, "bad engine");
+  if (engine == QMC_ENGINE_REGISTER && !s->ops)
+    return fail(QMC_ERR_UNSUPPORTED, "no register engine for this electron count");
+  s->engine = engine;
+  s->version++;  // ensembles rebuild their state with the new engine
   return QMC_OK;
 }
 
@@ -935,10 +964,16 @@ int qmc_evaluate(const qmc_system* sys, int64_t W, const double* pos, double* lo
   std::unique_ptr<int, void (*)(int*)> sing_guard(sing, [](int* p) { (void)hipFree(p); });
   HIP_TRY(hipMemsetAsync(sing, 0, sizeof(int), st));
   EvalOut O{(double*)L.dev, (double*)S.dev, (double*)G.dev, (double*)Lp.dev, (double*)E.dev};
-  if (sys->precision == QMC_FP64)
+  if (sys->wave()) {
+    if (sys->precision == QMC_FP64)
+      HIP_TRY(wave_eval<double>(sys->d64, sys->d64, W, (const double*)P.dev, sys->shape(), O, sing, st));
+    else
+      HIP_TRY(wave_eval<float>(sys->d64, sys->d32, W, (const double*)P.dev, sys->shape(), O, sing, st));
+  } else if (sys->precision == QMC_FP64) {
     HIP_TRY(sys->ops->eval64(sys->d64, sys->d64, W, (const double*)P.dev, O, sing, st));
-  else
+  } else {
     HIP_TRY(sys->ops->eval32(sys->d64, sys->d32, W, (const double*)P.dev, O, sing, st));
+  }
   for (Staged* s : {&L, &S, &G, &Lp, &E})
     if ((rc = s->copy_out())) return rc;
   int nsing = 0;
@@ -1091,10 +1126,16 @@ namespace {
 
 int ens_refresh(qmc_ensemble* e, int want_eloc) {
   const qmc_system* s = e->sys;
-  if (e->prec == QMC_FP64)
+  if (s->wave()) {
+    if (e->prec == QMC_FP64)
+      HIP_TRY(wave_build<double>(s->d64, s->d64, e->view<double>(), s->shape(), want_eloc, nullptr, e->st));
+    else
+      HIP_TRY(wave_build<float>(s->d64, s->d32, e->view<float>(), s->shape(), want_eloc, nullptr, e->st));
+  } else if (e->prec == QMC_FP64) {
     HIP_TRY(s->ops->build64(s->d64, s->d64, e->view<double>(), want_eloc, nullptr, e->st));
-  else
+  } else {
     HIP_TRY(s->ops->build32(s->d64, s->d32, e->view<float>(), want_eloc, nullptr, e->st));
+  }
   e->inverse_stale = false;
   e->since_refresh = 0;
   e->sys_version = s->version;
@@ -1216,7 +1257,10 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
   int rc;
   if (e->sys_version != s->version && (rc = ens_refresh(e, 1))) return rc;  // parameters or nuclei moved
   if (p->move == QMC_MOVE_SINGLE && e->inverse_stale && (rc = ens_refresh(e, 0))) return rc;
-  const int nblk = (int)((e->W + 63) / 64);
+  const bool wave = s->wave();
+  if (wave && p->move != QMC_MOVE_SINGLE)
+    return fail(QMC_ERR_UNSUPPORTED, "the wave engine runs single-electron moves only");
+  const int nblk = wave ? (int)e->W : (int)((e->W + 63) / 64);
   const int refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
   int chunk = p->steps_per_launch > 0 ? p->steps_per_launch : 16;
   if ((rc = ensure_cap(&e->partial, &e->partial_cap, (size_t)chunk * nblk * 3))) return rc;
@@ -1238,7 +1282,12 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
       }
       HIP_TRY(hipEventRecord(e->ev[2 * nl], e->st));
     }
-    if (p->move == QMC_MOVE_SINGLE) {
+    if (wave) {
+      if (e->prec == QMC_FP64)
+        HIP_TRY(wave_sweep<double>(s->d64, e->view<double>(), s->shape(), A, e->partial, trace, e->st));
+      else
+        HIP_TRY(wave_sweep<float>(s->d32, e->view<float>(), s->shape(), A, e->partial, trace, e->st));
+    } else if (p->move == QMC_MOVE_SINGLE) {
       if (e->prec == QMC_FP64)
         HIP_TRY(s->ops->sweep64(s->d64, e->view<double>(), A, e->partial, trace, e->st));
       else

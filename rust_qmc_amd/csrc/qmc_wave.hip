@@ -1,0 +1,502 @@
+// qmc_wave.hip -- the wave engine: one 64-lane wavefront owns one walker
+// whose positions and inverse Slater matrices live in LDS.  Used for
+// systems too large for the register engine (benzene: 21+21 electrons, 66
+// AOs) and for any electron count the register engine was not compiled
+// for.  Within a single-electron move the lanes split the work:
+//   shells      -> AO values of the proposed position
+//   MOs         -> MO values and the determinant ratio (wave reduction)
+//   electrons   -> Jastrow difference (wave reduction)
+//   (mo, row)   -> Sherman-Morrison update of the inverse
+// and the local energy puts one electron on each lane.
+#include "qmc_wave.hpp"
+
+#include "qmc_engine.hpp"
+
+namespace qmc {
+
+// wave-wide sum, deterministic butterfly
+template <class T>
+__device__ __forceinline__ T wsum(T v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// LDS carve-out of one walker (element counts in T unless noted)
+struct WaveLayout {
+  int N, nu, nd, nmax, A;
+  __device__ __host__ int pos() const { return 0; }
+  __device__ __host__ int ainv(int spin) const { return 3 * N + (spin ? nu * nu : 0); }
+  __device__ __host__ int chi() const { return 3 * N + nu * nu + nd * nd; }
+  __device__ __host__ int phi() const { return chi() + A; }
+  __device__ __host__ int cvec() const { return phi() + nmax; }
+  __device__ __host__ int wvec() const { return cvec() + nmax; }
+  __device__ __host__ int total() const { return wvec() + nmax; }
+};
+
+template <class T>
+__device__ __forceinline__ T* ainv_at(T* L, const WaveLayout& Y, int spin, int n, int m, int k) {
+  return L + Y.ainv(spin) + m * n + k;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 build of one spin sector: Slater matrix from the positions in LDS,
+// Gauss-Jordan inversion with partial pivoting by the wave.  a and inv are
+// FP64 LDS scratch of n*n each.  Returns ln|det| and sign (wave-uniform).
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ bool wave_sector_inverse(const DevSys<double>& S, int spin, const T* pos, int n, double* a, double* inv,
+                                    double& logabs, double& sgn) {
+  const int lane = threadIdx.x;
+  logabs = 0.0;
+  sgn = 1.0;
+  if (n == 0) return false;
+  // rows: lane k evaluates all MOs at electron k of the sector
+  for (int k = lane; k < n; k += 64) {
+    const int e = S.sector_elec[spin][k];
+    double xr = (double)pos[3 * e], yr = (double)pos[3 * e + 1], zr = (double)pos[3 * e + 2];
+    for (int m = 0; m < n; ++m) a[k * n + m] = 0.0;
+    for (int sh = 0; sh < S.n_shell; ++sh) {
+      double chi[4];
+      const int a0 = S.sh_ao0[sh];
+      const int kind = S.sh_kind[sh];
+      const int nao = kind == SH_SP ? 4 : (kind == SH_P ? 3 : 1);
+      shell_values(S, sh, xr, yr, zr, chi, true);
+      for (int q = 0; q < nao; ++q)
+        for (int m = 0; m < n; ++m) a[k * n + m] += S.moT[spin][a0 + q][m] * chi[q];
+    }
+  }
+  for (int i = lane; i < n * n; i += 64) inv[i] = (i / n == i % n) ? 1.0 : 0.0;
+  __syncthreads();
+  bool singular = false;
+  __shared__ double fcol[64];
+  for (int c = 0; c < n; ++c) {
+    // pivot: largest |a[i][c]|, i >= c (ties to the lowest row)
+    double best = -1.0;
+    int p = c;
+    for (int i = c + lane; i < n; i += 64) {
+      double v = fabs(a[i * n + c]);
+      if (v > best) {
+        best = v;
+        p = i;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      double ob = __shfl_xor(best, off, 64);
+      int op = __shfl_xor(p, off, 64);
+      if (ob > best || (ob == best && op < p)) {
+        best = ob;
+        p = op;
+      }
+    }
+    if (best == 0.0) singular = true;
+    if (p != c) {
+      sgn = -sgn;
+      for (int j = lane; j < n; j += 64) {
+        double t = a[c * n + j];
+        a[c * n + j] = a[p * n + j];
+        a[p * n + j] = t;
+        t = inv[c * n + j];
+        inv[c * n + j] = inv[p * n + j];
+        inv[p * n + j] = t;
+      }
+    }
+    __syncthreads();
+    const double pv = a[c * n + c];
+    logabs += log(fabs(pv));
+    if (pv < 0.0) sgn = -sgn;
+    const double ip = singular ? 0.0 : 1.0 / pv;
+    __syncthreads();
+    for (int j = lane; j < n; j += 64) {
+      a[c * n + j] *= ip;
+      inv[c * n + j] *= ip;
+    }
+    for (int i = lane; i < n; i += 64) fcol[i] = a[i * n + c];
+    __syncthreads();
+    for (int idx = lane; idx < n * n; idx += 64) {
+      const int i = idx / n, j = idx % n;
+      if (i == c) continue;
+      const double f = fcol[i];
+      a[i * n + j] -= f * a[c * n + j];
+      inv[i * n + j] -= f * inv[c * n + j];
+    }
+    __syncthreads();
+  }
+  if (singular) {
+    for (int i = lane; i < n * n; i += 64) inv[i] = (i / n == i % n) ? 1.0 : 0.0;
+    logabs = -INFINITY;
+    sgn = 0.0;
+    __syncthreads();
+  }
+  return singular;
+}
+
+// full ln J (FP64): lane i sums pairs (i, j > i) and its e-n terms
+template <class T>
+__device__ double wave_jastrow_u(const DevSys<double>& S, const T* pos, int N) {
+  const int lane = threadIdx.x;
+  double u = 0.0;
+  for (int i = lane; i < N; i += 64) {
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    for (int j = i + 1; j < N; ++j) {
+      double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
+      const bool same = S.elec_spin[i] == S.elec_spin[j];
+      u += uee_val(S, sqrt(dx * dx + dy * dy + dz * dz), same) + S.cst_ee;
+    }
+    if (S.en_on)
+      for (int I = 0; I < S.n_nuc; ++I) {
+        double dx = xi - S.nuc[I][0], dy = yi - S.nuc[I][1], dz = zi - S.nuc[I][2];
+        u += uen_val(S, sqrt(dx * dx + dy * dy + dz * dz), I);
+      }
+  }
+  return wsum(u);
+}
+
+// local energy: lane i handles electron i; returns the wave-uniform E_L
+template <class T>
+__device__ double wave_local_energy(const DevSys<T>& S, const WaveLayout& Y, T* L, double* gout, double* lout) {
+  const int lane = threadIdx.x;
+  const T* pos = L + Y.pos();
+  double kin = 0.0, pot = 0.0;
+  for (int i = lane; i < Y.N; i += 64) {
+    const int spin = S.elec_spin[i], k = S.elec_slot[i];
+    const int n = spin ? Y.nd : Y.nu;
+    const T* ai = L + Y.ainv(spin);
+    const T x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    auto wgt = [&](int ao) {
+      T w = 0;
+      for (int m = 0; m < n; ++m) w += S.moT[spin][ao][m] * ai[m * n + k];
+      return w;
+    };
+    T gdx, gdy, gdz, ld;
+    det_terms_w(S, x, y, z, wgt, gdx, gdy, gdz, ld, NoShell{});
+    T gux = 0, guy = 0, guz = 0, lu = 0, pairsq = 0;
+    double pe = 0.0;
+    for (int j = 0; j < Y.N; ++j) {
+      if (j == i) continue;
+      T dx = x - pos[3 * j], dy = y - pos[3 * j + 1], dz = z - pos[3 * j + 2];
+      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+      T ir = M<T>::rcp(r);
+      T u1, lp;
+      uee_d(S, r, ir, S.elec_spin[j] == spin, u1, lp);
+      if (r >= T(1e-10)) {
+        T f = u1 * ir;
+        gux += f * dx;
+        guy += f * dy;
+        guz += f * dz;
+        lu += lp;
+        pairsq += u1 * u1;
+      }
+      pe += 0.5 * (double)ir;
+    }
+    for (int I = 0; I < S.n_nuc; ++I) {
+      T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+      T ir = M<T>::rcp(r);
+      pe -= (double)(S.Z[I] * ir);
+      if (S.en_on) {
+        T u1, lp;
+        uen_d(S, r, ir, I, u1, lp);
+        if (r >= T(1e-10)) {
+          T f = u1 * ir;
+          gux += f * dx;
+          guy += f * dy;
+          guz += f * dz;
+          lu += lp;
+        }
+      }
+    }
+    const T gs = S.grad_scale;
+    T gjx = gs * gux, gjy = gs * guy, gjz = gs * guz;
+    T lj = (S.flags & F_J2_LAP_QUIRK) ? (lu + pairsq) : (lu + gux * gux + guy * guy + guz * guz);
+    const T lap = ld + lj + T(2) * (gdx * gjx + gdy * gjy + gdz * gjz);
+    kin += -0.5 * (double)lap;
+    pot += pe;
+    if (gout) {
+      gout[3 * i] = (double)(gdx + gjx);
+      gout[3 * i + 1] = (double)(gdy + gjy);
+      gout[3 * i + 2] = (double)(gdz + gjz);
+    }
+    if (lout) lout[i] = (double)lap;
+  }
+  return wsum(kin) + wsum(pot) + S.vnn;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void load_walker(const EnsView<T>& E, const WaveLayout& Y, int64_t w, T* L) {
+  for (int q = threadIdx.x; q < 3 * Y.N; q += 64) L[Y.pos() + q] = E.pos[(int64_t)q * E.ld + w];
+  for (int q = threadIdx.x; q < Y.nu * Y.nu + Y.nd * Y.nd; q += 64) L[Y.ainv(0) + q] = E.ainv[(int64_t)q * E.ld + w];
+  __syncthreads();
+}
+
+template <class T>
+__device__ __forceinline__ void store_walker(const EnsView<T>& E, const WaveLayout& Y, int64_t w, const T* L) {
+  __syncthreads();
+  for (int q = threadIdx.x; q < 3 * Y.N; q += 64) E.pos[(int64_t)q * E.ld + w] = L[Y.pos() + q];
+  for (int q = threadIdx.x; q < Y.nu * Y.nu + Y.nd * Y.nd; q += 64) E.ainv[(int64_t)q * E.ld + w] = L[Y.ainv(0) + q];
+}
+
+// build / refresh: inverse (FP64) from the positions, ln|D|, ln J, E_L
+template <class T>
+__global__ void __launch_bounds__(64) wave_build_kernel(const DevSys<double>* __restrict__ Sd,
+                                                        const DevSys<T>* __restrict__ St, EnsView<T> E,
+                                                        WaveLayout Y, int want_eloc, int* singular_count) {
+  extern __shared__ double lds_d[];
+  const int64_t w = blockIdx.x;
+  T* L = (T*)lds_d;
+  double* a = lds_d + (Y.total() * sizeof(T) + 7) / 8;
+  double* inv = a + Y.nmax * Y.nmax;
+  for (int q = threadIdx.x; q < 3 * Y.N; q += 64) L[Y.pos() + q] = E.pos[(int64_t)q * E.ld + w];
+  __syncthreads();
+  double ltot = 0.0, stot = 1.0;
+  bool sing = false;
+  for (int spin = 0; spin < 2; ++spin) {
+    const int n = spin ? Y.nd : Y.nu;
+    double la, sg;
+    sing |= wave_sector_inverse(*Sd, spin, L + Y.pos(), n, a, inv, la, sg);
+    ltot += la;
+    stot *= sg;
+    for (int q = threadIdx.x; q < n * n; q += 64) L[Y.ainv(spin) + q] = (T)inv[q];
+    __syncthreads();
+  }
+  const double ju = wave_jastrow_u(*Sd, L + Y.pos(), Y.N);
+  double el = 0.0;
+  if (want_eloc) el = wave_local_energy(*St, Y, L, nullptr, nullptr);
+  store_walker(E, Y, w, L);
+  if (threadIdx.x == 0) {
+    E.logdet[w] = ltot;
+    E.sign[w] = stot;
+    E.ju[w] = ju;
+    if (want_eloc) E.eloc[w] = el;
+    if (sing && singular_count) atomicAdd(singular_count, 1);
+  }
+}
+
+// persistent single-electron sweeps
+template <class T>
+__global__ void __launch_bounds__(64) wave_sweep_kernel(const DevSys<T>* __restrict__ Sp, EnsView<T> E, WaveLayout Y,
+                                                        SweepArgs A, double* __restrict__ partial,
+                                                        double* __restrict__ el_trace) {
+  extern __shared__ double lds_d[];
+  const DevSys<T>& S = *Sp;
+  const int64_t w = blockIdx.x;
+  const int lane = threadIdx.x;
+  T* L = (T*)lds_d;
+  load_walker(E, Y, w, L);
+  T* pos = L + Y.pos();
+  T* chi = L + Y.chi();
+  T* phi = L + Y.phi();
+  T* cv = L + Y.cvec();
+  T* wv = L + Y.wvec();
+  double logdet = E.logdet[w], sgn = E.sign[w], ju = E.ju[w], el = 0.0;
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  const uint32_t wid = (uint32_t)(A.walker0 + w);
+  const T sigma = (T)A.step_size;
+  for (int s = 0; s < A.n_steps; ++s) {
+    const uint32_t step = A.step0 + (uint32_t)s;
+    int acc = 0;
+    for (int spin = 0; spin < 2; ++spin) {
+      const int n = spin ? Y.nd : Y.nu;
+      T* ai = L + Y.ainv(spin);
+      for (int k = 0; k < n; ++k) {
+        const int e = S.sector_elec[spin][k];
+        T z[3];
+        Rng<T>::normals3(philox(wid, step, 2u * (uint32_t)e, RNG_SINGLE, k0, k1), z);
+        const T U = Rng<T>::uniform(philox(wid, step, 2u * (uint32_t)e + 1u, RNG_SINGLE, k0, k1).x);
+        const T ox = pos[3 * e], oy = pos[3 * e + 1], oz = pos[3 * e + 2];
+        const T nx = ox + sigma * z[0], ny = oy + sigma * z[1], nz = oz + sigma * z[2];
+        // AO values of the proposal: one shell per lane
+        for (int sh = lane; sh < S.n_shell; sh += 64) shell_values(S, sh, nx, ny, nz, chi);
+        __syncthreads();
+        // MO values and the ratio R = sum_m phi_m ainv[m][k]
+        T part = 0;
+        for (int m = lane; m < n; m += 64) {
+          T v = 0;
+          for (int ao = 0; ao < Y.A; ++ao) v += S.moT[spin][ao][m] * chi[ao];
+          phi[m] = v;
+          part += v * ai[m * n + k];
+        }
+        const T R = wsum(part);
+        // Jastrow difference: one partner electron or nucleus per lane
+        T du = 0;
+        for (int j = lane; j < Y.N; j += 64) {
+          if (j == e) continue;
+          const bool same = S.elec_spin[j] == spin;
+          T ax = nx - pos[3 * j], ay = ny - pos[3 * j + 1], az = nz - pos[3 * j + 2];
+          T bx = ox - pos[3 * j], by = oy - pos[3 * j + 1], bz = oz - pos[3 * j + 2];
+          du += uee_val(S, M<T>::sq(ax * ax + ay * ay + az * az), same) -
+                uee_val(S, M<T>::sq(bx * bx + by * by + bz * bz), same);
+        }
+        if (S.en_on)
+          for (int I = lane; I < S.n_nuc; I += 64) {
+            T ax = nx - S.nuc[I][0], ay = ny - S.nuc[I][1], az = nz - S.nuc[I][2];
+            T bx = ox - S.nuc[I][0], by = oy - S.nuc[I][1], bz = oz - S.nuc[I][2];
+            du += uen_val(S, M<T>::sq(ax * ax + ay * ay + az * az), I) -
+                  uen_val(S, M<T>::sq(bx * bx + by * by + bz * bz), I);
+          }
+        du = wsum(du);
+        const T p = R * R * M<T>::ex(T(2) * du);
+        __syncthreads();  // phi complete, chi reads done
+        if (U < p) {       // wave-uniform
+          const T iR = M<T>::rcp(R);
+          for (int j = lane; j < n; j += 64) {
+            T v = 0;
+            for (int m = 0; m < n; ++m) v += phi[m] * ai[m * n + j];
+            wv[j] = v * iR;
+            cv[j] = ai[j * n + k];
+          }
+          __syncthreads();
+          for (int idx = lane; idx < n * n; idx += 64) {
+            const int m = idx / n, j = idx % n;
+            ai[idx] = (j == k) ? cv[m] * iR : ai[idx] - cv[m] * wv[j];
+          }
+          if (lane == 0) {
+            pos[3 * e] = nx;
+            pos[3 * e + 1] = ny;
+            pos[3 * e + 2] = nz;
+          }
+          logdet += (double)M<T>::lg(R < T(0) ? -R : R);
+          if (R < T(0)) sgn = -sgn;
+          ju += (double)du;
+          ++acc;
+          __syncthreads();
+        }
+      }
+    }
+    el = wave_local_energy(S, Y, L, nullptr, nullptr);
+    if (lane == 0) {
+      if (el_trace) el_trace[(int64_t)s * A.trace_ld + w] = el;
+      double* pp = partial + ((int64_t)s * gridDim.x + blockIdx.x) * 3;
+      pp[0] = el;
+      pp[1] = el * el;
+      pp[2] = (double)acc;
+    }
+    __syncthreads();
+  }
+  store_walker(E, Y, w, L);
+  if (lane == 0) {
+    E.logdet[w] = logdet;
+    E.sign[w] = sgn;
+    E.ju[w] = ju;
+    E.eloc[w] = el;
+  }
+}
+
+// per-configuration evaluation from AoS input
+template <class T>
+__global__ void __launch_bounds__(64) wave_eval_kernel(const DevSys<double>* __restrict__ Sd,
+                                                       const DevSys<T>* __restrict__ St, int64_t W,
+                                                       const double* __restrict__ in, WaveLayout Y, EvalOut O,
+                                                       int* singular_count) {
+  extern __shared__ double lds_d[];
+  const int64_t w = blockIdx.x;
+  T* L = (T*)lds_d;
+  double* a = lds_d + (Y.total() * sizeof(T) + 7) / 8;
+  double* inv = a + Y.nmax * Y.nmax;
+  double* pd = inv + Y.nmax * Y.nmax;  // FP64 copy of the positions
+  for (int q = threadIdx.x; q < 3 * Y.N; q += 64) {
+    pd[q] = in[w * 3 * Y.N + q];
+    L[Y.pos() + q] = (T)pd[q];
+  }
+  __syncthreads();
+  double ltot = 0.0, stot = 1.0;
+  bool sing = false;
+  for (int spin = 0; spin < 2; ++spin) {
+    const int n = spin ? Y.nd : Y.nu;
+    double la, sg;
+    sing |= wave_sector_inverse(*Sd, spin, pd, n, a, inv, la, sg);
+    ltot += la;
+    stot *= sg;
+    for (int q = threadIdx.x; q < n * n; q += 64) L[Y.ainv(spin) + q] = (T)inv[q];
+    __syncthreads();
+  }
+  const double ju = wave_jastrow_u(*Sd, pd, Y.N);
+  double* g = O.grad ? O.grad + w * 3 * Y.N : nullptr;
+  double* l = O.lap ? O.lap + w * Y.N : nullptr;
+  double el = 0.0;
+  if (O.eloc || O.grad || O.lap) el = wave_local_energy(*St, Y, L, g, l);
+  if (threadIdx.x == 0) {
+    if (O.logabs) O.logabs[w] = ltot + ju;
+    if (O.sign) O.sign[w] = stot;
+    if (O.eloc) O.eloc[w] = el;
+    if (sing && singular_count) atomicAdd(singular_count, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static WaveLayout layout(int N, int nu, int nd, int A) {
+  WaveLayout Y;
+  Y.N = N;
+  Y.nu = nu;
+  Y.nd = nd;
+  Y.nmax = nu > nd ? nu : nd;
+  Y.A = A;
+  return Y;
+}
+
+template <class T>
+static size_t lds_bytes(const WaveLayout& Y, bool fp64_scratch) {
+  size_t b = ((size_t)Y.total() * sizeof(T) + 7) / 8 * 8;
+  if (fp64_scratch) b += sizeof(double) * (2 * (size_t)Y.nmax * Y.nmax + 3 * (size_t)Y.N);
+  return b;
+}
+
+template <class K>
+static hipError_t set_lds(K kernel, size_t bytes) {
+  if (bytes > 160 * 1024) return hipErrorInvalidValue;
+  return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <class T>
+hipError_t wave_build(const void* Sd, const void* St, EnsView<T> E, WaveShape sh, int want_eloc, int* sing,
+                      hipStream_t st) {
+  WaveLayout Y = layout(sh.N, sh.nu, sh.nd, sh.A);
+  const size_t b = lds_bytes<T>(Y, true);
+  hipError_t e = set_lds(wave_build_kernel<T>, b);
+  if (e != hipSuccess) return e;
+  wave_build_kernel<T><<<(unsigned)E.W, 64, b, st>>>((const DevSys<double>*)Sd, (const DevSys<T>*)St, E, Y,
+                                                     want_eloc, sing);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t wave_sweep(const void* St, EnsView<T> E, WaveShape sh, SweepArgs A, double* partial, double* trace,
+                      hipStream_t st) {
+  WaveLayout Y = layout(sh.N, sh.nu, sh.nd, sh.A);
+  const size_t b = lds_bytes<T>(Y, false);
+  hipError_t e = set_lds(wave_sweep_kernel<T>, b);
+  if (e != hipSuccess) return e;
+  wave_sweep_kernel<T><<<(unsigned)E.W, 64, b, st>>>((const DevSys<T>*)St, E, Y, A, partial, trace);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t wave_eval(const void* Sd, const void* St, int64_t W, const double* pos, WaveShape sh, EvalOut O,
+                     int* sing, hipStream_t st) {
+  WaveLayout Y = layout(sh.N, sh.nu, sh.nd, sh.A);
+  const size_t b = lds_bytes<T>(Y, true);
+  hipError_t e = set_lds(wave_eval_kernel<T>, b);
+  if (e != hipSuccess) return e;
+  wave_eval_kernel<T><<<(unsigned)W, 64, b, st>>>((const DevSys<double>*)Sd, (const DevSys<T>*)St, W, pos, Y, O,
+                                                  sing);
+  return hipGetLastError();
+}
+
+template hipError_t wave_build<double>(const void*, const void*, EnsView<double>, WaveShape, int, int*, hipStream_t);
+template hipError_t wave_build<float>(const void*, const void*, EnsView<float>, WaveShape, int, int*, hipStream_t);
+template hipError_t wave_sweep<double>(const void*, EnsView<double>, WaveShape, SweepArgs, double*, double*,
+                                       hipStream_t);
+template hipError_t wave_sweep<float>(const void*, EnsView<float>, WaveShape, SweepArgs, double*, double*,
+                                      hipStream_t);
+template hipError_t wave_eval<double>(const void*, const void*, int64_t, const double*, WaveShape, EvalOut, int*,
+                                      hipStream_t);
+template hipError_t wave_eval<float>(const void*, const void*, int64_t, const double*, WaveShape, EvalOut, int*,
+                                     hipStream_t);
+
+}  // namespace qmc

@@ -1,0 +1,102 @@
+"""GPU parity of the wave engine (one wavefront per walker, state in LDS):
+benzene (BenzeneGTO, benzene.rs) against the oracle, and the smaller
+systems forced onto the wave engine against both the oracle and the
+register engine."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def Q():
+    import rust_qmc_amd as Q
+    Q.lib()
+    return Q
+
+
+def _close(a, b, rtol, scale=None):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    s = np.abs(b) if scale is None else scale
+    return np.all(np.abs(a - b) <= rtol * np.maximum(s, 1e-300) + rtol * 1e-3)
+
+
+@pytest.mark.parametrize("ch", [2.042, 2.05])
+def test_benzene_eval_parity_fp64(Q, oracle, ch):
+    g = Q.BenzeneGTO(1.5, 2.0, 2.64, ch)
+    o = oracle.System.benzene(1.5, 2.0, 2.64, ch)
+    assert g.nuclear_repulsion() == pytest.approx(o.nuclear_repulsion(), rel=1e-14)
+    pos = o.initialize_batch(2, 12)
+    ref = o.batch_eval(pos)
+    out = g.evaluate_batch(pos)
+    assert np.array_equal(out["sign"], ref["sign"])
+    assert _close(out["logabs"], ref["logabs"], 1e-10)
+    gs = np.abs(ref["grad"]).max(axis=(1, 2))[:, None, None]
+    assert _close(out["grad"], ref["grad"], 1e-10, scale=gs)
+    ls = np.abs(ref["lap"]).max(axis=1)[:, None]
+    assert _close(out["lap"], ref["lap"], 1e-10, scale=ls)
+    assert _close(out["eloc"], ref["eloc"], 1e-10)
+
+
+def test_benzene_mixed_precision_error(Q, oracle):
+    g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="mixed")
+    o = oracle.System.benzene(1.5, 2.0, 2.64, 2.05)
+    pos = o.initialize_batch(5, 8)
+    ref = o.batch_eval(pos)
+    out = g.evaluate_batch(pos)
+    assert _close(out["logabs"], ref["logabs"], 1e-12)
+    err = np.abs(out["eloc"] - ref["eloc"])
+    assert np.all(err <= 2e-4 * np.abs(ref["eloc"]) + 1e-2), err.max()
+
+
+def test_benzene_trajectory_parity(Q, oracle):
+    g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05)
+    o = oracle.System.benzene(1.5, 2.0, 2.64, 2.05)
+    W, steps, seed, step = 3, 2, 7, 0.2
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    es, e2, acc, tr = ens.sweeps(steps, step, trace=True, refresh_interval=1)
+    pos_o, lp_o, el_o, acc_o = o.single_electron_steps(p0, seed, 0, steps, step)
+    assert acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=1e-9)
+    assert _close(tr, el_o, 1e-9)
+
+
+@pytest.mark.parametrize("name", ["h2", "li_seeded", "ch4_sto3g"])
+def test_wave_engine_matches_register_engine(Q, oracle, name):
+    from test_gpu_parity import _configs, _pair
+    g, o = _pair(Q, oracle, name)
+    pos = _configs(o, 40)
+    reg = g.evaluate_batch(pos)
+    g.set_engine("wave")
+    wav = g.evaluate_batch(pos)
+    ref = o.batch_eval(pos)
+    for k in ("logabs", "eloc"):
+        assert _close(wav[k], ref[k], 1e-10), k
+        assert _close(wav[k], reg[k], 1e-11), k
+    W, steps, seed, step = 65, 6, 3, 0.4
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    es, e2, acc, tr = ens.sweeps(steps, step, trace=True, refresh_interval=4, steps_per_launch=3)
+    pos_o, lp_o, el_o, acc_o = o.single_electron_steps(p0, seed, 0, steps, step)
+    assert acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=1e-9)
+    assert _close(tr, el_o, 1e-9)
+
+
+def test_wave_engine_mixed_ensemble(Q, oracle):
+    """Mixed-precision wave-engine sweeps of benzene stay finite and agree
+    with FP64 sweeps of the same seed on the ensemble energy within 3 sigma."""
+    res = {}
+    for prec in ("fp64", "mixed"):
+        g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision=prec)
+        ens = Q.Ensemble(g, 512, 1)
+        ens.sweeps(20, 0.2)
+        es, e2, acc = ens.sweeps(40, 0.2, refresh_interval=20)
+        tr = es / 512
+        assert np.isfinite(tr).all()
+        tau = Q.autocorrelation_time(tr)
+        res[prec] = (tr.mean(), max(Q.blocking_error(tr, tau), tr.std() / np.sqrt(len(tr))))
+    (a, sa), (b, sb) = res["fp64"], res["mixed"]
+    assert abs(a - b) < 3 * np.hypot(sa, sb) + 1e-3
