@@ -341,6 +341,12 @@ class DeviceBuffer:
             raise QmcError(_lib.QMC_ERR_HIP, "hipMemcpy D2H failed")
         return out
 
+    def copy_from(self, other: "DeviceBuffer", nbytes: int):
+        """Device-to-device copy (runs as the HIP runtime's copy kernel)."""
+        assert nbytes <= self.nbytes and nbytes <= other.nbytes
+        if _hip().hipMemcpy(self.ptr, other.ptr, int(nbytes), 3) != 0:
+            raise QmcError(_lib.QMC_ERR_HIP, "hipMemcpy D2D failed")
+
     def free(self):
         if self.ptr:
             _hip().hipFree(self.ptr)

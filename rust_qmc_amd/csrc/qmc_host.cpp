@@ -1271,7 +1271,11 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
   int64_t nl = 0;
   while (done < p->n_steps) {
     int ns = std::min(chunk, p->n_steps - done);
-    if (p->move == QMC_MOVE_SINGLE && refresh > 0) ns = std::min(ns, refresh - e->since_refresh);
+    if (p->move == QMC_MOVE_SINGLE && refresh > 0) {
+      // an earlier call may have run past this call's (shorter) interval
+      if (e->since_refresh >= refresh && (rc = ens_refresh(e, 0))) return rc;
+      ns = std::min(ns, refresh - e->since_refresh);
+    }
     SweepArgs A{ns, e->step, e->seed, 0, p->step_size, e->W};
     double* trace = tr.dev ? (double*)tr.dev + (size_t)done * e->W : nullptr;
     if (e->timing) {
