@@ -36,6 +36,9 @@ METRIC = "walker-moves/s and SR-grad samples/s, mixed vs FP64 on 1 GPU at ≤0.1
 # (FMA = 2, exp/sqrt/div = 1).
 FLOP_PER_WM = 16338
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector FP64 / FP32
+# measured HBM traffic of one mixed-precision sweep launch at 65,536 walkers
+# and 16 sweeps per launch (22,680 KiB read + 23,296 KiB written)
+TRAFFIC_BYTES = (22680 + 23296) * 1024
 
 
 def run_mode(Q, precision, W, steps, warmup, spl, refresh):
@@ -176,7 +179,11 @@ def main():
             "peak": PEAK["mixed"],
             "unit": "TFLOP/s",
             "frac": achieved / PEAK["mixed"],
-            "traffic": None,
+            # HBM bytes per launch of the mixed sweep (16 sweeps x 65,536
+            # walkers) from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
+            # factors from a copy kernel of known size
+            "traffic": TRAFFIC_BYTES if (W == 65536 and args.steps_per_launch == 16) else None,
+            "traffic_unit": "bytes per launch",
             "kernel": "sweep_kernel<5,5,float>",
             "flop_per_walker_move": FLOP_PER_WM,
         },

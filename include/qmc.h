@@ -143,11 +143,18 @@ int qmc_set_precision(qmc_system* sys, int32_t precision);
 /* Engine selection.  The register engine keeps one walker per lane in
  * VGPRs (compiled electron counts up to 5 + 5); the wave engine keeps one
  * walker per wavefront in LDS (any count up to 32 per spin).  AUTO picks
- * the register engine when it exists for the system. */
+ * the register engine when it exists for the system.  The register engine
+ * uses kernels specialised to the system's shell structure when one is
+ * compiled (H2, both Li atoms, CH4 STO-3G and 6-31G); GENERIC forces its
+ * kernels with run-time shell loops. */
 #define QMC_ENGINE_AUTO 0
 #define QMC_ENGINE_REGISTER 1
 #define QMC_ENGINE_WAVE 2
+#define QMC_ENGINE_GENERIC 3
 int qmc_set_engine(qmc_system* sys, int32_t engine);
+/* Name of the kernels a system runs on: "wave", "register/generic" or
+ * "register/<shell structure>" (no reference counterpart; diagnostics). */
+int qmc_engine_name(const qmc_system* sys, char* buf, int32_t len);
 int qmc_set_flags(qmc_system* sys, uint32_t flags);
 int qmc_system_info(const qmc_system* sys, int32_t* n_elec, int32_t* n_up, int32_t* n_nuc, int32_t* n_ao);
 /* MO matrix of one spin (0 up, 1 down) as [n_mo][n_ao] */

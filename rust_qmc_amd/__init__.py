@@ -113,8 +113,15 @@ class System:
         check(lib().qmc_set_flags(self._h, flags))
 
     def set_engine(self, engine):
-        """'auto', 'register' (one walker per lane) or 'wave' (one walker per wavefront)."""
-        check(lib().qmc_set_engine(self._h, {"auto": 0, "register": 1, "wave": 2}[engine]))
+        """'auto', 'register' (one walker per lane), 'wave' (one walker per
+        wavefront) or 'generic' (register engine with run-time shell loops)."""
+        check(lib().qmc_set_engine(self._h, {"auto": 0, "register": 1, "wave": 2, "generic": 3}[engine]))
+
+    def engine_name(self):
+        """The kernels this system runs on, e.g. 'register/ch4-sto3g'."""
+        buf = C.create_string_buffer(64)
+        check(lib().qmc_engine_name(self._h, buf, 64))
+        return buf.value.decode()
 
     def nuclear_repulsion(self):
         v = C.c_double()

@@ -35,7 +35,7 @@ QMC_MOVE_SINGLE, QMC_MOVE_ALL = 0, 1
 EXPORTED = [
     "qmc_system_create", "qmc_system_destroy", "qmc_h2_molecule_mo_create", "qmc_sto_slater_det_create",
     "qmc_lithium_seeded_create", "qmc_methane_gto_create", "qmc_benzene_gto_create", "qmc_set_precision",
-    "qmc_set_engine",
+    "qmc_set_engine", "qmc_engine_name",
     "qmc_set_flags", "qmc_system_info", "qmc_get_mo", "qmc_nuclear_repulsion", "qmc_num_params",
     "qmc_get_params", "qmc_set_params", "qmc_set_jastrow_series", "qmc_get_nuclei", "qmc_set_nuclei",
     "qmc_eval_log_psi", "qmc_grad_log_psi", "qmc_lap_over_psi", "qmc_local_energy", "qmc_evaluate",
@@ -118,6 +118,7 @@ _SIGS = {
     "qmc_benzene_gto_create": (_i32, [_d, _d, _d, _d, _i32, _pp]),
     "qmc_set_precision": (_i32, [_p, _i32]),
     "qmc_set_engine": (_i32, [_p, _i32]),
+    "qmc_engine_name": (_i32, [_p, C.c_char_p, _i32]),
     "qmc_set_flags": (_i32, [_p, _u32]),
     "qmc_system_info": (_i32, [_p, _p, _p, _p, _p]),
     "qmc_get_mo": (_i32, [_p, _i32, _p]),

@@ -5,64 +5,10 @@
 // compile-time indices only, so it lives in VGPRs.
 #pragma once
 
-#include <hip/hip_runtime.h>
-#include <stdint.h>
+#include "qmc_basis.hpp"
+#include "qmc_sys.hpp"
 
 namespace qmc {
-
-constexpr int MAXN = 16;      // nuclei
-constexpr int MAXSH = 96;     // shells
-constexpr int MAXAO = 128;    // atomic orbitals
-constexpr int MAXPRIM = 512;  // primitive slots over all shells
-constexpr int MAXMO = 32;     // molecular orbitals per spin
-constexpr int MAXK = 8;       // Pade series terms
-constexpr int MAXSPEC = 4;    // nuclear species of the e-n series
-constexpr int MAXE = 64;      // electrons
-
-// shell kinds
-enum : int { SH_S = 0, SH_P = 1, SH_SP = 2, SH_P1 = 3, SH_STO = 4, SH_SLATER1S = 5 };
-// Jastrow kinds
-enum : int { JAS_NONE = 0, JAS_PAIR = 1, JAS_PADE = 2 };
-// Philox streams (fourth counter word)
-enum : uint32_t { RNG_INIT = 1u, RNG_SINGLE = 2u, RNG_ALL = 3u, RNG_SYS = 4u, RNG_DRIFT = 5u };
-
-// flags (mirrors include/qmc.h)
-enum : uint32_t {
-  F_J2_GRAD_QUIRK = 1u << 0,
-  F_J2_LAP_QUIRK = 1u << 1,
-  F_HF_NN_REFSIGN = 1u << 2,
-  F_SINGULAR_IDENTITY = 1u << 3,
-};
-
-// Everything a kernel needs to know about the system.  All fields are read
-// with wave-uniform addresses, so they come through the scalar cache.
-template <class T>
-struct DevSys {
-  int n_elec, n_up, n_dn, n_nuc, n_shell, n_ao, n_mo;
-  int jas_kind, en_on, k_ee, k_en, n_species;
-  uint32_t flags;
-  T b_ee, b_en, a_anti, a_para, cst_ee, grad_scale;
-  T c_anti[MAXK], c_para[MAXK];
-  T d_en[MAXSPEC][MAXK];
-  int species[MAXN];
-  T nuc[MAXN][3];
-  T Z[MAXN];
-  double vnn;
-  int sh_kind[MAXSH], sh_center[MAXSH], sh_nprim[MAXSH], sh_prim0[MAXSH], sh_ao0[MAXSH], sh_m[MAXSH];
-  T prim_exp[MAXPRIM], coef_s[MAXPRIM], coef_p[MAXPRIM];
-  int prim_pow[MAXPRIM];
-  int sector_elec[2][MAXE];   // global index of the k-th electron of each spin sector
-  int elec_spin[MAXE];        // 0 up, 1 down
-  int elec_slot[MAXE];        // row of the electron in its sector
-  int init_nuc[MAXE];
-  T init_sigma;
-  T moT[2][MAXAO][MAXMO];     // [spin][ao][mo], mo fastest
-};
-
-template <int N>
-struct Cap {
-  static constexpr int v = N > 0 ? N : 1;
-};
 
 // ---------------------------------------------------------------------------
 // precision-dependent math: FP64 uses the correctly rounded library
@@ -153,26 +99,25 @@ __device__ __forceinline__ void mo_add(const DevSys<T>& S, int spin, int ao, T v
 }
 
 // phi[m] = sum_ao C[spin][m][ao] chi_ao(r), m < NS
-template <int NS, class T>
+template <int NS, class T, class Bs = RtBasis>
 __device__ __forceinline__ void mo_values(const DevSys<T>& S, int spin, T x, T y, T z, T phi[Cap<NS>::v]) {
 #pragma unroll
   for (int m = 0; m < NS; ++m) phi[m] = T(0);
-  for (int sh = 0; sh < S.n_shell; ++sh) {
-    const int c = S.sh_center[sh], kind = S.sh_kind[sh], p0 = S.sh_prim0[sh], np = S.sh_nprim[sh];
-    const int a0 = S.sh_ao0[sh];
+  Bs::shells(S, [&](auto sh) {
+    const int c = sh.center(), kind = sh.kind(), p0 = sh.p0(), np = sh.nprim(), a0 = sh.a0();
     T dx = x - S.nuc[c][0], dy = y - S.nuc[c][1], dz = z - S.nuc[c][2];
     T r2 = dx * dx + dy * dy + dz * dz;
     if (kind <= SH_P1) {
       T rs = T(0), rp = T(0);
       for (int k = 0; k < np; ++k) {
         T e = M<T>::ex(-S.prim_exp[p0 + k] * r2);
-        rs += S.coef_s[p0 + k] * e;
-        rp += S.coef_p[p0 + k] * e;
+        if (kind == SH_S || kind == SH_SP) rs += S.coef_s[p0 + k] * e;
+        if (kind != SH_S) rp += S.coef_p[p0 + k] * e;
       }
       if (kind == SH_S) {
         mo_add<NS>(S, spin, a0, rs, phi);
       } else if (kind == SH_P1) {
-        const int m = S.sh_m[sh];
+        const int m = sh.m();
         mo_add<NS>(S, spin, a0, (m == 0 ? dx : (m == 1 ? dy : dz)) * rp, phi);
       } else {
         int b = a0;
@@ -189,104 +134,12 @@ __device__ __forceinline__ void mo_values(const DevSys<T>& S, int spin, T x, T y
       T v = T(0);
       for (int k = 0; k < np; ++k) {
         T sp = T(1);
-        for (int q = 0; q < S.prim_pow[p0 + k]; ++q) sp *= s;
+        for (int q = 0; q < sh.pw(k); ++q) sp *= s;
         v += S.coef_s[p0 + k] * sp * M<T>::ex(-S.prim_exp[p0 + k] * s);
       }
       mo_add<NS>(S, spin, a0, v, phi);
     }
-  }
-}
-
-// gradient and Laplacian of every MO at r: g[m][3], l[m]
-template <int NS, class T>
-__device__ __forceinline__ void mo_grad_lap(const DevSys<T>& S, int spin, T x, T y, T z, T g[Cap<NS>::v][3],
-                                            T l[Cap<NS>::v]) {
-#pragma unroll
-  for (int m = 0; m < NS; ++m) {
-    g[m][0] = g[m][1] = g[m][2] = T(0);
-    l[m] = T(0);
-  }
-  auto add = [&](int ao, T gx, T gy, T gz, T lap) {
-#pragma unroll
-    for (int m = 0; m < NS; ++m) {
-      const T c = S.moT[spin][ao][m];
-      g[m][0] += c * gx;
-      g[m][1] += c * gy;
-      g[m][2] += c * gz;
-      l[m] += c * lap;
-    }
-  };
-  for (int sh = 0; sh < S.n_shell; ++sh) {
-    const int c = S.sh_center[sh], kind = S.sh_kind[sh], p0 = S.sh_prim0[sh], np = S.sh_nprim[sh];
-    const int a0 = S.sh_ao0[sh];
-    T dx = x - S.nuc[c][0], dy = y - S.nuc[c][1], dz = z - S.nuc[c][2];
-    T r2 = dx * dx + dy * dy + dz * dz;
-    if (kind <= SH_P1) {
-      // R = sum c e, Ra = sum c a e, Raa = sum c a^2 e (s and p coefficient sets)
-      T rs = 0, rsa = 0, rsaa = 0, rp = 0, rpa = 0, rpaa = 0;
-      for (int k = 0; k < np; ++k) {
-        const T a = S.prim_exp[p0 + k];
-        T e = M<T>::ex(-a * r2);
-        T cs = S.coef_s[p0 + k] * e, cp = S.coef_p[p0 + k] * e;
-        rs += cs;
-        rsa += cs * a;
-        rsaa += cs * a * a;
-        rp += cp;
-        rpa += cp * a;
-        rpaa += cp * a * a;
-      }
-      if (kind == SH_S || kind == SH_SP) {
-        // grad = -2 Ra d; lap = -6 Ra + 4 r^2 Raa
-        add(a0, T(-2) * rsa * dx, T(-2) * rsa * dy, T(-2) * rsa * dz, T(-6) * rsa + T(4) * r2 * rsaa);
-      }
-      if (kind != SH_S) {
-        // p_m = d_m R: grad_j = delta_jm R - 2 d_m Ra d_j; lap = d_m (-10 Ra + 4 r^2 Raa)
-        const T lr = T(-10) * rpa + T(4) * r2 * rpaa;
-        const int b = (kind == SH_SP) ? a0 + 1 : a0;
-        if (kind == SH_P1) {
-          const int m = S.sh_m[sh];
-          T dm = (m == 0 ? dx : (m == 1 ? dy : dz));
-          T gx = T(-2) * dm * rpa * dx + (m == 0 ? rp : T(0));
-          T gy = T(-2) * dm * rpa * dy + (m == 1 ? rp : T(0));
-          T gz = T(-2) * dm * rpa * dz + (m == 2 ? rp : T(0));
-          add(a0, gx, gy, gz, dm * lr);
-        } else {
-          add(b, T(-2) * dx * rpa * dx + rp, T(-2) * dx * rpa * dy, T(-2) * dx * rpa * dz, dx * lr);
-          add(b + 1, T(-2) * dy * rpa * dx, T(-2) * dy * rpa * dy + rp, T(-2) * dy * rpa * dz, dy * lr);
-          add(b + 2, T(-2) * dz * rpa * dx, T(-2) * dz * rpa * dy, T(-2) * dz * rpa * dz + rp, dz * lr);
-        }
-      }
-    } else {
-      // radial STO: R' and R''; grad = R' d/s, lap = R'' + 2 R'/s
-      T s = M<T>::sq(r2);
-      T d1 = 0, d2 = 0;
-      for (int k = 0; k < np; ++k) {
-        const T zt = S.prim_exp[p0 + k];
-        const int p = S.prim_pow[p0 + k];
-        const T pf = T(p);
-        T spm2 = T(1), sp = T(1);
-        // s^p, s^(p-1), s^(p-2) with the p = 0, 1 terms multiplied by zero
-        for (int q = 0; q < p; ++q) sp *= s;
-        T spm1 = p >= 1 ? sp / s : T(0);
-        spm2 = p >= 2 ? spm1 / s : T(0);
-        T e = S.coef_s[p0 + k] * M<T>::ex(-zt * s);
-        d1 += e * (-zt * sp + pf * spm1);
-        d2 += e * (zt * zt * sp - T(2) * zt * pf * spm1 + pf * (pf - T(1)) * spm2);
-      }
-      T gx, gy, gz, lap;
-      if (s == T(0)) {
-        gx = gy = gz = T(0);
-        lap = (kind == SH_SLATER1S) ? -INFINITY : T(0);
-      } else {
-        T is = T(1) / s;
-        gx = d1 * dx * is;
-        gy = d1 * dy * is;
-        gz = d1 * dz * is;
-        lap = d2 + T(2) * d1 * is;
-      }
-      add(a0, gx, gy, gz, lap);
-    }
-  }
+  });
 }
 
 // Determinant terms of one electron from AO derivatives contracted with the
@@ -294,36 +147,37 @@ __device__ __forceinline__ void mo_grad_lap(const DevSys<T>& S, int spin, T x, T
 // grad D / D = sum_ao w_ao grad chi_ao and lap D / D = sum_ao w_ao lap chi_ao.
 // on_shell(center, sx, sy, sz) receives each shell's sum_ao w_ao grad chi_ao
 // (minus d ln|D| / dR_center for this electron).
-template <int NS, class T, class OnShell>
-__device__ __forceinline__ void det_terms(const DevSys<T>& S, int spin, T x, T y, T z,
-                                          const T ai[Cap<NS>::v][Cap<NS>::v], T& gdx, T& gdy, T& gdz, T& ld,
-                                          OnShell on_shell);
-
-// the same with the AO weights w_ao supplied by wgt(ao)
-template <class T, class Wgt, class OnShell>
+// The same with the AO weights w_ao supplied by wgt(ao):
+template <class T, class Wgt, class OnShell, class Bs = RtBasis>
 __device__ __forceinline__ void det_terms_w(const DevSys<T>& S, T x, T y, T z, Wgt wgt, T& gdx, T& gdy, T& gdz,
                                             T& ld, OnShell on_shell) {
   gdx = gdy = gdz = ld = T(0);
-  for (int sh = 0; sh < S.n_shell; ++sh) {
-    const int c = S.sh_center[sh], kind = S.sh_kind[sh], p0 = S.sh_prim0[sh], np = S.sh_nprim[sh];
-    const int a0 = S.sh_ao0[sh];
+  Bs::shells(S, [&](auto sh) {
+    const int c = sh.center(), kind = sh.kind(), p0 = sh.p0(), np = sh.nprim(), a0 = sh.a0();
     T dx = x - S.nuc[c][0], dy = y - S.nuc[c][1], dz = z - S.nuc[c][2];
     T r2 = dx * dx + dy * dy + dz * dz;
     T sx = 0, sy = 0, sz = 0, sl = 0;
     if (kind <= SH_P1) {
+      // R = sum c e, Ra = sum c a e, Raa = sum c a^2 e (s and p coefficient sets)
       T rs = 0, rsa = 0, rsaa = 0, rp = 0, rpa = 0, rpaa = 0;
       for (int k = 0; k < np; ++k) {
         const T a = S.prim_exp[p0 + k];
         T e = M<T>::ex(-a * r2);
-        T cs = S.coef_s[p0 + k] * e, cp = S.coef_p[p0 + k] * e;
-        rs += cs;
-        rsa += cs * a;
-        rsaa += cs * a * a;
-        rp += cp;
-        rpa += cp * a;
-        rpaa += cp * a * a;
+        if (kind == SH_S || kind == SH_SP) {
+          T cs = S.coef_s[p0 + k] * e;
+          rs += cs;
+          rsa += cs * a;
+          rsaa += cs * a * a;
+        }
+        if (kind != SH_S) {
+          T cp = S.coef_p[p0 + k] * e;
+          rp += cp;
+          rpa += cp * a;
+          rpaa += cp * a * a;
+        }
       }
       if (kind == SH_S || kind == SH_SP) {
+        // grad = -2 Ra d; lap = -6 Ra + 4 r^2 Raa
         const T w = wgt(a0);
         const T g = T(-2) * rsa * w;
         sx += g * dx;
@@ -336,7 +190,7 @@ __device__ __forceinline__ void det_terms_w(const DevSys<T>& S, T x, T y, T z, W
         const T lr = T(-10) * rpa + T(4) * r2 * rpaa;
         T wx, wy, wz;
         if (kind == SH_P1) {
-          const int m = S.sh_m[sh];
+          const int m = sh.m();
           const T w = wgt(a0);
           wx = m == 0 ? w : T(0);
           wy = m == 1 ? w : T(0);
@@ -355,12 +209,14 @@ __device__ __forceinline__ void det_terms_w(const DevSys<T>& S, T x, T y, T z, W
         sl += wd * lr;
       }
     } else {
+      // radial STO: R' and R''; grad = R' d/s, lap = R'' + 2 R'/s
       T s = M<T>::sq(r2);
       T d1 = 0, d2 = 0;
       for (int k = 0; k < np; ++k) {
         const T zt = S.prim_exp[p0 + k];
-        const int p = S.prim_pow[p0 + k];
+        const int p = sh.pw(k);
         const T pf = T(p);
+        // s^p, s^(p-1), s^(p-2) with the p = 0, 1 terms multiplied by zero
         T sp = T(1);
         for (int q = 0; q < p; ++q) sp *= s;
         T spm1 = p >= 1 ? sp / s : T(0);
@@ -386,10 +242,10 @@ __device__ __forceinline__ void det_terms_w(const DevSys<T>& S, T x, T y, T z, W
     gdz += sz;
     ld += sl;
     on_shell(c, sx, sy, sz);
-  }
+  });
 }
 
-template <int NS, class T, class OnShell>
+template <int NS, class T, class OnShell, class Bs = RtBasis>
 __device__ __forceinline__ void det_terms(const DevSys<T>& S, int spin, T x, T y, T z,
                                           const T ai[Cap<NS>::v][Cap<NS>::v], T& gdx, T& gdy, T& gdz, T& ld,
                                           OnShell on_shell) {
@@ -399,7 +255,7 @@ __device__ __forceinline__ void det_terms(const DevSys<T>& S, int spin, T x, T y
     for (int m = 0; m < NS; ++m) w += S.moT[spin][ao][m] * ai[m][0];
     return w;
   };
-  det_terms_w(S, x, y, z, wgt, gdx, gdy, gdz, ld, on_shell);
+  det_terms_w<T, decltype(wgt), OnShell, Bs>(S, x, y, z, wgt, gdx, gdy, gdz, ld, on_shell);
 }
 
 // AO values of one shell at (x, y, z), written to chi[ao - base] for the

@@ -40,7 +40,7 @@ __device__ __forceinline__ void rotate(T ra[Cap<NA>::v][3], T ai[Cap<NA>::v][Cap
 // Per-electron local-energy terms of the electron in slot 0 of sector A.
 // Returns the contribution to -1/2 sum lap(psi)/psi and adds the electron's
 // share of the potential (1/2 of each e-e pair) to pot.
-template <int NA, int NB, class T>
+template <int NA, int NB, class T, class Bs = RtBasis>
 __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, const T ra[Cap<NA>::v][3],
                                                const T ai[Cap<NA>::v][Cap<NA>::v],
                                                const T rb[Cap<NB>::v][3], T& gx, T& gy, T& gz, T& lap,
@@ -48,7 +48,7 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
   const T x = ra[0][0], y = ra[0][1], z = ra[0][2];
   // determinant part: grad D / D and lap D / D from the inverse column
   T gdx, gdy, gdz, ld;
-  det_terms<NA>(S, spin, x, y, z, ai, gdx, gdy, gdz, ld, NoShell{});
+  det_terms<NA, T, NoShell, Bs>(S, spin, x, y, z, ai, gdx, gdy, gdz, ld, NoShell{});
   // Jastrow part
   T gux = 0, guy = 0, guz = 0, lu = 0, pairsq = 0;
   double pe = 0.0;
@@ -72,12 +72,13 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
   for (int j = 1; j < NA; ++j) ee(ra[j][0], ra[j][1], ra[j][2], true);
 #pragma unroll
   for (int j = 0; j < NB; ++j) ee(rb[j][0], rb[j][1], rb[j][2], false);
-  for (int I = 0; I < S.n_nuc; ++I) {
+  const bool en_on = S.en_on;
+  Bs::nuclei(S, [&](int I) {
     T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
     T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
     T ir = M<T>::rcp(r);
     pe -= (double)(S.Z[I] * ir);
-    if (S.en_on) {
+    if (en_on) {
       T u1, lp;
       uen_d(S, r, ir, I, u1, lp);
       if (r >= T(1e-10)) {
@@ -88,7 +89,7 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
         lu += lp;
       }
     }
-  }
+  });
   // Jastrow2's gradient carries an extra factor F (jastrow.rs:102) and its
   // Laplacian omits the cross terms (jastrow.rs:110-125) under the flags.
   const T gs = S.grad_scale;
@@ -105,7 +106,7 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
 // (methane.rs:1154-1182).  Sectors are left in their canonical order.
 // When gout/lout are given, per-electron grad ln psi and lap psi / psi are
 // written to gout[(e*3+c)*ld], lout[e*ld] for global electron index e.
-template <int NU, int ND, class T>
+template <int NU, int ND, class T, class Bs = RtBasis>
 __device__ __forceinline__ double local_energy(const DevSys<T>& S, T ru[Cap<NU>::v][3],
                                                T au[Cap<NU>::v][Cap<NU>::v], T rd[Cap<ND>::v][3],
                                                T ad[Cap<ND>::v][Cap<ND>::v], double* gout, double* lout,
@@ -114,7 +115,7 @@ __device__ __forceinline__ double local_energy(const DevSys<T>& S, T ru[Cap<NU>:
 #pragma unroll 1
   for (int k = 0; k < NU; ++k) {
     T gx, gy, gz, lap;
-    electron_terms<NU, ND>(S, 0, ru, au, rd, gx, gy, gz, lap, pot);
+    electron_terms<NU, ND, T, Bs>(S, 0, ru, au, rd, gx, gy, gz, lap, pot);
     kin += -0.5 * (double)lap;
     if (gout) {
       const int e = S.sector_elec[0][k];
@@ -128,7 +129,7 @@ __device__ __forceinline__ double local_energy(const DevSys<T>& S, T ru[Cap<NU>:
 #pragma unroll 1
   for (int k = 0; k < ND; ++k) {
     T gx, gy, gz, lap;
-    electron_terms<ND, NU>(S, 1, rd, ad, ru, gx, gy, gz, lap, pot);
+    electron_terms<ND, NU, T, Bs>(S, 1, rd, ad, ru, gx, gy, gz, lap, pot);
     kin += -0.5 * (double)lap;
     if (gout) {
       const int e = S.sector_elec[1][k];
@@ -143,7 +144,7 @@ __device__ __forceinline__ double local_energy(const DevSys<T>& S, T ru[Cap<NU>:
 }
 
 // Full Jastrow exponent ln J (FP64), including the pair constant.
-template <int NU, int ND, class T>
+template <int NU, int ND, class T, class Bs = RtBasis>
 __device__ __forceinline__ double jastrow_u(const DevSys<double>& S, const T ru[Cap<NU>::v][3],
                                             const T rd[Cap<ND>::v][3]) {
   constexpr int N = NU + ND;
@@ -168,15 +169,15 @@ __device__ __forceinline__ double jastrow_u(const DevSys<double>& S, const T ru[
   if (S.en_on)
 #pragma unroll
     for (int i = 0; i < N; ++i)
-      for (int I = 0; I < S.n_nuc; ++I) {
+      Bs::nuclei(S, [&](int I) {
         double dx = r[i][0] - S.nuc[I][0], dy = r[i][1] - S.nuc[I][1], dz = r[i][2] - S.nuc[I][2];
         u += uen_val(S, sqrt(dx * dx + dy * dy + dz * dz), I);
-      }
+      });
   return u;
 }
 
 // Build one sector's Slater matrix in FP64 and invert it.
-template <int NA, class T>
+template <int NA, class T, class Bs = RtBasis>
 __device__ __forceinline__ bool sector_inverse(const DevSys<double>& S, int spin, const T ra[Cap<NA>::v][3],
                                                T ai[Cap<NA>::v][Cap<NA>::v], double& logabs, double& sgn) {
   if (NA == 0) {
@@ -189,7 +190,7 @@ __device__ __forceinline__ bool sector_inverse(const DevSys<double>& S, int spin
 #pragma unroll
   for (int k = 0; k < NA; ++k) {
     double phi[C];
-    mo_values<C>(S, spin, (double)ra[k][0], (double)ra[k][1], (double)ra[k][2], phi);
+    mo_values<C, double, Bs>(S, spin, (double)ra[k][0], (double)ra[k][1], (double)ra[k][2], phi);
 #pragma unroll
     for (int m = 0; m < NA; ++m) a[k][m] = phi[m];
   }
@@ -204,7 +205,7 @@ __device__ __forceinline__ bool sector_inverse(const DevSys<double>& S, int spin
 // One sector's single-electron Metropolis moves (Gaussian proposal, ratio
 // from the inverse column, Jastrow difference from the moved electron's
 // pairs, Sherman-Morrison row update on acceptance).
-template <int NA, int NB, class T>
+template <int NA, int NB, class T, class Bs = RtBasis>
 __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[Cap<NA>::v][3],
                                              T ai[Cap<NA>::v][Cap<NA>::v], const T rb[Cap<NB>::v][3],
                                              uint32_t w, uint32_t step, uint32_t k0, uint32_t k1, T sigma,
@@ -218,7 +219,7 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
     const T ox = ra[0][0], oy = ra[0][1], oz = ra[0][2];
     const T nx = ox + sigma * z[0], ny = oy + sigma * z[1], nz = oz + sigma * z[2];
     T phi[Cap<NA>::v];
-    mo_values<NA>(S, spin, nx, ny, nz, phi);
+    mo_values<NA, T, Bs>(S, spin, nx, ny, nz, phi);
     T R = 0;
 #pragma unroll
     for (int m = 0; m < NA; ++m) R += phi[m] * ai[m][0];
@@ -235,12 +236,12 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
 #pragma unroll
     for (int j = 0; j < NB; ++j) pair(rb[j][0], rb[j][1], rb[j][2], false);
     if (S.en_on)
-      for (int I = 0; I < S.n_nuc; ++I) {
+      Bs::nuclei(S, [&](int I) {
         T ax = nx - S.nuc[I][0], ay = ny - S.nuc[I][1], az = nz - S.nuc[I][2];
         T bx = ox - S.nuc[I][0], by = oy - S.nuc[I][1], bz = oz - S.nuc[I][2];
         du += uen_val(S, M<T>::sq(ax * ax + ay * ay + az * az), I) -
               uen_val(S, M<T>::sq(bx * bx + by * by + bz * bz), I);
-      }
+      });
     const T p = R * R * M<T>::ex(T(2) * du);
     if (U < p) {
       // Sherman-Morrison: A'^-1 = A^-1 - (A^-1 e_0)(phi^T A^-1 - e_0^T)/R

@@ -97,9 +97,12 @@ struct qmc_system {
   int32_t precision = QMC_FP64;
   DevSys<double>* d64 = nullptr;  // device parameter blocks
   DevSys<float>* d32 = nullptr;
-  const EngineOps* ops = nullptr;
+  const EngineOps* ops = nullptr;      // register-engine kernels in use
+  const EngineOps* ops_spec = nullptr; // compiled shell structure matching the system, if any
+  const EngineOps* ops_gen = nullptr;  // generic kernels of the electron-count shape
   const DerivOps* dops = nullptr;
   int32_t engine = QMC_ENGINE_AUTO;
+  void pick_ops() { ops = (engine == QMC_ENGINE_GENERIC || !ops_spec) ? ops_gen : ops_spec; }
   bool dirty = true;
   uint64_t version = 0;  // bumped on every upload; ensembles refresh when it moves
   // the wave engine runs when selected, or when no register engine was
@@ -287,7 +290,10 @@ int upload(qmc_system* s) {
   if (!s->d32) HIP_TRY(hipMalloc(&s->d32, sizeof(DevSys<float>)));
   HIP_TRY(hipMemcpy(s->d64, d64.get(), sizeof(DevSys<double>), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->d32, d32.get(), sizeof(DevSys<float>), hipMemcpyHostToDevice));
-  s->ops = find_engine(s->h.n_up(), s->h.n_dn());
+  s->ops_spec = find_engine(*d64, false);
+  if (s->ops_spec && !s->ops_spec->match) s->ops_spec = nullptr;
+  s->ops_gen = find_engine(*d64, true);
+  s->pick_ops();
   s->dops = find_derivs(s->h.n_up(), s->h.n_dn());
   s->dirty = false;
   s->version++;
@@ -836,11 +842,19 @@ int qmc_set_precision(qmc_system* s, int32_t precision) {
 }
 
 int qmc_set_engine(qmc_system* s, int32_t engine) {
-  if (!s || engine < QMC_ENGINE_AUTO || engine > QMC_ENGINE_WAVE) return fail(QMC_ERR_INVALID, "bad engine");
-  if (engine == QMC_ENGINE_REGISTER && !s->ops)
+  if (!s || engine < QMC_ENGINE_AUTO || engine > QMC_ENGINE_GENERIC) return fail(QMC_ERR_INVALID, "bad engine");
+  if ((engine == QMC_ENGINE_REGISTER || engine == QMC_ENGINE_GENERIC) && !s->ops_gen)
     return fail(QMC_ERR_UNSUPPORTED, "no register engine for this electron count");
   s->engine = engine;
+  s->pick_ops();
   s->version++;  // ensembles rebuild their state with the new engine
+  return QMC_OK;
+}
+
+int qmc_engine_name(const qmc_system* s, char* buf, int32_t len) {
+  if (!s || !buf || len < 1) return fail(QMC_ERR_INVALID, "bad arguments");
+  std::string n = s->wave() ? std::string("wave") : std::string("register/") + (s->ops ? s->ops->name : "none");
+  std::snprintf(buf, (size_t)len, "%s", n.c_str());
   return QMC_OK;
 }
 

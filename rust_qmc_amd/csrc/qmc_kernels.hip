@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(64) init_positions_kernel(const DevSys<double>
 // state build / refresh: Slater matrices and inverses in FP64 from the
 // current positions, ln|D|, sign, Jastrow exponent; optionally E_L.
 // ---------------------------------------------------------------------------
-template <int NU, int ND, class T>
+template <int NU, int ND, class Bs, class T>
 __global__ void __launch_bounds__(64) build_state_kernel(const DevSys<double>* __restrict__ Sd,
                                                          const DevSys<T>* __restrict__ St, EnsView<T> E,
                                                          int want_eloc, int* singular_count) {
@@ -100,21 +100,21 @@ __global__ void __launch_bounds__(64) build_state_kernel(const DevSys<double>* _
   WalkerRegs<NU, ND, T> R;
   load_positions<NU, ND>(*St, E.pos, E.ld, w, R);
   double lu, su, ldn, sd;
-  bool s1 = sector_inverse<NU>(*Sd, 0, R.ru, R.au, lu, su);
-  bool s2 = sector_inverse<ND>(*Sd, 1, R.rd, R.ad, ldn, sd);
+  bool s1 = sector_inverse<NU, T, Bs>(*Sd, 0, R.ru, R.au, lu, su);
+  bool s2 = sector_inverse<ND, T, Bs>(*Sd, 1, R.rd, R.ad, ldn, sd);
   if ((s1 || s2) && singular_count) atomicAdd(singular_count, 1);
   store_inverse<NU, ND>(E.ainv, E.ld, w, R);
   E.logdet[w] = lu + ldn;
   E.sign[w] = su * sd;
-  E.ju[w] = jastrow_u<NU, ND>(*Sd, R.ru, R.rd);
-  if (want_eloc) E.eloc[w] = local_energy<NU, ND>(*St, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
+  E.ju[w] = jastrow_u<NU, ND, T, Bs>(*Sd, R.ru, R.rd);
+  if (want_eloc) E.eloc[w] = local_energy<NU, ND, T, Bs>(*St, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
 }
 
 // ---------------------------------------------------------------------------
 // persistent single-electron sweeps: n_steps sweeps, E_L after each, and
 // per-(step, block) partial sums {sum E_L, sum E_L^2, accepted moves}
 // ---------------------------------------------------------------------------
-template <int NU, int ND, class T>
+template <int NU, int ND, class Bs, class T>
 __global__ void __launch_bounds__(64) sweep_kernel(const DevSys<T>* __restrict__ Sp, EnsView<T> E,
                                                    SweepArgs A, double* __restrict__ partial,
                                                    double* __restrict__ el_trace) {
@@ -133,9 +133,9 @@ __global__ void __launch_bounds__(64) sweep_kernel(const DevSys<T>* __restrict__
   for (int s = 0; s < A.n_steps; ++s) {
     const uint32_t step = A.step0 + (uint32_t)s;
     int acc = 0;
-    sector_moves<NU, ND>(S, 0, R.ru, R.au, R.rd, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
-    sector_moves<ND, NU>(S, 1, R.rd, R.ad, R.ru, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
-    el = local_energy<NU, ND>(S, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
+    sector_moves<NU, ND, T, Bs>(S, 0, R.ru, R.au, R.rd, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
+    sector_moves<ND, NU, T, Bs>(S, 1, R.rd, R.ad, R.ru, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
+    el = local_energy<NU, ND, T, Bs>(S, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
     if (el_trace && live) el_trace[(int64_t)s * A.trace_ld + w] = el;
     double e1 = live ? el : 0.0, e2 = live ? el * el : 0.0, a = live ? (double)acc : 0.0;
     e1 = wave_sum(e1);
@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(64) sweep_kernel(const DevSys<T>* __restrict__
 // every electron moved at once, full recomputation of ln|psi|, E_L
 // recomputed on acceptance only and cached otherwise.
 // ---------------------------------------------------------------------------
-template <int NU, int ND, class T>
+template <int NU, int ND, class Bs, class T>
 __global__ void __launch_bounds__(64) all_electron_kernel(const DevSys<double>* __restrict__ Sd,
                                                           const DevSys<T>* __restrict__ Sp, EnsView<T> E,
                                                           SweepArgs A, double* __restrict__ partial,
@@ -196,17 +196,25 @@ __global__ void __launch_bounds__(64) all_electron_kernel(const DevSys<double>* 
       for (int c = 0; c < 3; ++c) Q.rd[k][c] = R.rd[k][c] + sigma * z[c];
     }
     double lu, su, ldn, sd;
-    sector_inverse<NU>(*Sd, 0, Q.ru, Q.au, lu, su);
-    sector_inverse<ND>(*Sd, 1, Q.rd, Q.ad, ldn, sd);
-    const double jun = jastrow_u<NU, ND>(*Sd, Q.ru, Q.rd);
+    sector_inverse<NU, T, Bs>(*Sd, 0, Q.ru, Q.au, lu, su);
+    sector_inverse<ND, T, Bs>(*Sd, 1, Q.rd, Q.ad, ldn, sd);
+    const double jun = jastrow_u<NU, ND, T, Bs>(*Sd, Q.ru, Q.rd);
     const double U = u01d(philox(wid, step, N, RNG_ALL, k0, k1).x);
     int acc = 0;
     if (U < exp(2.0 * ((lu + ldn + jun) - (ldet + ju)))) {
-      R = Q;
+      // only the positions persist; the inverses serve this E_L and go
+#pragma unroll
+      for (int k = 0; k < NU; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R.ru[k][c] = Q.ru[k][c];
+#pragma unroll
+      for (int k = 0; k < ND; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R.rd[k][c] = Q.rd[k][c];
       ldet = lu + ldn;
       ju = jun;
       sgn = su * sd;
-      el = local_energy<NU, ND>(S, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
+      el = local_energy<NU, ND, T, Bs>(S, Q.ru, Q.au, Q.rd, Q.ad, nullptr, nullptr, 0);
       acc = 1;
     }
     if (el_trace && live) el_trace[(int64_t)s * A.trace_ld + w] = el;
@@ -235,7 +243,7 @@ __global__ void __launch_bounds__(64) all_electron_kernel(const DevSys<double>* 
 // positions are [W][N][3] FP64 in global electron order (the reference's
 // Vec<Vector3<f64>> layout); outputs are written walker-major.
 // ---------------------------------------------------------------------------
-template <int NU, int ND, class T>
+template <int NU, int ND, class Bs, class T>
 __global__ void __launch_bounds__(64) eval_kernel(const DevSys<double>* __restrict__ Sd,
                                                   const DevSys<T>* __restrict__ St, int64_t W,
                                                   const double* __restrict__ pos, EvalOut O, int* singular_count) {
@@ -255,10 +263,10 @@ __global__ void __launch_bounds__(64) eval_kernel(const DevSys<double>* __restri
 #pragma unroll
     for (int c = 0; c < 3; ++c) D.rd[k][c] = pos[(w * N + S.sector_elec[1][k]) * 3 + c];
   double lu, su, ldn, sd;
-  bool s1 = sector_inverse<NU>(*Sd, 0, D.ru, D.au, lu, su);
-  bool s2 = sector_inverse<ND>(*Sd, 1, D.rd, D.ad, ldn, sd);
+  bool s1 = sector_inverse<NU, double, Bs>(*Sd, 0, D.ru, D.au, lu, su);
+  bool s2 = sector_inverse<ND, double, Bs>(*Sd, 1, D.rd, D.ad, ldn, sd);
   if ((s1 || s2) && singular_count) atomicAdd(singular_count, 1);
-  if (O.logabs) O.logabs[w] = lu + ldn + jastrow_u<NU, ND>(*Sd, D.ru, D.rd);
+  if (O.logabs) O.logabs[w] = lu + ldn + jastrow_u<NU, ND, double, Bs>(*Sd, D.ru, D.rd);
   if (O.sign) O.sign[w] = su * sd;
   WalkerRegs<NU, ND, T> R;
 #pragma unroll
@@ -279,7 +287,7 @@ __global__ void __launch_bounds__(64) eval_kernel(const DevSys<double>* __restri
     // per-electron terms land in private arrays in global electron order
     const bool per_e = O.grad || O.lap;
     double g[3 * (N > 0 ? N : 1)], l[N > 0 ? N : 1];
-    double el = local_energy<NU, ND>(S, R.ru, R.au, R.rd, R.ad, per_e ? g : nullptr, per_e ? l : nullptr, 1);
+    double el = local_energy<NU, ND, T, Bs>(S, R.ru, R.au, R.rd, R.ad, per_e ? g : nullptr, per_e ? l : nullptr, 1);
     if (O.eloc) O.eloc[w] = el;
     if (O.grad)
 #pragma unroll
@@ -320,58 +328,81 @@ static hipError_t launch_init(const void* Sd, EnsView<T> E, uint64_t seed, int64
   return hipGetLastError();
 }
 
-template <int NU, int ND, class T>
+template <int NU, int ND, class Bs, class T>
 struct Launch {
   static hipError_t build(const void* Sd, const void* St, EnsView<T> E, int want_eloc, int* sing,
                           hipStream_t st) {
-    build_state_kernel<NU, ND, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<double>*)Sd,
-                                                              (const DevSys<T>*)St, E, want_eloc, sing);
+    build_state_kernel<NU, ND, Bs, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<double>*)Sd,
+                                                                  (const DevSys<T>*)St, E, want_eloc, sing);
     return hipGetLastError();
   }
   static hipError_t sweep(const void* St, EnsView<T> E, SweepArgs A, double* partial, double* trace,
                           hipStream_t st) {
-    sweep_kernel<NU, ND, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<T>*)St, E, A, partial, trace);
+    sweep_kernel<NU, ND, Bs, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<T>*)St, E, A, partial, trace);
     return hipGetLastError();
   }
   static hipError_t all_electron(const void* Sd, const void* St, EnsView<T> E, SweepArgs A, double* partial,
                                  double* trace, hipStream_t st) {
-    all_electron_kernel<NU, ND, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<double>*)Sd,
-                                                               (const DevSys<T>*)St, E, A, partial, trace);
+    all_electron_kernel<NU, ND, Bs, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<double>*)Sd,
+                                                                   (const DevSys<T>*)St, E, A, partial, trace);
     return hipGetLastError();
   }
   static hipError_t eval(const void* Sd, const void* St, int64_t W, const double* pos, EvalOut O, int* sing,
                          hipStream_t st) {
-    eval_kernel<NU, ND, T><<<nblocks(W), 64, 0, st>>>((const DevSys<double>*)Sd, (const DevSys<T>*)St, W,
-                                                     pos, O, sing);
+    eval_kernel<NU, ND, Bs, T><<<nblocks(W), 64, 0, st>>>((const DevSys<double>*)Sd, (const DevSys<T>*)St, W,
+                                                         pos, O, sing);
     return hipGetLastError();
   }
 };
 
-template <int NU, int ND>
-static EngineOps make_ops() {
+template <int NU, int ND, class Bs>
+static EngineOps make_ops(const char* name, bool (*match)(const DevSys<double>&)) {
   EngineOps o;
   o.nu = NU;
   o.nd = ND;
-  o.build64 = &Launch<NU, ND, double>::build;
-  o.build32 = &Launch<NU, ND, float>::build;
-  o.sweep64 = &Launch<NU, ND, double>::sweep;
-  o.sweep32 = &Launch<NU, ND, float>::sweep;
-  o.ae64 = &Launch<NU, ND, double>::all_electron;
-  o.ae32 = &Launch<NU, ND, float>::all_electron;
-  o.eval64 = &Launch<NU, ND, double>::eval;
-  o.eval32 = &Launch<NU, ND, float>::eval;
+  o.name = name;
+  o.match = match;
+  o.build64 = &Launch<NU, ND, Bs, double>::build;
+  o.build32 = &Launch<NU, ND, Bs, float>::build;
+  o.sweep64 = &Launch<NU, ND, Bs, double>::sweep;
+  o.sweep32 = &Launch<NU, ND, Bs, float>::sweep;
+  // All-electron moves always run the run-time-loop kernels: the CH4/STO-3G
+  // all-electron kernel specialised to its shell structure disagreed with
+  // them (and with the oracle) from the first step while every other
+  // specialised kernel agrees, and no cause has been found in the source.
+  o.ae64 = &Launch<NU, ND, RtBasis, double>::all_electron;
+  o.ae32 = &Launch<NU, ND, RtBasis, float>::all_electron;
+  o.eval64 = &Launch<NU, ND, Bs, double>::eval;
+  o.eval32 = &Launch<NU, ND, Bs, float>::eval;
   return o;
 }
 
-// electron-count shapes compiled for the register engine
+template <int NU, int ND>
+static EngineOps generic_ops() {
+  return make_ops<NU, ND, RtBasis>("generic", nullptr);
+}
+
+template <class Bd>
+static EngineOps compiled_ops() {
+  return make_ops<Bd::nu, Bd::nd, CtBasis<Bd>>(Bd::name, &basis_matches<Bd, double>);
+}
+
+// Shell structures compiled into the register engine (tried first), then
+// the electron-count shapes with run-time shell loops.
 static const EngineOps kOps[] = {
-    make_ops<1, 0>(), make_ops<1, 1>(), make_ops<2, 1>(), make_ops<2, 2>(), make_ops<3, 2>(),
-    make_ops<3, 3>(), make_ops<4, 4>(), make_ops<5, 5>(),
+    compiled_ops<BdH2>(),       compiled_ops<BdLiSeeded>(), compiled_ops<BdLiRef>(),
+    compiled_ops<BdCH4Sto3g>(), compiled_ops<BdCH4631g>(),  generic_ops<1, 0>(),
+    generic_ops<1, 1>(),        generic_ops<2, 1>(),        generic_ops<2, 2>(),
+    generic_ops<3, 2>(),        generic_ops<3, 3>(),        generic_ops<4, 4>(),
+    generic_ops<5, 5>(),
 };
 
-const EngineOps* find_engine(int nu, int nd) {
-  for (const auto& o : kOps)
-    if (o.nu == nu && o.nd == nd) return &o;
+const EngineOps* find_engine(const DevSys<double>& D, bool generic_only) {
+  for (const auto& o : kOps) {
+    if (o.nu != D.n_up || o.nd != D.n_dn) continue;
+    if (o.match && (generic_only || !o.match(D))) continue;
+    return &o;
+  }
   return nullptr;
 }
 

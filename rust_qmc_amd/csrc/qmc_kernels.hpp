@@ -38,8 +38,13 @@ struct EvalOut {
   double* eloc;    // [W]
 };
 
+template <class T>
+struct DevSys;
+
 struct EngineOps {
   int nu, nd;
+  const char* name;                          // "generic" or the compiled shell structure
+  bool (*match)(const DevSys<double>&);      // null for the generic shapes
   hipError_t (*build64)(const void*, const void*, EnsView<double>, int, int*, hipStream_t);
   hipError_t (*build32)(const void*, const void*, EnsView<float>, int, int*, hipStream_t);
   hipError_t (*sweep64)(const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
@@ -50,7 +55,10 @@ struct EngineOps {
   hipError_t (*eval32)(const void*, const void*, int64_t, const double*, EvalOut, int*, hipStream_t);
 };
 
-const EngineOps* find_engine(int nu, int nd);
+// the register-engine kernels for a system: a compiled shell structure when
+// one matches (unless generic_only), else the generic kernels of its
+// electron-count shape; null when neither exists
+const EngineOps* find_engine(const DevSys<double>& D, bool generic_only);
 hipError_t launch_init64(const void* Sd, EnsView<double> E, uint64_t seed, int64_t w0, hipStream_t st);
 hipError_t launch_init32(const void* Sd, EnsView<float> E, uint64_t seed, int64_t w0, hipStream_t st);
 hipError_t launch_reduce(const double* partial, int nsteps, int nblk, int nq, double* out, hipStream_t st);

@@ -223,3 +223,59 @@ def test_nuclei_and_params_roundtrip(Q, oracle):
     o.set_params(p2)
     pos = _configs(o, 20)
     assert _close(g.evaluate_batch(pos)["eloc"], o.batch_eval(pos)["eloc"], RTOL64)
+
+
+SPEC_NAMES = {"h2": "h2-slater1s", "li_ref": "li-init-li-sto", "li_ref_exact": "li-init-li-sto",
+              "li_seeded": "li-sto3term", "ch4_631g": "ch4-631g", "ch4_sto3g": "ch4-sto3g"}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_compiled_shell_kernels_match_generic(Q, oracle, name, precision):
+    """The kernels specialised to a system's shell structure are selected for
+    the reference systems and agree with the run-time-loop kernels on
+    evaluations and on a short Sherman-Morrison trajectory."""
+    g, o = _pair(Q, oracle, name, precision=precision)
+    assert g.engine_name() == "register/" + SPEC_NAMES[name]
+    pos = _configs(o, 70)
+    spec = g.evaluate_batch(pos)
+    W, steps, seed, step = 65, 5, 11, 0.4
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    tr_spec = ens.sweeps(steps, step, trace=True, refresh_interval=3, steps_per_launch=2)[3]
+    pos_spec = ens.positions()
+    g.set_engine("generic")
+    assert g.engine_name() == "register/generic"
+    gen = g.evaluate_batch(pos)
+    ens2 = Q.Ensemble(g, W, seed)
+    assert np.array_equal(ens2.positions(), p0)
+    tr_gen = ens2.sweeps(steps, step, trace=True, refresh_interval=3, steps_per_launch=2)[3]
+    tol = 1e-12 if precision == "fp64" else 2e-5
+    for k in ("logabs", "eloc"):
+        assert _close(spec[k], gen[k], tol), k
+    assert np.allclose(pos_spec, ens2.positions(), rtol=0, atol=1e-9 if precision == "fp64" else 1e-5)
+    assert _close(tr_spec, tr_gen, 1e-10 if precision == "fp64" else 1e-4)
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_compiled_shell_kernels_all_electron(Q, oracle, name):
+    """All-electron moves (vmc.rs:95-119) with the specialised kernels agree
+    with the run-time-loop kernels: ensemble state after the build and after
+    each step."""
+    g, o = _pair(Q, oracle, name)
+    W, seed, step = 66, 5, 0.3
+    res = []
+    for eng in ("auto", "generic"):
+        g.set_engine(eng)
+        ens = Q.Ensemble(g, W, seed)
+        st0 = ens.state()
+        out = [st0]
+        for _ in range(3):
+            acc = ens.sweeps(1, step, Q.QMC_MOVE_ALL)[2]
+            out.append(ens.state() + (acc,))
+        res.append(out)
+    ref0 = o.batch_eval(Q.Ensemble(g, W, seed).positions())
+    assert _close(res[0][0][0], ref0["logabs"], 1e-10), "state ln|psi| after the build"
+    for a, b in zip(res[0], res[1]):
+        for x, y in zip(a, b):
+            assert _close(x, y, 1e-10)
