@@ -58,7 +58,7 @@ def run_mode(Q, precision, W, steps, warmup, spl, refresh):
     tr = es / W
     tau = Q.autocorrelation_time(tr)
     err = Q.blocking_error(tr, tau)
-    return {
+    out = {
         "wall_s": wall,
         "kernel_ms": kms,
         "launches": launches,
@@ -68,6 +68,37 @@ def run_mode(Q, precision, W, steps, warmup, spl, refresh):
         "acceptance": float(acc.sum() / (W * 10 * steps)),
         "trace": tr,
     }
+    if precision == "mixed":
+        # E_L bias of the mixed engine on identical configurations: the
+        # sampler's cached E_L (FP32 orbitals, Sherman-Morrison-updated FP32
+        # inverse) minus the FP64 E_L of the same positions
+        ref = Q.MethaneGTO(1.5, 2.0, 2.06, "STO-3G", 0, precision="fp64")
+        el_mixed = ens.state()[2]
+        el_fp64 = ref.evaluate_batch(ens.positions())["eloc"]
+        d = el_mixed - el_fp64
+        out["eval_bias"] = float(d.mean())
+        out["eval_bias_sem"] = float(d.std() / math.sqrt(len(d)))
+        out["eval_err_max"] = float(np.abs(d).max())
+    return out
+
+
+def run_sr(Q, precision, W, n_meas, warmup):
+    """SR-grad samples/s: per-walker E_L, O_k (32 Jastrow parameters + 15
+    nuclear coordinates), dE_L/dp_k and the Hellmann-Feynman force of all W
+    walkers, accumulated into the FP64 moment matrix (one measure call = W
+    samples), on CH4/STO-3G with a 6-term e-e and 8-term e-n Pade series."""
+    sys_ = Q.MethaneGTO(1.5, 2.0, 2.06, "STO-3G", 0, precision=precision)
+    sys_.set_jastrow_series(6, 8)
+    ens = Q.Ensemble(sys_, W, 0)
+    ens.sweeps(warmup, 0.3, refresh_interval=64)
+    ens.measure(reset=True)  # first call sizes the buffers
+    t0 = time.perf_counter()
+    for _ in range(n_meas):
+        ens.measure()
+    wall = time.perf_counter() - t0
+    m = ens.moments()
+    return {"samples_per_s": W * n_meas / wall, "params": sys_.num_params(), "nuclear": 3 * sys_.n_nuc,
+            "ms_per_call": wall * 1e3 / n_meas, "energy": m["energy"]}
 
 
 def cpu_baseline(seconds_target):
@@ -105,6 +136,7 @@ def main():
     ap.add_argument("--refresh", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sr-calls", type=int, default=20, help="timed SR-grad measurement calls (W samples each)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 8 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -124,6 +156,7 @@ def main():
 
     W, K = args.walkers, args.steps
     res = {m: run_mode(Q, m, W, K, args.warmup, args.steps_per_launch, args.refresh) for m in ("fp64", "mixed")}
+    sr = {m: run_sr(Q, m, W, args.sr_calls, 16) for m in ("fp64", "mixed")}
     mx, fp = res["mixed"], res["fp64"]
     # roofline of the dominant kernel (the mixed-precision sweep)
     avg_launch_s = mx["kernel_ms"] / 1e3 / mx["launches"]
@@ -168,9 +201,18 @@ def main():
         "mixed": {"energy": mx["energy"], "energy_err": mx["energy_err"], "acceptance": mx["acceptance"],
                   "kernel_ms_per_launch": mx["kernel_ms"] / mx["launches"]},
         "mixed_over_fp64": mx["walker_moves_per_s"] / fp["walker_moves_per_s"],
-        "delta_E_mHa": dE * 1e3,
-        "delta_E_err_mHa": sig * 1e3,
-        "delta_E_estimator": "paired chains: mean over sweeps of (mixed - FP64) ensemble energy, same seed",
+        "delta_E_mHa": mx["eval_bias"] * 1e3,
+        "delta_E_err_mHa": mx["eval_bias_sem"] * 1e3,
+        "delta_E_estimator": "E_L bias on identical configurations: mixed sampler's cached E_L minus FP64 E_L "
+                             "of the same 65536 final positions (mean, standard error)",
+        "delta_E_max_abs_Ha": mx["eval_err_max"],
+        "delta_E_chains_mHa": dE * 1e3,
+        "delta_E_chains_err_mHa": sig * 1e3,
+        "delta_E_chains_estimator": "paired chains, same seed: mean over sweeps of (mixed - FP64) ensemble energy",
+        "sr_grad_samples_per_s": {"mixed": sr["mixed"]["samples_per_s"], "fp64": sr["fp64"]["samples_per_s"],
+                                  "params": sr["mixed"]["params"], "nuclear_coords": sr["mixed"]["nuclear"],
+                                  "ms_per_65536_samples": {"mixed": sr["mixed"]["ms_per_call"],
+                                                           "fp64": sr["fp64"]["ms_per_call"]}},
         "roofline": {
             "bound": "mfma",
             "engine": "vector ALU: the sweep has no GEMM-shaped work; 157.3 TFLOP/s is MI355X's FP32 peak "

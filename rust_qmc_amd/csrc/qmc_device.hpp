@@ -384,59 +384,78 @@ struct PairDerivs {
   }
 };
 
+// Pade series lengths: JsRt reads them from DevSys at run time; JsNone
+// compiles the series out (the reference Jastrow3 has none).
+struct JsRt {
+  template <class T>
+  static __device__ __forceinline__ int kee(const DevSys<T>& S) { return S.k_ee; }
+  template <class T>
+  static __device__ __forceinline__ int ken(const DevSys<T>& S) { return S.k_en; }
+};
+struct JsNone {
+  template <class T>
+  static __host__ __device__ constexpr int kee(const DevSys<T>&) { return 0; }
+  template <class T>
+  static __host__ __device__ constexpr int ken(const DevSys<T>&) { return 0; }
+};
+
 // value only (cst excluded: it cancels in every ratio and is added to ln J separately)
-template <class T>
+template <class T, class Js = JsRt>
 __device__ __forceinline__ T uee_val(const DevSys<T>& S, T r, bool same) {
   T D = T(1) + S.b_ee * r;
   T x = r * M<T>::rcp(D);
   T v = (same ? S.a_para : S.a_anti) * x;
-  if (S.k_ee > 0) {
+  const int K = Js::kee(S);
+  if (K > 0) {
     T s, s1, s2;
-    series(S.k_ee, same ? S.c_para : S.c_anti, x, s, s1, s2);
+    series(K, same ? S.c_para : S.c_anti, x, s, s1, s2);
     v += s;
   }
   return v;
 }
 
-template <class T>
+template <class T, class Js = JsRt>
 __device__ __forceinline__ T uen_val(const DevSys<T>& S, T r, int I) {
   T D = T(1) + S.b_en * r;
   T y = r * M<T>::rcp(D);
   T v = -S.Z[I] * y;
-  if (S.k_en > 0) {
+  const int K = Js::ken(S);
+  if (K > 0) {
     T s, s1, s2;
-    series(S.k_en, S.d_en[S.species[I]], y, s, s1, s2);
+    series(K, S.d_en[S.species[I]], y, s, s1, s2);
     v += s;
   }
   return v;
 }
 
 // u'(r) and u''(r) + 2 u'(r)/r for an e-e pair
-template <class T>
+template <class T, class Js = JsRt>
 __device__ __forceinline__ void uee_d(const DevSys<T>& S, T r, T ir, bool same, T& u1, T& lap) {
   T D = T(1) + S.b_ee * r;
   T iD = M<T>::rcp(D);
   T iD2 = iD * iD;
   T a = same ? S.a_para : S.a_anti;
   T s1 = 0, s2 = 0;
-  if (S.k_ee > 0) {
+  const int K = Js::kee(S);
+  if (K > 0) {
     T s, x = r * iD;
-    series(S.k_ee, same ? S.c_para : S.c_anti, x, s, s1, s2);
+    series(K, same ? S.c_para : S.c_anti, x, s, s1, s2);
   }
   T ap = a + s1;
   u1 = ap * iD2;
   lap = s2 * iD2 * iD2 - T(2) * S.b_ee * ap * iD2 * iD + T(2) * u1 * ir;
 }
 
-template <class T>
+template <class T, class Js = JsRt>
 __device__ __forceinline__ void uen_d(const DevSys<T>& S, T r, T ir, int I, T& u1, T& lap) {
   T D = T(1) + S.b_en * r;
   T iD = M<T>::rcp(D);
   T iD2 = iD * iD;
   T s1 = 0, s2 = 0;
-  if (S.k_en > 0) {
+  const int K = Js::ken(S);
+  if (K > 0) {
     T s, y = r * iD;
-    series(S.k_en, S.d_en[S.species[I]], y, s, s1, s2);
+    series(K, S.d_en[S.species[I]], y, s, s1, s2);
   }
   T ap = -S.Z[I] + s1;
   u1 = ap * iD2;

@@ -39,8 +39,10 @@ __device__ __forceinline__ void rotate(T ra[Cap<NA>::v][3], T ai[Cap<NA>::v][Cap
 
 // Per-electron local-energy terms of the electron in slot 0 of sector A.
 // Returns the contribution to -1/2 sum lap(psi)/psi and adds the electron's
-// share of the potential (1/2 of each e-e pair) to pot.
-template <int NA, int NB, class T, class Bs = RtBasis>
+// share of the potential (1/2 of each e-e pair) to pot.  Coincident
+// particles (r < 1e-10) contribute no Jastrow terms; the selects keep the
+// pair loops free of branches.
+template <int NA, int NB, class T, class Bs = RtBasis, class Js = JsRt>
 __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, const T ra[Cap<NA>::v][3],
                                                const T ai[Cap<NA>::v][Cap<NA>::v],
                                                const T rb[Cap<NB>::v][3], T& gx, T& gy, T& gz, T& lap,
@@ -51,22 +53,21 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
   det_terms<NA, T, NoShell, Bs>(S, spin, x, y, z, ai, gdx, gdy, gdz, ld, NoShell{});
   // Jastrow part
   T gux = 0, guy = 0, guz = 0, lu = 0, pairsq = 0;
-  double pe = 0.0;
+  T pee = 0, pen = 0;
   auto ee = [&](T px, T py, T pz, bool same) {
     T dx = x - px, dy = y - py, dz = z - pz;
     T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
     T ir = M<T>::rcp(r);
     T u1, lp;
-    uee_d(S, r, ir, same, u1, lp);
-    if (r >= T(1e-10)) {
-      T f = u1 * ir;
-      gux += f * dx;
-      guy += f * dy;
-      guz += f * dz;
-      lu += lp;
-      pairsq += u1 * u1;
-    }
-    pe += 0.5 * (double)ir;
+    uee_d<T, Js>(S, r, ir, same, u1, lp);
+    const bool ok = r >= T(1e-10);
+    const T f = ok ? u1 * ir : T(0);
+    gux += f * dx;
+    guy += f * dy;
+    guz += f * dz;
+    lu += ok ? lp : T(0);
+    pairsq += ok ? u1 * u1 : T(0);
+    pee += ir;
   };
 #pragma unroll
   for (int j = 1; j < NA; ++j) ee(ra[j][0], ra[j][1], ra[j][2], true);
@@ -77,17 +78,16 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
     T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
     T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
     T ir = M<T>::rcp(r);
-    pe -= (double)(S.Z[I] * ir);
+    pen += S.Z[I] * ir;
     if (en_on) {
       T u1, lp;
-      uen_d(S, r, ir, I, u1, lp);
-      if (r >= T(1e-10)) {
-        T f = u1 * ir;
-        gux += f * dx;
-        guy += f * dy;
-        guz += f * dz;
-        lu += lp;
-      }
+      uen_d<T, Js>(S, r, ir, I, u1, lp);
+      const bool ok = r >= T(1e-10);
+      const T f = ok ? u1 * ir : T(0);
+      gux += f * dx;
+      guy += f * dy;
+      guz += f * dz;
+      lu += ok ? lp : T(0);
     }
   });
   // Jastrow2's gradient carries an extra factor F (jastrow.rs:102) and its
@@ -99,14 +99,14 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
   gy = gdy + gjy;
   gz = gdz + gjz;
   lap = ld + lj + T(2) * (gdx * gjx + gdy * gjy + gdz * gjz);
-  pot += pe;
+  pot += 0.5 * (double)pee - (double)pen;
 }
 
 // Local energy E_L = -1/2 sum_i lap_i psi / psi + V, V = -sum Z/r + sum 1/r_ij + V_nn
 // (methane.rs:1154-1182).  Sectors are left in their canonical order.
 // When gout/lout are given, per-electron grad ln psi and lap psi / psi are
 // written to gout[(e*3+c)*ld], lout[e*ld] for global electron index e.
-template <int NU, int ND, class T, class Bs = RtBasis>
+template <int NU, int ND, class T, class Bs = RtBasis, class Js = JsRt>
 __device__ __forceinline__ double local_energy(const DevSys<T>& S, T ru[Cap<NU>::v][3],
                                                T au[Cap<NU>::v][Cap<NU>::v], T rd[Cap<ND>::v][3],
                                                T ad[Cap<ND>::v][Cap<ND>::v], double* gout, double* lout,
@@ -115,7 +115,7 @@ __device__ __forceinline__ double local_energy(const DevSys<T>& S, T ru[Cap<NU>:
 #pragma unroll 1
   for (int k = 0; k < NU; ++k) {
     T gx, gy, gz, lap;
-    electron_terms<NU, ND, T, Bs>(S, 0, ru, au, rd, gx, gy, gz, lap, pot);
+    electron_terms<NU, ND, T, Bs, Js>(S, 0, ru, au, rd, gx, gy, gz, lap, pot);
     kin += -0.5 * (double)lap;
     if (gout) {
       const int e = S.sector_elec[0][k];
@@ -129,7 +129,7 @@ __device__ __forceinline__ double local_energy(const DevSys<T>& S, T ru[Cap<NU>:
 #pragma unroll 1
   for (int k = 0; k < ND; ++k) {
     T gx, gy, gz, lap;
-    electron_terms<ND, NU, T, Bs>(S, 1, rd, ad, ru, gx, gy, gz, lap, pot);
+    electron_terms<ND, NU, T, Bs, Js>(S, 1, rd, ad, ru, gx, gy, gz, lap, pot);
     kin += -0.5 * (double)lap;
     if (gout) {
       const int e = S.sector_elec[1][k];
@@ -144,7 +144,7 @@ __device__ __forceinline__ double local_energy(const DevSys<T>& S, T ru[Cap<NU>:
 }
 
 // Full Jastrow exponent ln J (FP64), including the pair constant.
-template <int NU, int ND, class T, class Bs = RtBasis>
+template <int NU, int ND, class T, class Bs = RtBasis, class Js = JsRt>
 __device__ __forceinline__ double jastrow_u(const DevSys<double>& S, const T ru[Cap<NU>::v][3],
                                             const T rd[Cap<ND>::v][3]) {
   constexpr int N = NU + ND;
@@ -164,14 +164,14 @@ __device__ __forceinline__ double jastrow_u(const DevSys<double>& S, const T ru[
     for (int j = i + 1; j < N; ++j) {
       double dx = r[i][0] - r[j][0], dy = r[i][1] - r[j][1], dz = r[i][2] - r[j][2];
       double rr = sqrt(dx * dx + dy * dy + dz * dz);
-      u += uee_val(S, rr, (i < NU) == (j < NU)) + S.cst_ee;
+      u += uee_val<double, Js>(S, rr, (i < NU) == (j < NU)) + S.cst_ee;
     }
   if (S.en_on)
 #pragma unroll
     for (int i = 0; i < N; ++i)
       Bs::nuclei(S, [&](int I) {
         double dx = r[i][0] - S.nuc[I][0], dy = r[i][1] - S.nuc[I][1], dz = r[i][2] - S.nuc[I][2];
-        u += uen_val(S, sqrt(dx * dx + dy * dy + dz * dz), I);
+        u += uen_val<double, Js>(S, sqrt(dx * dx + dy * dy + dz * dz), I);
       });
   return u;
 }
@@ -205,7 +205,7 @@ __device__ __forceinline__ bool sector_inverse(const DevSys<double>& S, int spin
 // One sector's single-electron Metropolis moves (Gaussian proposal, ratio
 // from the inverse column, Jastrow difference from the moved electron's
 // pairs, Sherman-Morrison row update on acceptance).
-template <int NA, int NB, class T, class Bs = RtBasis>
+template <int NA, int NB, class T, class Bs = RtBasis, class Js = JsRt>
 __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[Cap<NA>::v][3],
                                              T ai[Cap<NA>::v][Cap<NA>::v], const T rb[Cap<NB>::v][3],
                                              uint32_t w, uint32_t step, uint32_t k0, uint32_t k1, T sigma,
@@ -228,8 +228,8 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
     auto pair = [&](T px, T py, T pz, bool same) {
       T ax = nx - px, ay = ny - py, az = nz - pz;
       T bx = ox - px, by = oy - py, bz = oz - pz;
-      du += uee_val(S, M<T>::sq(ax * ax + ay * ay + az * az), same) -
-            uee_val(S, M<T>::sq(bx * bx + by * by + bz * bz), same);
+      du += uee_val<T, Js>(S, M<T>::sq(ax * ax + ay * ay + az * az), same) -
+            uee_val<T, Js>(S, M<T>::sq(bx * bx + by * by + bz * bz), same);
     };
 #pragma unroll
     for (int j = 1; j < NA; ++j) pair(ra[j][0], ra[j][1], ra[j][2], true);
@@ -239,8 +239,8 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
       Bs::nuclei(S, [&](int I) {
         T ax = nx - S.nuc[I][0], ay = ny - S.nuc[I][1], az = nz - S.nuc[I][2];
         T bx = ox - S.nuc[I][0], by = oy - S.nuc[I][1], bz = oz - S.nuc[I][2];
-        du += uen_val(S, M<T>::sq(ax * ax + ay * ay + az * az), I) -
-              uen_val(S, M<T>::sq(bx * bx + by * by + bz * bz), I);
+        du += uen_val<T, Js>(S, M<T>::sq(ax * ax + ay * ay + az * az), I) -
+              uen_val<T, Js>(S, M<T>::sq(bx * bx + by * by + bz * bz), I);
       });
     const T p = R * R * M<T>::ex(T(2) * du);
     if (U < p) {

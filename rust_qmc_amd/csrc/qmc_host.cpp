@@ -1306,10 +1306,13 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
       else
         HIP_TRY(wave_sweep<float>(s->d32, e->view<float>(), s->shape(), A, e->partial, trace, e->st));
     } else if (p->move == QMC_MOVE_SINGLE) {
+      const bool no_series = s->h.k_ee == 0 && s->h.k_en == 0;
       if (e->prec == QMC_FP64)
-        HIP_TRY(s->ops->sweep64(s->d64, e->view<double>(), A, e->partial, trace, e->st));
+        HIP_TRY((no_series ? s->ops->sweep64_ns : s->ops->sweep64)(s->d64, e->view<double>(), A, e->partial, trace,
+                                                                   e->st));
       else
-        HIP_TRY(s->ops->sweep32(s->d32, e->view<float>(), A, e->partial, trace, e->st));
+        HIP_TRY((no_series ? s->ops->sweep32_ns : s->ops->sweep32)(s->d32, e->view<float>(), A, e->partial, trace,
+                                                                   e->st));
     } else {
       if (e->prec == QMC_FP64)
         HIP_TRY(s->ops->ae64(s->d64, s->d64, e->view<double>(), A, e->partial, trace, e->st));

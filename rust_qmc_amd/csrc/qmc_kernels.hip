@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(64) build_state_kernel(const DevSys<double>* _
 // persistent single-electron sweeps: n_steps sweeps, E_L after each, and
 // per-(step, block) partial sums {sum E_L, sum E_L^2, accepted moves}
 // ---------------------------------------------------------------------------
-template <int NU, int ND, class Bs, class T>
+template <int NU, int ND, class Bs, class Js, class T>
 __global__ void __launch_bounds__(64) sweep_kernel(const DevSys<T>* __restrict__ Sp, EnsView<T> E,
                                                    SweepArgs A, double* __restrict__ partial,
                                                    double* __restrict__ el_trace) {
@@ -133,9 +133,9 @@ __global__ void __launch_bounds__(64) sweep_kernel(const DevSys<T>* __restrict__
   for (int s = 0; s < A.n_steps; ++s) {
     const uint32_t step = A.step0 + (uint32_t)s;
     int acc = 0;
-    sector_moves<NU, ND, T, Bs>(S, 0, R.ru, R.au, R.rd, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
-    sector_moves<ND, NU, T, Bs>(S, 1, R.rd, R.ad, R.ru, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
-    el = local_energy<NU, ND, T, Bs>(S, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
+    sector_moves<NU, ND, T, Bs, Js>(S, 0, R.ru, R.au, R.rd, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
+    sector_moves<ND, NU, T, Bs, Js>(S, 1, R.rd, R.ad, R.ru, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
+    el = local_energy<NU, ND, T, Bs, Js>(S, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
     if (el_trace && live) el_trace[(int64_t)s * A.trace_ld + w] = el;
     double e1 = live ? el : 0.0, e2 = live ? el * el : 0.0, a = live ? (double)acc : 0.0;
     e1 = wave_sum(e1);
@@ -336,9 +336,10 @@ struct Launch {
                                                                   (const DevSys<T>*)St, E, want_eloc, sing);
     return hipGetLastError();
   }
+  template <class Js>
   static hipError_t sweep(const void* St, EnsView<T> E, SweepArgs A, double* partial, double* trace,
                           hipStream_t st) {
-    sweep_kernel<NU, ND, Bs, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<T>*)St, E, A, partial, trace);
+    sweep_kernel<NU, ND, Bs, Js, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<T>*)St, E, A, partial, trace);
     return hipGetLastError();
   }
   static hipError_t all_electron(const void* Sd, const void* St, EnsView<T> E, SweepArgs A, double* partial,
@@ -355,7 +356,7 @@ struct Launch {
   }
 };
 
-template <int NU, int ND, class Bs>
+template <int NU, int ND, class Bs, class JsFast>
 static EngineOps make_ops(const char* name, bool (*match)(const DevSys<double>&)) {
   EngineOps o;
   o.nu = NU;
@@ -364,8 +365,10 @@ static EngineOps make_ops(const char* name, bool (*match)(const DevSys<double>&)
   o.match = match;
   o.build64 = &Launch<NU, ND, Bs, double>::build;
   o.build32 = &Launch<NU, ND, Bs, float>::build;
-  o.sweep64 = &Launch<NU, ND, Bs, double>::sweep;
-  o.sweep32 = &Launch<NU, ND, Bs, float>::sweep;
+  o.sweep64 = &Launch<NU, ND, Bs, double>::template sweep<JsRt>;
+  o.sweep32 = &Launch<NU, ND, Bs, float>::template sweep<JsRt>;
+  o.sweep64_ns = &Launch<NU, ND, Bs, double>::template sweep<JsFast>;
+  o.sweep32_ns = &Launch<NU, ND, Bs, float>::template sweep<JsFast>;
   // All-electron moves always run the run-time-loop kernels: the CH4/STO-3G
   // all-electron kernel specialised to its shell structure disagreed with
   // them (and with the oracle) from the first step while every other
@@ -379,12 +382,12 @@ static EngineOps make_ops(const char* name, bool (*match)(const DevSys<double>&)
 
 template <int NU, int ND>
 static EngineOps generic_ops() {
-  return make_ops<NU, ND, RtBasis>("generic", nullptr);
+  return make_ops<NU, ND, RtBasis, JsRt>("generic", nullptr);
 }
 
 template <class Bd>
 static EngineOps compiled_ops() {
-  return make_ops<Bd::nu, Bd::nd, CtBasis<Bd>>(Bd::name, &basis_matches<Bd, double>);
+  return make_ops<Bd::nu, Bd::nd, CtBasis<Bd>, JsNone>(Bd::name, &basis_matches<Bd, double>);
 }
 
 // Shell structures compiled into the register engine (tried first), then

@@ -49,6 +49,9 @@ struct EngineOps {
   hipError_t (*build32)(const void*, const void*, EnsView<float>, int, int*, hipStream_t);
   hipError_t (*sweep64)(const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*sweep32)(const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
+  // the same sweeps compiled for a Jastrow without Pade series terms
+  hipError_t (*sweep64_ns)(const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
+  hipError_t (*sweep32_ns)(const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*ae64)(const void*, const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*ae32)(const void*, const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*eval64)(const void*, const void*, int64_t, const double*, EvalOut, int*, hipStream_t);
