@@ -142,18 +142,24 @@ int qmc_benzene_gto_create(double b_ee, double b_en, double cc_bond, double ch_b
 int qmc_set_precision(qmc_system* sys, int32_t precision);
 /* Engine selection.  The register engine keeps one walker per lane in
  * VGPRs (compiled electron counts up to 5 + 5); the wave engine keeps one
- * walker per wavefront in LDS (any count up to 32 per spin).  AUTO picks
- * the register engine when it exists for the system.  The register engine
- * uses kernels specialised to the system's shell structure when one is
- * compiled (H2, both Li atoms, CH4 STO-3G and 6-31G); GENERIC forces its
- * kernels with run-time shell loops. */
+ * walker per wavefront in LDS (any count up to 32 per spin); the tile
+ * engine moves 16 walkers per workgroup together with an MFMA MO
+ * contraction (mixed-precision sweeps of 21 + 21-electron systems such as
+ * benzene; state build, refresh and evaluation stay on the wave engine).
+ * AUTO picks the register engine when it exists for the system, else the
+ * tile engine for mixed-precision sweeps when it fits, else the wave
+ * engine.  The register engine uses kernels specialised to the system's
+ * shell structure when one is compiled (H2, both Li atoms, CH4 STO-3G and
+ * 6-31G); GENERIC forces its kernels with run-time shell loops. */
 #define QMC_ENGINE_AUTO 0
 #define QMC_ENGINE_REGISTER 1
 #define QMC_ENGINE_WAVE 2
 #define QMC_ENGINE_GENERIC 3
+#define QMC_ENGINE_TILE 4
 int qmc_set_engine(qmc_system* sys, int32_t engine);
-/* Name of the kernels a system runs on: "wave", "register/generic" or
- * "register/<shell structure>" (no reference counterpart; diagnostics). */
+/* Name of the kernels a system's sweeps run on: "tile", "wave",
+ * "register/generic" or "register/<shell structure>" (no reference
+ * counterpart; diagnostics). */
 int qmc_engine_name(const qmc_system* sys, char* buf, int32_t len);
 int qmc_set_flags(qmc_system* sys, uint32_t flags);
 int qmc_system_info(const qmc_system* sys, int32_t* n_elec, int32_t* n_up, int32_t* n_nuc, int32_t* n_ao);

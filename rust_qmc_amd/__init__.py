@@ -114,8 +114,9 @@ class System:
 
     def set_engine(self, engine):
         """'auto', 'register' (one walker per lane), 'wave' (one walker per
-        wavefront) or 'generic' (register engine with run-time shell loops)."""
-        check(lib().qmc_set_engine(self._h, {"auto": 0, "register": 1, "wave": 2, "generic": 3}[engine]))
+        wavefront), 'generic' (register engine with run-time shell loops) or
+        'tile' (16 walkers per workgroup, MFMA MO contraction)."""
+        check(lib().qmc_set_engine(self._h, {"auto": 0, "register": 1, "wave": 2, "generic": 3, "tile": 4}[engine]))
 
     def engine_name(self):
         """The kernels this system runs on, e.g. 'register/ch4-sto3g'."""

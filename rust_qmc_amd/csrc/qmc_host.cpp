@@ -18,6 +18,7 @@
 #include "qmc_derivs.hpp"
 #include "qmc_device.hpp"
 #include "qmc_kernels.hpp"
+#include "qmc_tile.hpp"
 #include "qmc_wave.hpp"
 
 using namespace qmc;
@@ -105,9 +106,16 @@ struct qmc_system {
   void pick_ops() { ops = (engine == QMC_ENGINE_GENERIC || !ops_spec) ? ops_gen : ops_spec; }
   bool dirty = true;
   uint64_t version = 0;  // bumped on every upload; ensembles refresh when it moves
-  // the wave engine runs when selected, or when no register engine was
-  // compiled for this electron count
-  bool wave() const { return engine == QMC_ENGINE_WAVE || (engine == QMC_ENGINE_AUTO && !ops); }
+  TilePlan tile;  // tile-engine plan (benzene-size sectors, mixed precision)
+  // the wave engine (and, for mixed-precision sweeps, the tile engine) runs
+  // when selected, or when no register engine was compiled for this
+  // electron count
+  bool wave() const {
+    return engine == QMC_ENGINE_WAVE || engine == QMC_ENGINE_TILE || (engine == QMC_ENGINE_AUTO && !ops);
+  }
+  bool tile_sweeps(int32_t prec) const {
+    return wave() && tile.ok && prec == QMC_MIXED && engine != QMC_ENGINE_WAVE;
+  }
   WaveShape shape() const { return WaveShape{h.n_elec(), h.n_up(), h.n_dn(), h.n_ao()}; }
   ~qmc_system() {
     if (d64) (void)hipFree(d64);
@@ -294,6 +302,7 @@ int upload(qmc_system* s) {
   if (s->ops_spec && !s->ops_spec->match) s->ops_spec = nullptr;
   s->ops_gen = find_engine(*d64, true);
   s->pick_ops();
+  tile_plan(*d32, s->tile);
   s->dops = find_derivs(s->h.n_up(), s->h.n_dn());
   s->dirty = false;
   s->version++;
@@ -842,7 +851,8 @@ int qmc_set_precision(qmc_system* s, int32_t precision) {
 }
 
 int qmc_set_engine(qmc_system* s, int32_t engine) {
-  if (!s || engine < QMC_ENGINE_AUTO || engine > QMC_ENGINE_GENERIC) return fail(QMC_ERR_INVALID, "bad engine");
+  if (!s || engine < QMC_ENGINE_AUTO || engine > QMC_ENGINE_TILE) return fail(QMC_ERR_INVALID, "bad engine");
+  if (engine == QMC_ENGINE_TILE && !s->tile.ok) return fail(QMC_ERR_UNSUPPORTED, "no tile kernel for this system");
   if ((engine == QMC_ENGINE_REGISTER || engine == QMC_ENGINE_GENERIC) && !s->ops_gen)
     return fail(QMC_ERR_UNSUPPORTED, "no register engine for this electron count");
   s->engine = engine;
@@ -853,7 +863,9 @@ int qmc_set_engine(qmc_system* s, int32_t engine) {
 
 int qmc_engine_name(const qmc_system* s, char* buf, int32_t len) {
   if (!s || !buf || len < 1) return fail(QMC_ERR_INVALID, "bad arguments");
-  std::string n = s->wave() ? std::string("wave") : std::string("register/") + (s->ops ? s->ops->name : "none");
+  std::string n = s->tile_sweeps(s->precision) ? std::string("tile")
+                  : s->wave()                    ? std::string("wave")
+                                                 : std::string("register/") + (s->ops ? s->ops->name : "none");
   std::snprintf(buf, (size_t)len, "%s", n.c_str());
   return QMC_OK;
 }
@@ -1274,7 +1286,8 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
   const bool wave = s->wave();
   if (wave && p->move != QMC_MOVE_SINGLE)
     return fail(QMC_ERR_UNSUPPORTED, "the wave engine runs single-electron moves only");
-  const int nblk = wave ? (int)e->W : (int)((e->W + 63) / 64);
+  const bool tile = p->move == QMC_MOVE_SINGLE && s->tile_sweeps(e->prec);
+  const int nblk = tile ? (int)tile_blocks(e->W) : wave ? (int)e->W : (int)((e->W + 63) / 64);
   const int refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
   int chunk = p->steps_per_launch > 0 ? p->steps_per_launch : 16;
   if ((rc = ensure_cap(&e->partial, &e->partial_cap, (size_t)chunk * nblk * 3))) return rc;
@@ -1300,7 +1313,9 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
       }
       HIP_TRY(hipEventRecord(e->ev[2 * nl], e->st));
     }
-    if (wave) {
+    if (tile) {
+      HIP_TRY(tile_sweep(s->d32, e->view<float>(), s->tile, A, e->partial, trace, e->st));
+    } else if (wave) {
       if (e->prec == QMC_FP64)
         HIP_TRY(wave_sweep<double>(s->d64, e->view<double>(), s->shape(), A, e->partial, trace, e->st));
       else

@@ -1,0 +1,37 @@
+// qmc_tile.hpp -- launchers of the tile engine (a 256-thread workgroup owns
+// a tile of 16 walkers; MFMA MO contraction), compiled for benzene-size
+// spin sectors in mixed precision.
+#pragma once
+
+#include "qmc_kernels.hpp"
+
+namespace qmc {
+
+constexpr int TILE_W = 16;   // walkers per tile (one workgroup)
+constexpr int TILE_MAXSH = 4;  // shells per lane in the AO phase
+
+template <class T>
+struct DevSys;
+
+// what a tile launch needs to know about the system, prepared on the host
+struct TilePlan {
+  bool ok = false;  // a tile kernel is compiled for this system
+  int N = 0, nu = 0, nd = 0, A = 0, n_nuc = 0, n_shell = 0;
+  bool no_series = true;
+  int lane_sh[16][TILE_MAXSH];  // shells of group lane g (-1: none)
+};
+
+// fills P; false when no tile kernel fits the system (21 + 21 electrons,
+// at most 68 AOs, electrons + nuclei <= 64, at most 64 shells of at most
+// 6 primitives)
+bool tile_plan(const DevSys<float>& D, TilePlan& P);
+
+// single-electron sweeps of a mixed-precision ensemble (FP32 state); the
+// FP64 inverse refresh and the state build are the wave engine's, which
+// keeps the same HBM layout
+hipError_t tile_sweep(const void* St, EnsView<float> E, const TilePlan& P, SweepArgs A, double* partial,
+                      double* trace, hipStream_t st);
+
+inline int64_t tile_blocks(int64_t W) { return (W + TILE_W - 1) / TILE_W; }
+
+}  // namespace qmc

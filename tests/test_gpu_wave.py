@@ -100,3 +100,44 @@ def test_wave_engine_mixed_ensemble(Q, oracle):
         res[prec] = (tr.mean(), max(Q.blocking_error(tr, tau), tr.std() / np.sqrt(len(tr))))
     (a, sa), (b, sb) = res["fp64"], res["mixed"]
     assert abs(a - b) < 3 * np.hypot(sa, sb) + 1e-3
+
+
+def test_tile_engine_selected_for_mixed_benzene(Q):
+    assert Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="mixed").engine_name() == "tile"
+    assert Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="fp64").engine_name() == "wave"
+
+
+def test_tile_engine_trajectory_vs_oracle(Q, oracle):
+    """Tile-engine sweeps (FP32 MFMA MO contraction, register-resident
+    inverse columns) follow the oracle's FP64 brute-force chain: same
+    accept/reject decisions, positions to FP32 rounding.  3 walkers leave 13
+    dead walkers in the tile."""
+    g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="mixed")
+    o = oracle.System.benzene(1.5, 2.0, 2.64, 2.05)
+    W, steps, seed, step = 3, 2, 7, 0.2
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    es, e2, acc, tr = ens.sweeps(steps, step, trace=True, refresh_interval=0)
+    pos_o, lp_o, el_o, acc_o = o.single_electron_steps(p0, seed, 0, steps, step)
+    assert acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=2e-4)
+    err = np.abs(tr - el_o)
+    assert np.all(err <= 2e-4 * np.abs(el_o) + 1e-2), err.max()
+
+
+def test_tile_engine_matches_wave_engine(Q):
+    """Mixed-precision sweeps of 37 walkers (three tiles, the last ragged) on
+    the tile and on the wave engine from the same seed: the same decisions
+    and, to FP32 rounding, the same positions and energies."""
+    out = []
+    for eng in ("tile", "wave"):
+        g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="mixed")
+        g.set_engine(eng)
+        ens = Q.Ensemble(g, 37, 9)
+        es, e2, acc, tr = ens.sweeps(3, 0.2, trace=True, refresh_interval=2, steps_per_launch=2)
+        out.append((ens.positions(), acc, tr, ens.state()))
+    (p1, a1, t1, s1), (p2, a2, t2, s2) = out
+    assert np.array_equal(a1, a2)
+    assert np.allclose(p1, p2, rtol=0, atol=2e-4)
+    assert _close(t1, t2, 2e-4, scale=np.abs(t2) + 10.0)
+    assert _close(s1[0], s2[0], 1e-6, scale=np.abs(s2[0]))
