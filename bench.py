@@ -101,6 +101,37 @@ def run_sr(Q, precision, W, n_meas, warmup):
             "ms_per_call": wall * 1e3 / n_meas, "energy": m["energy"]}
 
 
+def run_batch(Q, sweeps, trial):
+    """Heterogeneous batch: benzene (8192 walkers) + CH4 STO-3G (32768) + H2
+    (65536), mixed precision, sampled together; the autotuner picks the
+    launch configuration, then the timed sweeps run with it.  The fused
+    (one persistent launch) and sequential (one launch per ensemble)
+    configurations are timed beside it."""
+    systems = [Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="mixed"),
+               Q.MethaneGTO(1.5, 2.0, 2.06, "STO-3G", 0, precision="mixed"),
+               Q.H2MoleculeMO(precision="mixed")]
+    walkers = [8192, 32768, 65536]
+    steps = [0.2, 0.3, 0.5]
+    ens = [Q.Ensemble(s, W, i) for i, (s, W) in enumerate(zip(systems, walkers))]
+    b = Q.Batch(ens)
+    b.sweeps(8, steps, refresh_interval=64)
+    cfg, tuned_ms = b.tune(trial, steps, refresh_interval=64)
+    out = {"tuned_config": cfg, "engines": [s.engine_name() for s in systems], "walkers": walkers}
+    t0 = time.perf_counter()
+    es, e2, acc = b.sweeps(sweeps, steps, refresh_interval=64)
+    wall = time.perf_counter() - t0
+    kms, nl = b.timing()
+    tot = sum(walkers) * sweeps
+    out.update({"walker_moves_per_s": tot / wall, "kernel_walker_moves_per_s": tot / (kms / 1e3),
+                "ms_per_sweep": kms / sweeps, "launches": nl,
+                "energies": [float(es[i].mean() / walkers[i]) for i in range(3)]})
+    for mode in ("fused", "sequential"):
+        b.set_config(mode=mode, blocks_per_cu=0, steps_per_launch=cfg["steps_per_launch"], order="cost")
+        b.sweeps(sweeps // 2, steps, refresh_interval=64)
+        out[f"ms_per_sweep_{mode}"] = b.timing()[0] / (sweeps // 2)
+    return out
+
+
 def cpu_baseline(seconds_target):
     """The reference algorithm (vmc.rs:95-119: all-electron proposal, full
     recomputation of psi, E_L with 8 sector LU per accepted move) on the CH4
@@ -137,6 +168,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sr-calls", type=int, default=20, help="timed SR-grad measurement calls (W samples each)")
+    ap.add_argument("--batch-sweeps", type=int, default=32,
+                    help="timed sweeps of the benzene + CH4 + H2 batch (0: skip it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 8 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -157,6 +190,7 @@ def main():
     W, K = args.walkers, args.steps
     res = {m: run_mode(Q, m, W, K, args.warmup, args.steps_per_launch, args.refresh) for m in ("fp64", "mixed")}
     sr = {m: run_sr(Q, m, W, args.sr_calls, 16) for m in ("fp64", "mixed")}
+    batch = run_batch(Q, args.batch_sweeps, 4) if args.batch_sweeps > 0 else None
     mx, fp = res["mixed"], res["fp64"]
     # roofline of the dominant kernel (the mixed-precision sweep)
     avg_launch_s = mx["kernel_ms"] / 1e3 / mx["launches"]
@@ -213,6 +247,7 @@ def main():
                                   "params": sr["mixed"]["params"], "nuclear_coords": sr["mixed"]["nuclear"],
                                   "ms_per_65536_samples": {"mixed": sr["mixed"]["ms_per_call"],
                                                            "fp64": sr["fp64"]["ms_per_call"]}},
+        "heterogeneous_batch": batch,
         "roofline": {
             "bound": "mfma",
             "engine": "vector ALU: the sweep has no GEMM-shaped work; 157.3 TFLOP/s is MI355X's FP32 peak "

@@ -312,6 +312,50 @@ typedef struct {
 int qmc_sample_forces(const qmc_system* sys, const qmc_force_params* p, uint64_t seed, double* forces,
                       double* mean_energy, double* error, double* force_variance);
 
+/* ---- heterogeneous batches: several ensembles in one persistent launch ----
+ * No reference counterpart (the reference runs one system at a time); the
+ * batch serves ensembles of different molecules sampled together.  In
+ * FUSED mode one kernel launch runs every ensemble: workgroups take tiles
+ * of walkers from a queue (most expensive first) until it is empty.  The
+ * fused kernel is compiled for mixed-precision CH4/STO-3G and H2 register
+ * ensembles and benzene-size tile ensembles, without Pade series; other
+ * ensembles run in STREAMS (one launch per ensemble on concurrent streams)
+ * or SEQUENTIAL mode.  Results per ensemble are those of
+ * qmc_ensemble_sweeps with the same arguments. */
+#define QMC_BATCH_FUSED 0
+#define QMC_BATCH_STREAMS 1
+#define QMC_BATCH_SEQUENTIAL 2
+#define QMC_BATCH_ORDER_COST 0  /* queue holds the most expensive tiles first */
+#define QMC_BATCH_ORDER_GIVEN 1 /* queue follows the ensemble order */
+typedef struct {
+  int32_t mode;             /* QMC_BATCH_* */
+  int32_t blocks_per_cu;    /* FUSED: persistent workgroups per CU (0: as many as fit) */
+  int32_t steps_per_launch; /* sweeps per launch */
+  int32_t order;            /* QMC_BATCH_ORDER_* */
+} qmc_batch_config;
+typedef struct qmc_batch qmc_batch;
+/* ens: n distinct ensembles (kept by the caller, n <= 8) */
+int qmc_batch_create(qmc_ensemble* const* ens, int32_t n, qmc_batch** out);
+int qmc_batch_destroy(qmc_batch* b);
+/* 1 when every ensemble can run in the fused kernel */
+int qmc_batch_fusable(const qmc_batch* b, int32_t* fusable);
+int qmc_batch_set_config(qmc_batch* b, const qmc_batch_config* cfg);
+int qmc_batch_get_config(const qmc_batch* b, qmc_batch_config* cfg);
+/* n_steps single-electron sweeps of every ensemble; step_size[n]; outputs
+ * (host, may be NULL) [n][n_steps] per-sweep ensemble sums as
+ * qmc_ensemble_sweeps */
+int qmc_batch_sweeps(qmc_batch* b, int32_t n_steps, const double* step_size, int32_t refresh_interval, double* e_sum,
+                     double* e2_sum, double* accepted);
+/* kernel milliseconds (first launch start to last launch end, summed over
+ * chunks) and launch count of the last qmc_batch_sweeps call */
+int qmc_batch_timing(const qmc_batch* b, double* kernel_ms, int64_t* launches);
+/* Autotuner: times every candidate configuration (modes, workgroups per
+ * CU, sweeps per launch, queue order) on trial_steps sweeps of the
+ * ensembles themselves (which advance), keeps the fastest and returns it
+ * with its kernel milliseconds per sweep. */
+int qmc_batch_tune(qmc_batch* b, int32_t trial_steps, const double* step_size, int32_t refresh_interval,
+                   qmc_batch_config* best, double* best_ms_per_sweep);
+
 /* MCMCParams (vmc.rs:12-21) plus the options the reference hard-wires */
 typedef struct {
   int64_t n_walkers;

@@ -32,7 +32,7 @@ __all__ = [
     "QmcError", "System", "H2MoleculeMO", "Lithium", "STOSlaterDet", "MethaneGTO", "BenzeneGTO", "Ensemble",
     "DeviceBuffer", "check", "lib",
     "MCMCParams", "MCMCResults", "MCMCSimulation", "autocorrelation_time", "blocking_error", "QMC_FP64",
-    "QMC_MIXED", "QMC_MOVE_SINGLE", "QMC_MOVE_ALL", "QMC_FLAGS_REFERENCE",
+    "QMC_MIXED", "QMC_MOVE_SINGLE", "QMC_MOVE_ALL", "QMC_FLAGS_REFERENCE", "Batch",
 ]
 
 _PREC = {"fp64": QMC_FP64, "mixed": QMC_MIXED, QMC_FP64: QMC_FP64, QMC_MIXED: QMC_MIXED}
@@ -554,6 +554,66 @@ class GeometryOptimizer:
         me, err = C.c_double(), C.c_double()
         check(lib().qmc_sample_forces(wfn._h, C.byref(p), self.seed, _ptr(F), C.byref(me), C.byref(err), _ptr(fv)))
         return F, me.value, err.value, fv
+
+
+_BATCH_MODES = {"fused": 0, "streams": 1, "sequential": 2}
+_BATCH_ORDERS = {"cost": 0, "given": 1}
+
+
+class Batch:
+    """Several ensembles sampled together (qmc_batch): in 'fused' mode one
+    persistent launch runs them all from a queue of walker tiles."""
+
+    def __init__(self, ensembles):
+        self.ensembles = list(ensembles)
+        arr = (C.c_void_p * len(self.ensembles))(*[e._h for e in self.ensembles])
+        self._h = C.c_void_p()
+        check(lib().qmc_batch_create(arr, len(self.ensembles), C.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.qmc_batch_destroy(h)
+            self._h = None
+
+    @property
+    def fusable(self):
+        f = C.c_int32()
+        check(lib().qmc_batch_fusable(self._h, C.byref(f)))
+        return bool(f.value)
+
+    def config(self):
+        c = _lib.QmcBatchConfig()
+        check(lib().qmc_batch_get_config(self._h, C.byref(c)))
+        inv_m = {v: k for k, v in _BATCH_MODES.items()}
+        inv_o = {v: k for k, v in _BATCH_ORDERS.items()}
+        return {"mode": inv_m[c.mode], "blocks_per_cu": c.blocks_per_cu, "steps_per_launch": c.steps_per_launch,
+                "order": inv_o[c.order]}
+
+    def set_config(self, mode="fused", blocks_per_cu=0, steps_per_launch=8, order="cost"):
+        c = _lib.QmcBatchConfig(_BATCH_MODES[mode], blocks_per_cu, steps_per_launch, _BATCH_ORDERS[order])
+        check(lib().qmc_batch_set_config(self._h, C.byref(c)))
+
+    def sweeps(self, n_steps, step_sizes, refresh_interval=64):
+        """Per ensemble and sweep: sums of E_L, E_L^2 and accepted moves, each [n][n_steps]."""
+        n = len(self.ensembles)
+        st = _f64(step_sizes).reshape(n)
+        es, e2, acc = (np.zeros((n, n_steps)) for _ in range(3))
+        check(lib().qmc_batch_sweeps(self._h, n_steps, _ptr(st), refresh_interval, _ptr(es), _ptr(e2), _ptr(acc)))
+        return es, e2, acc
+
+    def timing(self):
+        ms, nl = C.c_double(), C.c_int64()
+        check(lib().qmc_batch_timing(self._h, C.byref(ms), C.byref(nl)))
+        return ms.value, nl.value
+
+    def tune(self, trial_steps, step_sizes, refresh_interval=64):
+        """Autotuner: keeps and returns the fastest configuration and its ms per sweep."""
+        st = _f64(step_sizes).reshape(len(self.ensembles))
+        c = _lib.QmcBatchConfig()
+        ms = C.c_double()
+        check(lib().qmc_batch_tune(self._h, trial_steps, _ptr(st), refresh_interval, C.byref(c), C.byref(ms)))
+        return self.config(), ms.value
 
 
 def autocorrelation_time(e):

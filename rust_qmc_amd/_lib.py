@@ -31,6 +31,8 @@ QMC_AO_GTO, QMC_AO_STO, QMC_AO_SLATER1S = 0, 1, 2
 QMC_JAS_NONE, QMC_JAS_PAIR, QMC_JAS_PADE = 0, 1, 2
 QMC_BASIS_631G, QMC_BASIS_STO3G = 0, 1
 QMC_MOVE_SINGLE, QMC_MOVE_ALL = 0, 1
+QMC_BATCH_FUSED, QMC_BATCH_STREAMS, QMC_BATCH_SEQUENTIAL = 0, 1, 2
+QMC_BATCH_ORDER_COST, QMC_BATCH_ORDER_GIVEN = 0, 1
 
 EXPORTED = [
     "qmc_system_create", "qmc_system_destroy", "qmc_h2_molecule_mo_create", "qmc_sto_slater_det_create",
@@ -44,6 +46,8 @@ EXPORTED = [
     "qmc_ensemble_timing", "qmc_vmc_run", "qmc_derivatives", "qmc_log_derivs", "qmc_dEL_dp", "qmc_hf_force",
     "qmc_nuc_grad_log", "qmc_sr_solve", "qmc_ensemble_measure", "qmc_ensemble_moments", "qmc_sr_optimize",
     "qmc_sample_forces",
+    "qmc_batch_create", "qmc_batch_destroy", "qmc_batch_fusable", "qmc_batch_set_config", "qmc_batch_get_config",
+    "qmc_batch_sweeps", "qmc_batch_timing", "qmc_batch_tune",
     "qmc_autocorrelation_time", "qmc_blocking_error", "qmc_last_error", "qmc_version",
 ]
 
@@ -103,6 +107,10 @@ class QmcForceParams(C.Structure):
                 ("zero_variance", _i32), ("decorrelation", _i32), ("move", _i32), ("refresh_interval", _i32)]
 
 
+class QmcBatchConfig(C.Structure):
+    _fields_ = [("mode", _i32), ("blocks_per_cu", _i32), ("steps_per_launch", _i32), ("order", _i32)]
+
+
 class QmcVmcResults(C.Structure):
     _fields_ = [("energy", _d), ("error", _d), ("autocorrelation_time", _d), ("acceptance", _d),
                 ("final_step_size", _d)]
@@ -154,6 +162,14 @@ _SIGS = {
     "qmc_ensemble_moments": (_i32, [_p, _p]),
     "qmc_sr_optimize": (_i32, [_p, _p, _u64, _p, _p, _p, _p]),
     "qmc_sample_forces": (_i32, [_p, _p, _u64, _p, _p, _p, _p]),
+    "qmc_batch_create": (_i32, [_p, _i32, _pp]),
+    "qmc_batch_destroy": (_i32, [_p]),
+    "qmc_batch_fusable": (_i32, [_p, _p]),
+    "qmc_batch_set_config": (_i32, [_p, _p]),
+    "qmc_batch_get_config": (_i32, [_p, _p]),
+    "qmc_batch_sweeps": (_i32, [_p, _i32, _p, _i32, _p, _p, _p]),
+    "qmc_batch_timing": (_i32, [_p, _p, _p]),
+    "qmc_batch_tune": (_i32, [_p, _i32, _p, _i32, _p, _p]),
     "qmc_autocorrelation_time": (_d, [_p, _i64]),
     "qmc_blocking_error": (_d, [_p, _i64, _d]),
     "qmc_last_error": (C.c_char_p, []),

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "qmc_device.hpp"
+#include "qmc_kernels.hpp"
 
 namespace qmc {
 
@@ -278,5 +279,116 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+
+// ---------------------------------------------------------------------------
+// Ensemble state in HBM (structure-of-arrays, walker index fastest) <->
+// one walker's registers.
+// ---------------------------------------------------------------------------
+template <int NU, int ND, class T>
+__device__ __forceinline__ void load_positions(const DevSys<T>& S, const T* pos, int64_t ld, int64_t w,
+                                               WalkerRegs<NU, ND, T>& W) {
+#pragma unroll
+  for (int k = 0; k < NU; ++k) {
+    const int e = S.sector_elec[0][k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) W.ru[k][c] = pos[(e * 3 + c) * ld + w];
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const int e = S.sector_elec[1][k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) W.rd[k][c] = pos[(e * 3 + c) * ld + w];
+  }
+}
+
+template <int NU, int ND, class T>
+__device__ __forceinline__ void store_positions(const DevSys<T>& S, T* pos, int64_t ld, int64_t w,
+                                                const WalkerRegs<NU, ND, T>& W) {
+#pragma unroll
+  for (int k = 0; k < NU; ++k) {
+    const int e = S.sector_elec[0][k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pos[(e * 3 + c) * ld + w] = W.ru[k][c];
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const int e = S.sector_elec[1][k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pos[(e * 3 + c) * ld + w] = W.rd[k][c];
+  }
+}
+
+template <int NU, int ND, class T>
+__device__ __forceinline__ void load_inverse(const T* ainv, int64_t ld, int64_t w, WalkerRegs<NU, ND, T>& W) {
+#pragma unroll
+  for (int m = 0; m < NU; ++m)
+#pragma unroll
+    for (int k = 0; k < NU; ++k) W.au[m][k] = ainv[(m * NU + k) * ld + w];
+#pragma unroll
+  for (int m = 0; m < ND; ++m)
+#pragma unroll
+    for (int k = 0; k < ND; ++k) W.ad[m][k] = ainv[(NU * NU + m * ND + k) * ld + w];
+}
+
+template <int NU, int ND, class T>
+__device__ __forceinline__ void store_inverse(T* ainv, int64_t ld, int64_t w, const WalkerRegs<NU, ND, T>& W) {
+#pragma unroll
+  for (int m = 0; m < NU; ++m)
+#pragma unroll
+    for (int k = 0; k < NU; ++k) ainv[(m * NU + k) * ld + w] = W.au[m][k];
+#pragma unroll
+  for (int m = 0; m < ND; ++m)
+#pragma unroll
+    for (int k = 0; k < ND; ++k) ainv[(NU * NU + m * ND + k) * ld + w] = W.ad[m][k];
+}
+
+// ---------------------------------------------------------------------------
+// n_steps single-electron sweeps of the 64 walkers of block blk (one
+// walker per lane), E_L after each, and per-(step, block) partial sums
+// {sum E_L, sum E_L^2, accepted moves} into partial[step][nblk][3].
+// ---------------------------------------------------------------------------
+template <int NU, int ND, class Bs, class Js, class T>
+__device__ __forceinline__ void sweep_block(const DevSys<T>& S, EnsView<T> E, SweepArgs A,
+                                            double* __restrict__ partial, double* __restrict__ el_trace,
+                                            int64_t blk, int64_t nblk, int lane) {
+  const int64_t w = blk * 64 + lane;
+  const bool live = w < E.W;
+  const int64_t wl = live ? w : E.W - 1;  // dead lanes shadow the last walker and write nothing
+  WalkerRegs<NU, ND, T> R;
+  load_positions<NU, ND>(S, E.pos, E.ld, wl, R);
+  load_inverse<NU, ND>(E.ainv, E.ld, wl, R);
+  double logdet = E.logdet[wl], sgn = E.sign[wl], ju = E.ju[wl];
+  double el = 0.0;
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  const uint32_t wid = (uint32_t)(A.walker0 + wl);
+  const T sigma = (T)A.step_size;
+  for (int s = 0; s < A.n_steps; ++s) {
+    const uint32_t step = A.step0 + (uint32_t)s;
+    int acc = 0;
+    sector_moves<NU, ND, T, Bs, Js>(S, 0, R.ru, R.au, R.rd, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
+    sector_moves<ND, NU, T, Bs, Js>(S, 1, R.rd, R.ad, R.ru, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
+    el = local_energy<NU, ND, T, Bs, Js>(S, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
+    if (el_trace && live) el_trace[(int64_t)s * A.trace_ld + w] = el;
+    double e1 = live ? el : 0.0, e2 = live ? el * el : 0.0, a = live ? (double)acc : 0.0;
+    e1 = wave_sum(e1);
+    e2 = wave_sum(e2);
+    a = wave_sum(a);
+    if (lane == 0) {
+      double* p = partial + ((int64_t)s * nblk + blk) * 3;
+      p[0] = e1;
+      p[1] = e2;
+      p[2] = a;
+    }
+  }
+  if (live) {
+    store_positions<NU, ND>(S, E.pos, E.ld, w, R);
+    store_inverse<NU, ND>(E.ainv, E.ld, w, R);
+    E.logdet[w] = logdet;
+    E.sign[w] = sgn;
+    E.ju[w] = ju;
+    E.eloc[w] = el;
+  }
+}
+
 
 }  // namespace qmc

@@ -18,6 +18,7 @@
 #include "qmc_derivs.hpp"
 #include "qmc_device.hpp"
 #include "qmc_kernels.hpp"
+#include "qmc_batch.hpp"
 #include "qmc_tile.hpp"
 #include "qmc_wave.hpp"
 
@@ -1177,6 +1178,35 @@ int ensure_cap(double** p, size_t* cap, size_t n) {
   return QMC_OK;
 }
 
+// blocks (partial-sum rows) of one single-electron sweep launch of e
+int64_t sweep_blocks(const qmc_ensemble* e) {
+  const qmc_system* s = e->sys;
+  if (s->tile_sweeps(e->prec)) return tile_blocks(e->W);
+  if (s->wave()) return e->W;
+  return (e->W + 63) / 64;
+}
+
+// enqueue A.n_steps single-electron sweeps of e on st with the engine the
+// system selects (partial sums into e->partial)
+int enqueue_single_sweeps(qmc_ensemble* e, const SweepArgs& A, double* trace, hipStream_t st) {
+  const qmc_system* s = e->sys;
+  if (s->tile_sweeps(e->prec)) {
+    HIP_TRY(tile_sweep(s->d32, e->view<float>(), s->tile, A, e->partial, trace, st));
+  } else if (s->wave()) {
+    if (e->prec == QMC_FP64)
+      HIP_TRY(wave_sweep<double>(s->d64, e->view<double>(), s->shape(), A, e->partial, trace, st));
+    else
+      HIP_TRY(wave_sweep<float>(s->d32, e->view<float>(), s->shape(), A, e->partial, trace, st));
+  } else {
+    const bool no_series = s->h.k_ee == 0 && s->h.k_en == 0;
+    if (e->prec == QMC_FP64)
+      HIP_TRY((no_series ? s->ops->sweep64_ns : s->ops->sweep64)(s->d64, e->view<double>(), A, e->partial, trace, st));
+    else
+      HIP_TRY((no_series ? s->ops->sweep32_ns : s->ops->sweep32)(s->d32, e->view<float>(), A, e->partial, trace, st));
+  }
+  return QMC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1286,8 +1316,7 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
   const bool wave = s->wave();
   if (wave && p->move != QMC_MOVE_SINGLE)
     return fail(QMC_ERR_UNSUPPORTED, "the wave engine runs single-electron moves only");
-  const bool tile = p->move == QMC_MOVE_SINGLE && s->tile_sweeps(e->prec);
-  const int nblk = tile ? (int)tile_blocks(e->W) : wave ? (int)e->W : (int)((e->W + 63) / 64);
+  const int nblk = p->move == QMC_MOVE_SINGLE ? (int)sweep_blocks(e) : (int)((e->W + 63) / 64);
   const int refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
   int chunk = p->steps_per_launch > 0 ? p->steps_per_launch : 16;
   if ((rc = ensure_cap(&e->partial, &e->partial_cap, (size_t)chunk * nblk * 3))) return rc;
@@ -1313,21 +1342,8 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
       }
       HIP_TRY(hipEventRecord(e->ev[2 * nl], e->st));
     }
-    if (tile) {
-      HIP_TRY(tile_sweep(s->d32, e->view<float>(), s->tile, A, e->partial, trace, e->st));
-    } else if (wave) {
-      if (e->prec == QMC_FP64)
-        HIP_TRY(wave_sweep<double>(s->d64, e->view<double>(), s->shape(), A, e->partial, trace, e->st));
-      else
-        HIP_TRY(wave_sweep<float>(s->d32, e->view<float>(), s->shape(), A, e->partial, trace, e->st));
-    } else if (p->move == QMC_MOVE_SINGLE) {
-      const bool no_series = s->h.k_ee == 0 && s->h.k_en == 0;
-      if (e->prec == QMC_FP64)
-        HIP_TRY((no_series ? s->ops->sweep64_ns : s->ops->sweep64)(s->d64, e->view<double>(), A, e->partial, trace,
-                                                                   e->st));
-      else
-        HIP_TRY((no_series ? s->ops->sweep32_ns : s->ops->sweep32)(s->d32, e->view<float>(), A, e->partial, trace,
-                                                                   e->st));
+    if (p->move == QMC_MOVE_SINGLE) {
+      if ((rc = enqueue_single_sweeps(e, A, trace, e->st))) return rc;
     } else {
       if (e->prec == QMC_FP64)
         HIP_TRY(s->ops->ae64(s->d64, s->d64, e->view<double>(), A, e->partial, trace, e->st));
@@ -1652,3 +1668,306 @@ int qmc_vmc_run(const qmc_system* sys, const qmc_mcmc_params* p, uint64_t seed, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// heterogeneous batches
+// ---------------------------------------------------------------------------
+
+struct qmc_batch {
+  std::vector<qmc_ensemble*> ens;
+  std::vector<int> kind;  // fused-kernel segment kind per ensemble (-1: none)
+  bool fusable = false;
+  qmc_batch_config cfg{QMC_BATCH_FUSED, 0, 8, QMC_BATCH_ORDER_COST};
+  hipStream_t st = nullptr;
+  std::vector<hipStream_t> sst;  // one stream per ensemble (STREAMS mode)
+  int* counter = nullptr;
+  std::vector<hipEvent_t> ev;
+  double kernel_ms = 0.0;
+  int64_t launches = 0;
+  ~qmc_batch() {
+    if (st) (void)hipStreamDestroy(st);
+    for (hipStream_t x : sst) (void)hipStreamDestroy(x);
+    if (counter) (void)hipFree(counter);
+    for (hipEvent_t x : ev) (void)hipEventDestroy(x);
+  }
+};
+
+namespace {
+
+int segment_kind(const qmc_ensemble* e) {
+  const qmc_system* s = e->sys;
+  if (e->prec != QMC_MIXED || s->h.k_ee != 0 || s->h.k_en != 0) return -1;
+  if (s->tile_sweeps(e->prec)) return SEG_TILE_21_21;
+  if (s->wave() || !s->ops || !s->ops->match) return -1;
+  const std::string n = s->ops->name;
+  if (n == "ch4-sto3g") return SEG_REG_CH4_STO3G;
+  if (n == "h2-slater1s") return SEG_REG_H2;
+  return -1;
+}
+
+// relative cost of one queue item of a segment kind (walkers x flop per walker-move)
+double item_cost(int kind) {
+  switch (kind) {
+    case SEG_TILE_21_21: return 16.0 * 827643.0;
+    case SEG_REG_CH4_STO3G: return 256.0 * 16338.0;
+    default: return 256.0 * 524.0;
+  }
+}
+
+hipEvent_t batch_event(qmc_batch* b, size_t i) {
+  while (b->ev.size() <= i) {
+    hipEvent_t x;
+    if (hipEventCreate(&x) != hipSuccess) return nullptr;
+    b->ev.push_back(x);
+  }
+  return b->ev[i];
+}
+
+// one chunk of ns sweeps of every ensemble with the current configuration
+int batch_chunk(qmc_batch* b, int ns, const double* step_size, int done, size_t* nev) {
+  const size_t n = b->ens.size();
+  std::vector<SweepArgs> args(n);
+  int rc;
+  for (size_t i = 0; i < n; ++i) {
+    qmc_ensemble* e = b->ens[i];
+    if ((rc = ensure_cap(&e->partial, &e->partial_cap, (size_t)ns * sweep_blocks(e) * 3))) return rc;
+    args[i] = SweepArgs{ns, e->step, e->seed, 0, step_size[i], e->W};
+  }
+  hipEvent_t e0 = batch_event(b, (*nev)++), e1 = batch_event(b, (*nev)++);
+  if (!e0 || !e1) return fail(QMC_ERR_HIP, "hipEventCreate failed");
+  HIP_TRY(hipEventRecord(e0, b->st));
+  if (b->cfg.mode == QMC_BATCH_FUSED) {
+    BatchLaunch L;
+    L.nseg = (int)n;
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    if (b->cfg.order == QMC_BATCH_ORDER_COST)
+      std::stable_sort(order.begin(), order.end(),
+                       [&](size_t x, size_t y) { return item_cost(b->kind[x]) > item_cost(b->kind[y]); });
+    L.start[0] = 0;
+    for (size_t j = 0; j < n; ++j) {
+      const size_t i = order[j];
+      qmc_ensemble* e = b->ens[i];
+      BatchSeg& G = L.seg[j];
+      G.kind = b->kind[i];
+      G.S = e->sys->d32;
+      G.E = e->view<float>();
+      G.A = args[i];
+      G.partial = e->partial;
+      G.nblk = sweep_blocks(e);
+      G.items = G.kind == SEG_TILE_21_21 ? G.nblk : (G.nblk + 3) / 4;
+      if (G.kind == SEG_TILE_21_21) L.tile = e->sys->tile;
+      L.start[j + 1] = L.start[j] + G.items;
+    }
+    L.total = L.start[n];
+    L.counter = b->counter;
+    int per_cu = 0, dev = 0, cus = 0;
+    HIP_TRY(batch_occupancy(batch_lds_bytes(L), &per_cu));
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (per_cu < 1) return fail(QMC_ERR_UNSUPPORTED, "the fused batch kernel does not fit a CU");
+    const int bpc = b->cfg.blocks_per_cu > 0 ? std::min(b->cfg.blocks_per_cu, per_cu) : per_cu;
+    const int64_t grid = std::min<int64_t>((int64_t)bpc * cus, L.total);
+    HIP_TRY(hipMemsetAsync(b->counter, 0, sizeof(int), b->st));
+    HIP_TRY(batch_launch(L, (int)std::max<int64_t>(grid, 1), b->st));
+  } else {
+    const bool par = b->cfg.mode == QMC_BATCH_STREAMS;
+    for (size_t i = 0; i < n; ++i) {
+      hipStream_t x = par ? b->sst[i] : b->st;
+      if (par) HIP_TRY(hipStreamWaitEvent(x, e0, 0));
+      if ((rc = enqueue_single_sweeps(b->ens[i], args[i], nullptr, x))) return rc;
+      if (par) {
+        hipEvent_t ei = batch_event(b, (*nev)++);
+        if (!ei) return fail(QMC_ERR_HIP, "hipEventCreate failed");
+        HIP_TRY(hipEventRecord(ei, x));
+        HIP_TRY(hipStreamWaitEvent(b->st, ei, 0));
+      }
+    }
+  }
+  HIP_TRY(hipEventRecord(e1, b->st));
+  ++b->launches;
+  for (size_t i = 0; i < n; ++i) {
+    qmc_ensemble* e = b->ens[i];
+    HIP_TRY(launch_reduce(e->partial, ns, (int)sweep_blocks(e), 3, e->sums + (size_t)done * 3, b->st));
+    e->step += (uint32_t)ns;
+    e->since_refresh += ns;
+  }
+  return QMC_OK;
+}
+
+int batch_run(qmc_batch* b, int32_t n_steps, const double* step_size, int32_t refresh, double* e_sum, double* e2_sum,
+              double* accepted) {
+  const size_t n = b->ens.size();
+  int rc;
+  // every ensemble's work goes to the batch stream while the batch runs
+  std::vector<hipStream_t> saved(n);
+  for (size_t i = 0; i < n; ++i) {
+    saved[i] = b->ens[i]->st;
+    HIP_TRY(hipStreamSynchronize(saved[i]));
+    b->ens[i]->st = b->st;
+  }
+  struct Restore {
+    qmc_batch* b;
+    std::vector<hipStream_t>& s;
+    ~Restore() {
+      for (size_t i = 0; i < s.size(); ++i) b->ens[i]->st = s[i];
+    }
+  } restore{b, saved};
+  for (qmc_ensemble* e : b->ens) {
+    if (e->sys_version != e->sys->version && (rc = ens_refresh(e, 1))) return rc;
+    if (e->inverse_stale && (rc = ens_refresh(e, 0))) return rc;
+    if ((rc = ensure_cap(&e->sums, &e->sums_cap, (size_t)n_steps * 3))) return rc;
+  }
+  size_t nev = 0;
+  b->launches = 0;
+  const int chunk = b->cfg.steps_per_launch > 0 ? b->cfg.steps_per_launch : 8;
+  for (int done = 0; done < n_steps;) {
+    int ns = std::min(chunk, n_steps - done);
+    if (refresh > 0)
+      for (qmc_ensemble* e : b->ens) {
+        if (e->since_refresh >= refresh && (rc = ens_refresh(e, 0))) return rc;
+        ns = std::min(ns, refresh - e->since_refresh);
+      }
+    if ((rc = batch_chunk(b, ns, step_size, done, &nev))) return rc;
+    done += ns;
+    if (refresh > 0)
+      for (qmc_ensemble* e : b->ens)
+        if (e->since_refresh >= refresh && (rc = ens_refresh(e, 0))) return rc;
+  }
+  std::vector<double> h(n * (size_t)n_steps * 3);
+  for (size_t i = 0; i < n; ++i)
+    HIP_TRY(hipMemcpyAsync(h.data() + i * (size_t)n_steps * 3, b->ens[i]->sums, sizeof(double) * n_steps * 3,
+                           hipMemcpyDeviceToHost, b->st));
+  HIP_TRY(hipStreamSynchronize(b->st));
+  b->kernel_ms = 0.0;
+  for (size_t k = 0; k + 1 < nev;) {
+    // chunk events come as (start, end, per-stream events ...); pair each start with its end
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev[k], b->ev[k + 1]));
+    b->kernel_ms += ms;
+    k += 2 + (b->cfg.mode == QMC_BATCH_STREAMS ? n : 0);
+  }
+  for (size_t i = 0; i < n; ++i)
+    for (int q = 0; q < 3; ++q) {
+      double* dst = q == 0 ? e_sum : (q == 1 ? e2_sum : accepted);
+      if (!dst) continue;
+      for (int t = 0; t < n_steps; ++t) dst[i * (size_t)n_steps + t] = h[(i * (size_t)n_steps + t) * 3 + q];
+    }
+  return QMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qmc_batch_create(qmc_ensemble* const* ens, int32_t n, qmc_batch** out) {
+  if (!ens || !out || n < 1 || n > BATCH_MAXSEG) return fail(QMC_ERR_INVALID, "bad arguments (1 <= n <= 8)");
+  auto b = std::make_unique<qmc_batch>();
+  int tiles = 0;
+  b->fusable = true;
+  for (int i = 0; i < n; ++i) {
+    if (!ens[i]) return fail(QMC_ERR_INVALID, "null ensemble");
+    for (int j = 0; j < i; ++j)
+      if (ens[j] == ens[i]) return fail(QMC_ERR_INVALID, "an ensemble appears twice");
+    b->ens.push_back(ens[i]);
+    const int k = segment_kind(ens[i]);
+    b->kind.push_back(k);
+    if (k < 0) b->fusable = false;
+    if (k == SEG_TILE_21_21) ++tiles;
+  }
+  if (tiles > 1) b->fusable = false;  // one tile plan per fused launch
+  if (!b->fusable) b->cfg.mode = QMC_BATCH_STREAMS;
+  HIP_TRY(hipStreamCreate(&b->st));
+  b->sst.resize(n);
+  for (int i = 0; i < n; ++i) HIP_TRY(hipStreamCreate(&b->sst[i]));
+  HIP_TRY(hipMalloc(&b->counter, sizeof(int)));
+  *out = b.release();
+  return QMC_OK;
+}
+
+int qmc_batch_destroy(qmc_batch* b) {
+  if (b) (void)hipStreamSynchronize(b->st);
+  delete b;
+  return QMC_OK;
+}
+
+int qmc_batch_fusable(const qmc_batch* b, int32_t* fusable) {
+  if (!b || !fusable) return fail(QMC_ERR_INVALID, "bad arguments");
+  *fusable = b->fusable ? 1 : 0;
+  return QMC_OK;
+}
+
+int qmc_batch_set_config(qmc_batch* b, const qmc_batch_config* c) {
+  if (!b || !c) return fail(QMC_ERR_INVALID, "bad arguments");
+  if (c->mode < QMC_BATCH_FUSED || c->mode > QMC_BATCH_SEQUENTIAL || c->blocks_per_cu < 0 ||
+      c->steps_per_launch < 0 || (c->order != QMC_BATCH_ORDER_COST && c->order != QMC_BATCH_ORDER_GIVEN))
+    return fail(QMC_ERR_INVALID, "bad batch configuration");
+  if (c->mode == QMC_BATCH_FUSED && !b->fusable)
+    return fail(QMC_ERR_UNSUPPORTED, "these ensembles cannot share the fused kernel");
+  b->cfg = *c;
+  return QMC_OK;
+}
+
+int qmc_batch_get_config(const qmc_batch* b, qmc_batch_config* c) {
+  if (!b || !c) return fail(QMC_ERR_INVALID, "bad arguments");
+  *c = b->cfg;
+  return QMC_OK;
+}
+
+int qmc_batch_sweeps(qmc_batch* b, int32_t n_steps, const double* step_size, int32_t refresh_interval, double* e_sum,
+                     double* e2_sum, double* accepted) {
+  if (!b || !step_size || n_steps < 0 || refresh_interval < 0) return fail(QMC_ERR_INVALID, "bad arguments");
+  for (size_t i = 0; i < b->ens.size(); ++i)
+    if (!(step_size[i] >= 0.0)) return fail(QMC_ERR_INVALID, "bad step size");
+  if (n_steps == 0) return QMC_OK;
+  return batch_run(b, n_steps, step_size, refresh_interval, e_sum, e2_sum, accepted);
+}
+
+int qmc_batch_timing(const qmc_batch* b, double* kernel_ms, int64_t* launches) {
+  if (!b) return fail(QMC_ERR_INVALID, "null batch");
+  if (kernel_ms) *kernel_ms = b->kernel_ms;
+  if (launches) *launches = b->launches;
+  return QMC_OK;
+}
+
+int qmc_batch_tune(qmc_batch* b, int32_t trial_steps, const double* step_size, int32_t refresh_interval,
+                   qmc_batch_config* best, double* best_ms_per_sweep) {
+  if (!b || !step_size || trial_steps < 1) return fail(QMC_ERR_INVALID, "bad arguments");
+  std::vector<qmc_batch_config> cand;
+  for (int spl : {4, 8, 16}) {
+    if (b->fusable)
+      for (int bpc : {1, 2})
+        for (int order : {QMC_BATCH_ORDER_COST, QMC_BATCH_ORDER_GIVEN})
+          cand.push_back(qmc_batch_config{QMC_BATCH_FUSED, bpc, spl, order});
+    cand.push_back(qmc_batch_config{QMC_BATCH_STREAMS, 0, spl, QMC_BATCH_ORDER_GIVEN});
+    cand.push_back(qmc_batch_config{QMC_BATCH_SEQUENTIAL, 0, spl, QMC_BATCH_ORDER_GIVEN});
+  }
+  const qmc_batch_config keep = b->cfg;
+  double best_ms = std::numeric_limits<double>::infinity();
+  qmc_batch_config pick = keep;
+  int rc;
+  for (const auto& c : cand) {
+    b->cfg = c;
+    if ((rc = batch_run(b, c.steps_per_launch, step_size, refresh_interval, nullptr, nullptr, nullptr))) {
+      b->cfg = keep;
+      return rc;
+    }
+    const int steps = std::max(trial_steps, c.steps_per_launch);
+    if ((rc = batch_run(b, steps, step_size, refresh_interval, nullptr, nullptr, nullptr))) {
+      b->cfg = keep;
+      return rc;
+    }
+    const double ms = b->kernel_ms / steps;
+    if (ms < best_ms) {
+      best_ms = ms;
+      pick = c;
+    }
+  }
+  b->cfg = pick;
+  if (best) *best = pick;
+  if (best_ms_per_sweep) *best_ms_per_sweep = best_ms;
+  return QMC_OK;
+}
+
+}  // extern "C"
+
