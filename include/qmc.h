@@ -229,13 +229,16 @@ int qmc_ensemble_refresh(qmc_ensemble* ens);
 
 #define QMC_MOVE_SINGLE 0 /* single-electron Gaussian moves, Sherman-Morrison */
 #define QMC_MOVE_ALL 1    /* all-electron moves, MCMCSimulation::metropolis_step (vmc.rs:95-119) */
+#define QMC_MOVE_DRIFT 2  /* drift-diffusion single-electron moves, DriftDiffusionVMC::sweep
+                             (vmc.rs:382-438); register engine */
 
 typedef struct {
   int32_t move;             /* QMC_MOVE_* */
   int32_t n_steps;          /* sweeps to run */
-  double step_size;         /* Gaussian proposal width (bohr) */
+  double step_size;         /* Gaussian proposal width (bohr), or the time step tau of QMC_MOVE_DRIFT */
   int32_t refresh_interval; /* FP64 inverse refresh every this many sweeps (0: never) */
   int32_t steps_per_launch; /* sweeps per persistent launch (0: library default) */
+  double max_drift;         /* QMC_MOVE_DRIFT: cap of |tau F| per electron (vmc.rs:313-321) */
 } qmc_sweep_params;
 
 /* Run sweeps.  Step counters continue across calls.  Outputs (any may be
@@ -383,6 +386,31 @@ typedef struct {
  * energy_trace (host) receives the per-step mean energies [n_steps]. */
 int qmc_vmc_run(const qmc_system* sys, const qmc_mcmc_params* p, uint64_t seed, qmc_vmc_results* res,
                 double* energy_trace);
+
+/* DDVMCParams (vmc.rs:196-212) */
+typedef struct {
+  int64_t n_walkers;
+  int64_t n_steps;  /* production sweeps */
+  int64_t n_burnin; /* burn-in sweeps (discarded) */
+  double time_step;
+  double max_drift;
+  double target_acceptance;
+  int64_t adaptation_interval;
+  int32_t refresh_interval;
+} qmc_ddvmc_params;
+/* DDVMCResults (vmc.rs:228-239) */
+typedef struct {
+  double energy;
+  double error;
+  double autocorrelation_time;
+  double acceptance_rate;
+  double final_time_step;
+} qmc_ddvmc_results;
+/* DriftDiffusionVMC::run (vmc.rs:441-499): burn-in with time-step
+ * adaptation, production with windowed adaptation (vmc.rs:373-380), mean
+ * E_L of every production sweep; energy_trace (host) receives [n_steps]. */
+int qmc_ddvmc_run(const qmc_system* sys, const qmc_ddvmc_params* p, uint64_t seed, qmc_ddvmc_results* res,
+                  double* energy_trace);
 
 /* vmc.rs:141-163 and vmc.rs:166-188 on a host series */
 double qmc_autocorrelation_time(const double* e, int64_t n);

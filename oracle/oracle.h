@@ -47,6 +47,7 @@
 #define ORC_RNG_SINGLE 2u
 #define ORC_RNG_ALL 3u
 #define ORC_RNG_SYS 4u
+#define ORC_RNG_DRIFT 5u
 
 /* ---- slater.rs:10-42 ---- */
 typedef struct { double alpha; double center[3]; } orc_slater1s;
@@ -218,6 +219,8 @@ void orc_mcmc_time_steps(const orc_system* s, int64_t W, int64_t warm, int64_t s
 void orc_single_electron_steps(const orc_system* s, int64_t W, double* pos, double* logpsi,
                                uint64_t seed, uint32_t step0, int nsteps, double step,
                                double* eloc, int64_t* accepted);
+void orc_dd_steps(const orc_system* s, int64_t W, double* pos, uint64_t seed, uint32_t step0, int nsteps,
+                  double tau, double max_drift, int no_green, double* eloc, int64_t* accepted);
 void orc_all_electron_steps(const orc_system* s, int64_t W, double* pos, double* logpsi,
                             double* eloc, uint64_t seed, uint32_t step0, int nsteps, double step,
                             int64_t* accepted);

@@ -16,7 +16,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-RNG_INIT, RNG_SINGLE, RNG_ALL, RNG_SYS = 1, 2, 3, 4
+RNG_INIT, RNG_SINGLE, RNG_ALL, RNG_SYS, RNG_DRIFT = 1, 2, 3, 4, 5
 
 
 def build(force: bool = False) -> str:
@@ -95,6 +95,7 @@ def lib():
             "orc_mcmc_run": (None, [_p, _p, _u64, _p, _p]),
             "orc_mcmc_time_steps": (None, [_p, _i64, _i64, _i64, _d, _u64, _p, _p]),
             "orc_single_electron_steps": (None, [_p, _i64, _p, _p, _u64, _u32, _i, _d, _p, _p]),
+            "orc_dd_steps": (None, [_p, _i64, _p, _u64, _u32, _i, _d, _d, _i, _p, _p]),
             "orc_all_electron_steps": (None, [_p, _i64, _p, _p, _p, _u64, _u32, _i, _d, _p]),
             "orc_sr_update": (_i, [_i64, _i, _p, _p, _d, _p, _p, _p]),
             "orc_zv_forces": (None, [_i64, _i, _p, _p, _p, _i, _p, _p, _p, _p]),
@@ -339,6 +340,16 @@ class System:
         lib().orc_single_electron_steps(self._h, W, _ptr(pos), _ptr(lp), seed, step0, nsteps, step,
                                         _ptr(el), C.byref(acc))
         return pos, lp, el, acc.value
+
+    def dd_steps(self, pos, seed, step0, nsteps, tau, max_drift, no_green=False):
+        """DriftDiffusionVMC sweeps (vmc.rs:382-438) with the GPU's counters."""
+        pos = _f64(pos).copy()
+        W = pos.shape[0]
+        el = np.zeros((nsteps, W))
+        acc = C.c_int64()
+        lib().orc_dd_steps(self._h, W, _ptr(pos), seed, step0, nsteps, tau, max_drift, int(no_green), _ptr(el),
+                           C.byref(acc))
+        return pos, el, acc.value
 
     def all_electron_steps(self, pos, logpsi, eloc, seed, step0, nsteps, step):
         pos = _f64(pos).copy()

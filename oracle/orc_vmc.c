@@ -195,6 +195,93 @@ void orc_single_electron_steps(const orc_system* s, int64_t W, double* pos, doub
     if (accepted) *accepted = acc;
 }
 
+/* DriftDiffusionVMC (vmc.rs:195-438) literally: drift F_i = 2 grad_i psi / psi
+ * (zero when |psi| < 1e-30, vmc.rs:255-261), the displacement tau F capped
+ * at max_drift in norm (vmc.rs:313-321), Langevin proposal
+ * r_i' = r_i + cap(tau F_i) + sqrt(2 tau) z, moves with |psi'| < 1e-30
+ * skipped, acceptance |psi'/psi|^2 exp(G_b - G_f) with
+ * G_f = -|r' - r - cap(tau F(r))|^2 / (4 tau),
+ * G_b = -|r - r' - cap(tau F'(r'))|^2 / (4 tau) (vmc.rs:323-346, 375-431),
+ * full recomputation of psi and of every drift at the trial configuration.
+ * Electrons in the sector order of the GPU sampler; z from counter
+ * (w, step, 2e, DRIFT), U from (w, step, 2e+1, DRIFT) word 0.  E_L after
+ * each sweep into eloc [nsteps][W].  no_green != 0 drops the Green's
+ * function factor (a deliberately wrong sampler for the tests). */
+static void dd_drift(const orc_system* s, const double (*r)[3], double psi, double (*F)[3]) {
+    int n = s->n_elec;
+    if (fabs(psi) < 1e-30) {
+        for (int e = 0; e < n; ++e) F[e][0] = F[e][1] = F[e][2] = 0.0;
+        return;
+    }
+    orc_derivative(s, r, F);
+    for (int e = 0; e < n; ++e)
+        for (int q = 0; q < 3; ++q) F[e][q] = 2.0 * F[e][q] / psi;
+}
+
+static void dd_cap(const double F[3], double tau, double max_drift, double d[3]) {
+    double m = 0.0;
+    for (int q = 0; q < 3; ++q) {
+        d[q] = tau * F[q];
+        m += d[q] * d[q];
+    }
+    m = sqrt(m);
+    if (m > max_drift)
+        for (int q = 0; q < 3; ++q) d[q] *= max_drift / m;
+}
+
+void orc_dd_steps(const orc_system* s, int64_t W, double* pos, uint64_t seed, uint32_t step0, int nsteps,
+                  double tau, double max_drift, int no_green, double* eloc, int64_t* accepted) {
+    int n = s->n_elec;
+    int64_t acc = 0;
+    int order[ORC_MAXE], q = 0;
+    for (int e = 0; e < n; ++e)
+        if (s->spins[e] == 1) order[q++] = e;
+    for (int e = 0; e < n; ++e)
+        if (s->spins[e] != 1) order[q++] = e;
+    const double s2t = sqrt(2.0 * tau);
+    for (int64_t w = 0; w < W; ++w) {
+        double(*r)[3] = (double(*)[3])(pos + w * n * 3);
+        double psi = orc_evaluate(s, (const double(*)[3])r);
+        double F[ORC_MAXE][3], Fn[ORC_MAXE][3], t[ORC_MAXE][3];
+        dd_drift(s, (const double(*)[3])r, psi, F);
+        for (int k = 0; k < nsteps; ++k) {
+            uint32_t st = step0 + (uint32_t)k;
+            for (int qq = 0; qq < n; ++qq) {
+                const int e = order[qq];
+                double z[3], d[3];
+                orc_normals3(seed, (uint32_t)w, st, (uint32_t)(2 * e), ORC_RNG_DRIFT, z);
+                dd_cap(F[e], tau, max_drift, d);
+                memcpy(t, r, sizeof(double) * 3 * n);
+                for (int c = 0; c < 3; ++c) t[e][c] = r[e][c] + d[c] + s2t * z[c];
+                const double psin = orc_evaluate(s, (const double(*)[3])t);
+                if (fabs(psin) < 1e-30) continue;
+                dd_drift(s, (const double(*)[3])t, psin, Fn);
+                double dn[3], gf = 0.0, gb = 0.0;
+                dd_cap(Fn[e], tau, max_drift, dn);
+                for (int c = 0; c < 3; ++c) {
+                    const double a = t[e][c] - r[e][c] - d[c], b = r[e][c] - t[e][c] - dn[c];
+                    gf -= a * a;
+                    gb -= b * b;
+                }
+                gf /= 4.0 * tau;
+                gb /= 4.0 * tau;
+                const double ratio = psin / psi;
+                double p = ratio * ratio * (no_green ? 1.0 : exp(gb - gf));
+                if (p > 1.0) p = 1.0;
+                const double u = orc_u01(philox_w0(seed, (uint32_t)w, st, (uint32_t)(2 * e + 1), ORC_RNG_DRIFT));
+                if (u < p) {
+                    memcpy(r, t, sizeof(double) * 3 * n);
+                    psi = psin;
+                    memcpy(F, Fn, sizeof(double) * 3 * n);
+                    ++acc;
+                }
+            }
+            if (eloc) eloc[(int64_t)k * W + w] = orc_local_energy(s, (const double(*)[3])r);
+        }
+    }
+    if (accepted) *accepted = acc;
+}
+
 /* The all-electron move of vmc.rs:95-119 in the log domain, stepping a
  * caller-held ensemble: cached E_L kept on reject (vmc.rs:111-116).
  * eloc is [W] in/out (the current cached energies). */

@@ -25,14 +25,15 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (QMC_FP64, QMC_MIXED, QMC_FLAGS_REFERENCE, QMC_MOVE_ALL, QMC_MOVE_SINGLE, QmcError, check,
-                   lib)
+from ._lib import (QMC_FP64, QMC_MIXED, QMC_FLAGS_REFERENCE, QMC_MOVE_ALL, QMC_MOVE_DRIFT, QMC_MOVE_SINGLE,
+                   QmcError, check, lib)
 
 __all__ = [
     "QmcError", "System", "H2MoleculeMO", "Lithium", "STOSlaterDet", "MethaneGTO", "BenzeneGTO", "Ensemble",
     "DeviceBuffer", "check", "lib",
     "MCMCParams", "MCMCResults", "MCMCSimulation", "autocorrelation_time", "blocking_error", "QMC_FP64",
-    "QMC_MIXED", "QMC_MOVE_SINGLE", "QMC_MOVE_ALL", "QMC_FLAGS_REFERENCE", "Batch",
+    "QMC_MIXED", "QMC_MOVE_SINGLE", "QMC_MOVE_ALL", "QMC_MOVE_DRIFT", "QMC_FLAGS_REFERENCE", "Batch",
+    "DDVMCParams", "DDVMCResults", "DriftDiffusionVMC",
 ]
 
 _PREC = {"fp64": QMC_FP64, "mixed": QMC_MIXED, QMC_FP64: QMC_FP64, QMC_MIXED: QMC_MIXED}
@@ -425,9 +426,10 @@ class Ensemble:
         return ms.value, n.value
 
     def sweeps(self, n_steps, step_size, move=QMC_MOVE_SINGLE, refresh_interval=64, steps_per_launch=0,
-               trace=False):
-        """Returns (e_sum, e2_sum, accepted[, eloc_trace]) per sweep."""
-        p = _lib.QmcSweepParams(move, n_steps, step_size, refresh_interval, steps_per_launch)
+               trace=False, max_drift=1.0):
+        """Returns (e_sum, e2_sum, accepted[, eloc_trace]) per sweep.  For
+        move=QMC_MOVE_DRIFT step_size is the time step tau."""
+        p = _lib.QmcSweepParams(move, n_steps, step_size, refresh_interval, steps_per_launch, max_drift)
         es, e2, acc = np.zeros(n_steps), np.zeros(n_steps), np.zeros(n_steps)
         tr = np.zeros((n_steps, self.W)) if trace else None
         check(lib().qmc_ensemble_sweeps(self._h, C.byref(p), _ptr(es), _ptr(e2), _ptr(acc), _ptr(tr)))
@@ -554,6 +556,48 @@ class GeometryOptimizer:
         me, err = C.c_double(), C.c_double()
         check(lib().qmc_sample_forces(wfn._h, C.byref(p), self.seed, _ptr(F), C.byref(me), C.byref(err), _ptr(fv)))
         return F, me.value, err.value, fv
+
+
+@dataclass
+class DDVMCParams:
+    """vmc.rs:196-226 (defaults as Default for DDVMCParams) plus refresh_interval."""
+    n_walkers: int = 10
+    n_steps: int = 100_000
+    n_burnin: int = 1000
+    time_step: float = 0.01
+    max_drift: float = 1.0
+    target_acceptance: float = 0.5
+    adaptation_interval: int = 100
+    refresh_interval: int = 64
+
+
+@dataclass
+class DDVMCResults:
+    """vmc.rs:228-239"""
+    energy: float
+    error: float
+    autocorrelation_time: float
+    acceptance_rate: float
+    final_time_step: float
+
+
+class DriftDiffusionVMC:
+    """vmc.rs:279-535 on the GPU (qmc_ddvmc_run); register-engine systems."""
+
+    def __init__(self, wavefunction, params: DDVMCParams, seed: int = 0):
+        self.wavefunction = wavefunction
+        self.params = params
+        self.seed = seed
+        self.energies = None
+
+    def run(self) -> DDVMCResults:
+        p = self.params
+        c = _lib.QmcDdvmcParams(p.n_walkers, p.n_steps, p.n_burnin, p.time_step, p.max_drift, p.target_acceptance,
+                                p.adaptation_interval, p.refresh_interval)
+        r = _lib.QmcDdvmcResults()
+        self.energies = np.zeros(p.n_steps)
+        check(lib().qmc_ddvmc_run(self.wavefunction._h, C.byref(c), self.seed, C.byref(r), _ptr(self.energies)))
+        return DDVMCResults(r.energy, r.error, r.autocorrelation_time, r.acceptance_rate, r.final_time_step)
 
 
 _BATCH_MODES = {"fused": 0, "streams": 1, "sequential": 2}

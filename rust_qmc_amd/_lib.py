@@ -30,7 +30,7 @@ QMC_FLAGS_REFERENCE = 0xF
 QMC_AO_GTO, QMC_AO_STO, QMC_AO_SLATER1S = 0, 1, 2
 QMC_JAS_NONE, QMC_JAS_PAIR, QMC_JAS_PADE = 0, 1, 2
 QMC_BASIS_631G, QMC_BASIS_STO3G = 0, 1
-QMC_MOVE_SINGLE, QMC_MOVE_ALL = 0, 1
+QMC_MOVE_SINGLE, QMC_MOVE_ALL, QMC_MOVE_DRIFT = 0, 1, 2
 QMC_BATCH_FUSED, QMC_BATCH_STREAMS, QMC_BATCH_SEQUENTIAL = 0, 1, 2
 QMC_BATCH_ORDER_COST, QMC_BATCH_ORDER_GIVEN = 0, 1
 
@@ -46,6 +46,7 @@ EXPORTED = [
     "qmc_ensemble_timing", "qmc_vmc_run", "qmc_derivatives", "qmc_log_derivs", "qmc_dEL_dp", "qmc_hf_force",
     "qmc_nuc_grad_log", "qmc_sr_solve", "qmc_ensemble_measure", "qmc_ensemble_moments", "qmc_sr_optimize",
     "qmc_sample_forces",
+    "qmc_ddvmc_run",
     "qmc_batch_create", "qmc_batch_destroy", "qmc_batch_fusable", "qmc_batch_set_config", "qmc_batch_get_config",
     "qmc_batch_sweeps", "qmc_batch_timing", "qmc_batch_tune",
     "qmc_autocorrelation_time", "qmc_blocking_error", "qmc_last_error", "qmc_version",
@@ -77,7 +78,7 @@ class QmcSystemDesc(C.Structure):
 
 class QmcSweepParams(C.Structure):
     _fields_ = [("move", _i32), ("n_steps", _i32), ("step_size", _d), ("refresh_interval", _i32),
-                ("steps_per_launch", _i32)]
+                ("steps_per_launch", _i32), ("max_drift", _d)]
 
 
 class QmcMcmcParams(C.Structure):
@@ -109,6 +110,16 @@ class QmcForceParams(C.Structure):
 
 class QmcBatchConfig(C.Structure):
     _fields_ = [("mode", _i32), ("blocks_per_cu", _i32), ("steps_per_launch", _i32), ("order", _i32)]
+
+
+class QmcDdvmcParams(C.Structure):
+    _fields_ = [("n_walkers", _i64), ("n_steps", _i64), ("n_burnin", _i64), ("time_step", _d), ("max_drift", _d),
+                ("target_acceptance", _d), ("adaptation_interval", _i64), ("refresh_interval", _i32)]
+
+
+class QmcDdvmcResults(C.Structure):
+    _fields_ = [("energy", _d), ("error", _d), ("autocorrelation_time", _d), ("acceptance_rate", _d),
+                ("final_time_step", _d)]
 
 
 class QmcVmcResults(C.Structure):
@@ -162,6 +173,7 @@ _SIGS = {
     "qmc_ensemble_moments": (_i32, [_p, _p]),
     "qmc_sr_optimize": (_i32, [_p, _p, _u64, _p, _p, _p, _p]),
     "qmc_sample_forces": (_i32, [_p, _p, _u64, _p, _p, _p, _p]),
+    "qmc_ddvmc_run": (_i32, [_p, _p, _u64, _p, _p]),
     "qmc_batch_create": (_i32, [_p, _i32, _pp]),
     "qmc_batch_destroy": (_i32, [_p]),
     "qmc_batch_fusable": (_i32, [_p, _p]),

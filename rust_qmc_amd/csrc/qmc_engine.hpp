@@ -391,4 +391,184 @@ __device__ __forceinline__ void sweep_block(const DevSys<T>& S, EnsView<T> E, Sw
 }
 
 
+// Jastrow value of the slot-0 electron of sector A placed at (x, y, z)
+// (its pairs with the other electrons and the nuclei) and the gradient of
+// that sum with respect to its position.
+template <int NA, int NB, class T, class Bs, class Js>
+__device__ __forceinline__ void slot0_jastrow(const DevSys<T>& S, T x, T y, T z, const T ra[Cap<NA>::v][3],
+                                              const T rb[Cap<NB>::v][3], T& u, T& gx, T& gy, T& gz) {
+  u = gx = gy = gz = T(0);
+  auto ee = [&](T px, T py, T pz, bool same) {
+    const T dx = x - px, dy = y - py, dz = z - pz;
+    const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+    const T ir = M<T>::rcp(r);
+    T u1, lp;
+    uee_d<T, Js>(S, r, ir, same, u1, lp);
+    u += uee_val<T, Js>(S, r, same);
+    const T f = r >= T(1e-10) ? u1 * ir : T(0);
+    gx += f * dx;
+    gy += f * dy;
+    gz += f * dz;
+  };
+#pragma unroll
+  for (int j = 1; j < NA; ++j) ee(ra[j][0], ra[j][1], ra[j][2], true);
+#pragma unroll
+  for (int j = 0; j < NB; ++j) ee(rb[j][0], rb[j][1], rb[j][2], false);
+  if (S.en_on)
+    Bs::nuclei(S, [&](int I) {
+      const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+      const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+      const T ir = M<T>::rcp(r);
+      T u1, lp;
+      uen_d<T, Js>(S, r, ir, I, u1, lp);
+      u += uen_val<T, Js>(S, r, I);
+      const T f = r >= T(1e-10) ? u1 * ir : T(0);
+      gx += f * dx;
+      gy += f * dy;
+      gz += f * dz;
+    });
+}
+
+// tau F capped at max_drift in norm (DriftDiffusionVMC::limit_drift, vmc.rs:313-321)
+template <class T>
+__device__ __forceinline__ void cap_drift(T fx, T fy, T fz, T tau, T max_drift, T d[3]) {
+  d[0] = tau * fx;
+  d[1] = tau * fy;
+  d[2] = tau * fz;
+  const T m = M<T>::sq(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const T s = m > max_drift ? max_drift / m : T(1);
+  d[0] *= s;
+  d[1] *= s;
+  d[2] *= s;
+}
+
+// One sector's drift-diffusion single-electron moves (DriftDiffusionVMC::
+// sweep, vmc.rs:382-438): Langevin proposal r' = r + cap(tau F(r)) +
+// sqrt(2 tau) z with the quantum force F = 2 grad ln|psi| of the moved
+// electron, acceptance R^2 e^{2 du} e^{G_b - G_f} with the Green's function
+// exponents of vmc.rs:323-346.  grad ln|D| at both positions comes from the
+// AO gradients weighted by w_ao = sum_m C[m][ao] Ainv[m][0] (at r' divided
+// by R, the new inverse column being the old one over R); the Jastrow part
+// carries the Jastrow2 gradient factor of the flags.  Sherman-Morrison on
+// acceptance as in sector_moves.
+template <int NA, int NB, class T, class Bs = RtBasis, class Js = JsRt>
+__device__ __forceinline__ void sector_moves_dd(const DevSys<T>& S, int spin, T ra[Cap<NA>::v][3],
+                                                T ai[Cap<NA>::v][Cap<NA>::v], const T rb[Cap<NB>::v][3],
+                                                uint32_t w, uint32_t step, uint32_t k0, uint32_t k1, T tau,
+                                                T max_drift, double& logdet, double& sgn, double& ju, int& acc) {
+  const T s2t = M<T>::sq(T(2) * tau), i4t = T(0.25) / tau;
+  const T gs = S.grad_scale;
+#pragma unroll 1
+  for (int k = 0; k < NA; ++k) {
+    const uint32_t e = (uint32_t)S.sector_elec[spin][k];
+    auto wgt = [&](int ao) {
+      T v = 0;
+#pragma unroll
+      for (int m = 0; m < NA; ++m) v += S.moT[spin][ao][m] * ai[m][0];
+      return v;
+    };
+    const T ox = ra[0][0], oy = ra[0][1], oz = ra[0][2];
+    // quantum force at the current position
+    T gdx, gdy, gdz, ld, uo, gux, guy, guz;
+    det_terms_w<T, decltype(wgt), NoShell, Bs>(S, ox, oy, oz, wgt, gdx, gdy, gdz, ld, NoShell{});
+    slot0_jastrow<NA, NB, T, Bs, Js>(S, ox, oy, oz, ra, rb, uo, gux, guy, guz);
+    T dold[3];
+    cap_drift(T(2) * (gdx + gs * gux), T(2) * (gdy + gs * guy), T(2) * (gdz + gs * guz), tau, max_drift, dold);
+    // Langevin proposal
+    T z[3];
+    Rng<T>::normals3(philox(w, step, 2u * e, RNG_DRIFT, k0, k1), z);
+    const T U = Rng<T>::uniform(philox(w, step, 2u * e + 1u, RNG_DRIFT, k0, k1).x);
+    const T nx = ox + dold[0] + s2t * z[0], ny = oy + dold[1] + s2t * z[1], nz = oz + dold[2] + s2t * z[2];
+    T phi[Cap<NA>::v];
+    mo_values<NA, T, Bs>(S, spin, nx, ny, nz, phi);
+    T R = 0;
+#pragma unroll
+    for (int m = 0; m < NA; ++m) R += phi[m] * ai[m][0];
+    T hx, hy, hz, hl, un, vx, vy, vz;
+    det_terms_w<T, decltype(wgt), NoShell, Bs>(S, nx, ny, nz, wgt, hx, hy, hz, hl, NoShell{});
+    slot0_jastrow<NA, NB, T, Bs, Js>(S, nx, ny, nz, ra, rb, un, vx, vy, vz);
+    const T iR = M<T>::rcp(R);
+    T dnew[3];
+    cap_drift(T(2) * (hx * iR + gs * vx), T(2) * (hy * iR + gs * vy), T(2) * (hz * iR + gs * vz), tau, max_drift,
+              dnew);
+    T gf = 0, gb = 0;
+    {
+      const T a0 = nx - ox - dold[0], a1 = ny - oy - dold[1], a2 = nz - oz - dold[2];
+      const T b0 = ox - nx - dnew[0], b1 = oy - ny - dnew[1], b2 = oz - nz - dnew[2];
+      gf = -(a0 * a0 + a1 * a1 + a2 * a2) * i4t;
+      gb = -(b0 * b0 + b1 * b1 + b2 * b2) * i4t;
+    }
+    const T du = un - uo;
+    const T p = R * R * M<T>::ex(T(2) * du + gb - gf);
+    if (U < p) {
+      T c[Cap<NA>::v];
+#pragma unroll
+      for (int m = 0; m < NA; ++m) c[m] = ai[m][0];
+#pragma unroll
+      for (int j = 1; j < NA; ++j) {
+        T wj = 0;
+#pragma unroll
+        for (int m = 0; m < NA; ++m) wj += phi[m] * ai[m][j];
+        const T f = wj * iR;
+#pragma unroll
+        for (int m = 0; m < NA; ++m) ai[m][j] -= c[m] * f;
+      }
+#pragma unroll
+      for (int m = 0; m < NA; ++m) ai[m][0] = c[m] * iR;
+      ra[0][0] = nx;
+      ra[0][1] = ny;
+      ra[0][2] = nz;
+      logdet += (double)M<T>::lg(R < T(0) ? -R : R);
+      if (R < T(0)) sgn = -sgn;
+      ju += (double)du;
+      ++acc;
+    }
+    rotate<NA>(ra, ai);
+  }
+}
+
+// n_steps drift-diffusion sweeps of block blk, as sweep_block
+template <int NU, int ND, class Bs, class Js, class T>
+__device__ __forceinline__ void dd_sweep_block(const DevSys<T>& S, EnsView<T> E, SweepArgs A,
+                                               double* __restrict__ partial, double* __restrict__ el_trace,
+                                               int64_t blk, int64_t nblk, int lane) {
+  const int64_t w = blk * 64 + lane;
+  const bool live = w < E.W;
+  const int64_t wl = live ? w : E.W - 1;
+  WalkerRegs<NU, ND, T> R;
+  load_positions<NU, ND>(S, E.pos, E.ld, wl, R);
+  load_inverse<NU, ND>(E.ainv, E.ld, wl, R);
+  double logdet = E.logdet[wl], sgn = E.sign[wl], ju = E.ju[wl];
+  double el = 0.0;
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  const uint32_t wid = (uint32_t)(A.walker0 + wl);
+  const T tau = (T)A.step_size, md = (T)A.max_drift;
+  for (int s = 0; s < A.n_steps; ++s) {
+    const uint32_t step = A.step0 + (uint32_t)s;
+    int acc = 0;
+    sector_moves_dd<NU, ND, T, Bs, Js>(S, 0, R.ru, R.au, R.rd, wid, step, k0, k1, tau, md, logdet, sgn, ju, acc);
+    sector_moves_dd<ND, NU, T, Bs, Js>(S, 1, R.rd, R.ad, R.ru, wid, step, k0, k1, tau, md, logdet, sgn, ju, acc);
+    el = local_energy<NU, ND, T, Bs, Js>(S, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
+    if (el_trace && live) el_trace[(int64_t)s * A.trace_ld + w] = el;
+    double e1 = live ? el : 0.0, e2 = live ? el * el : 0.0, a = live ? (double)acc : 0.0;
+    e1 = wave_sum(e1);
+    e2 = wave_sum(e2);
+    a = wave_sum(a);
+    if (lane == 0) {
+      double* p = partial + ((int64_t)s * nblk + blk) * 3;
+      p[0] = e1;
+      p[1] = e2;
+      p[2] = a;
+    }
+  }
+  if (live) {
+    store_positions<NU, ND>(S, E.pos, E.ld, w, R);
+    store_inverse<NU, ND>(E.ainv, E.ld, w, R);
+    E.logdet[w] = logdet;
+    E.sign[w] = sgn;
+    E.ju[w] = ju;
+    E.eloc[w] = el;
+  }
+}
+
 }  // namespace qmc

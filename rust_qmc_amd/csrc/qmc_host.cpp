@@ -1307,15 +1307,20 @@ int qmc_ensemble_refresh(qmc_ensemble* e) {
 int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_sum, double* e2_sum,
                         double* accepted, double* eloc_trace) {
   if (!e || !p || p->n_steps < 0 || !(p->step_size >= 0.0)) return fail(QMC_ERR_INVALID, "bad arguments");
-  if (p->move != QMC_MOVE_SINGLE && p->move != QMC_MOVE_ALL) return fail(QMC_ERR_INVALID, "bad move kind");
+  if (p->move != QMC_MOVE_SINGLE && p->move != QMC_MOVE_ALL && p->move != QMC_MOVE_DRIFT)
+    return fail(QMC_ERR_INVALID, "bad move kind");
+  if (p->move == QMC_MOVE_DRIFT && !(p->step_size > 0.0 && p->max_drift > 0.0))
+    return fail(QMC_ERR_INVALID, "drift-diffusion needs time_step > 0 and max_drift > 0");
+  // single-electron moves keep the inverse by Sherman-Morrison updates
+  const bool sm = p->move != QMC_MOVE_ALL;
   if (p->n_steps == 0) return QMC_OK;
   const qmc_system* s = e->sys;
   int rc;
   if (e->sys_version != s->version && (rc = ens_refresh(e, 1))) return rc;  // parameters or nuclei moved
-  if (p->move == QMC_MOVE_SINGLE && e->inverse_stale && (rc = ens_refresh(e, 0))) return rc;
+  if (sm && e->inverse_stale && (rc = ens_refresh(e, 0))) return rc;
   const bool wave = s->wave();
   if (wave && p->move != QMC_MOVE_SINGLE)
-    return fail(QMC_ERR_UNSUPPORTED, "the wave engine runs single-electron moves only");
+    return fail(QMC_ERR_UNSUPPORTED, "the wave and tile engines run Metropolis single-electron moves only");
   const int nblk = p->move == QMC_MOVE_SINGLE ? (int)sweep_blocks(e) : (int)((e->W + 63) / 64);
   const int refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
   int chunk = p->steps_per_launch > 0 ? p->steps_per_launch : 16;
@@ -1327,12 +1332,12 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
   int64_t nl = 0;
   while (done < p->n_steps) {
     int ns = std::min(chunk, p->n_steps - done);
-    if (p->move == QMC_MOVE_SINGLE && refresh > 0) {
+    if (sm && refresh > 0) {
       // an earlier call may have run past this call's (shorter) interval
       if (e->since_refresh >= refresh && (rc = ens_refresh(e, 0))) return rc;
       ns = std::min(ns, refresh - e->since_refresh);
     }
-    SweepArgs A{ns, e->step, e->seed, 0, p->step_size, e->W};
+    SweepArgs A{ns, e->step, e->seed, 0, p->step_size, e->W, p->max_drift};
     double* trace = tr.dev ? (double*)tr.dev + (size_t)done * e->W : nullptr;
     if (e->timing) {
       while ((int64_t)e->ev.size() < 2 * (nl + 1)) {
@@ -1344,6 +1349,11 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
     }
     if (p->move == QMC_MOVE_SINGLE) {
       if ((rc = enqueue_single_sweeps(e, A, trace, e->st))) return rc;
+    } else if (p->move == QMC_MOVE_DRIFT) {
+      if (e->prec == QMC_FP64)
+        HIP_TRY(s->ops->dd64(s->d64, e->view<double>(), A, e->partial, trace, e->st));
+      else
+        HIP_TRY(s->ops->dd32(s->d32, e->view<float>(), A, e->partial, trace, e->st));
     } else {
       if (e->prec == QMC_FP64)
         HIP_TRY(s->ops->ae64(s->d64, s->d64, e->view<double>(), A, e->partial, trace, e->st));
@@ -1356,7 +1366,7 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
     HIP_TRY(launch_reduce(e->partial, ns, nblk, 3, e->sums + (size_t)done * 3, e->st));
     e->step += (uint32_t)ns;
     done += ns;
-    if (p->move == QMC_MOVE_SINGLE) {
+    if (sm) {
       e->since_refresh += ns;
       if (refresh > 0 && e->since_refresh >= refresh && (rc = ens_refresh(e, 0))) return rc;
     }
@@ -1663,6 +1673,69 @@ int qmc_vmc_run(const qmc_system* sys, const qmc_mcmc_params* p, uint64_t seed, 
   res->error = qmc_blocking_error(e0, n, res->autocorrelation_time);
   res->acceptance = acc_total / (per_step_moves * (double)p->n_steps);
   res->final_step_size = step;
+  if (energy_trace) std::copy(energies.begin(), energies.end(), energy_trace);
+  return QMC_OK;
+}
+
+int qmc_ddvmc_run(const qmc_system* sys, const qmc_ddvmc_params* p, uint64_t seed, qmc_ddvmc_results* res,
+                  double* energy_trace) {
+  if (!sys || !p || !res || p->n_walkers < 1 || p->n_steps < 1 || p->n_burnin < 0 || p->adaptation_interval < 1 ||
+      !(p->time_step > 0.0) || !(p->max_drift > 0.0) || !(p->target_acceptance > 0.0))
+    return fail(QMC_ERR_INVALID, "bad arguments");
+  qmc_ensemble* e = nullptr;
+  int rc = qmc_ensemble_create(sys, p->n_walkers, seed, &e);
+  if (rc) return rc;
+  std::unique_ptr<qmc_ensemble> guard(e);
+  const double moves_per_sweep = (double)p->n_walkers * sys->h.n_elec();
+  double tau = p->time_step;
+  auto adapt = [&](double rate) {  // DriftDiffusionVMC::adapt_time_step, vmc.rs:441-446
+    const double adj = std::min(std::max(std::sqrt(rate / p->target_acceptance), 0.8), 1.2);
+    tau = std::min(std::max(tau * adj, 0.001), 0.05);
+  };
+  const int64_t I = p->adaptation_interval;
+  std::vector<double> es(I), acc(I), energies(p->n_steps);
+  auto run = [&](int ns) {
+    qmc_sweep_params sp{QMC_MOVE_DRIFT, ns, tau, p->refresh_interval, 0, p->max_drift};
+    return qmc_ensemble_sweeps(e, &sp, es.data(), nullptr, acc.data(), nullptr);
+  };
+  // burn-in with adaptation every I sweeps (vmc.rs:462-476)
+  double acc_b = 0.0, moves_b = 0.0;
+  for (int64_t st = 0; st < p->n_burnin;) {
+    const int ns = (int)std::min(I - st % I, p->n_burnin - st);
+    if ((rc = run(ns))) return rc;
+    for (int i = 0; i < ns; ++i) acc_b += acc[i];
+    moves_b += ns * moves_per_sweep;
+    st += ns;
+    if (st % I == 0 && moves_b > 0.0) {
+      adapt(acc_b / moves_b);
+      acc_b = moves_b = 0.0;
+    }
+  }
+  // production: mean E_L of every sweep, windowed adaptation (vmc.rs:478-507)
+  double acc_p = 0.0, acc_w = 0.0, moves_w = 0.0;
+  for (int64_t st = 0; st < p->n_steps;) {
+    const int ns = (int)std::min(I - st % I, p->n_steps - st);
+    if ((rc = run(ns))) return rc;
+    for (int i = 0; i < ns; ++i) {
+      energies[st + i] = es[i] / (double)p->n_walkers;
+      acc_p += acc[i];
+      acc_w += acc[i];
+    }
+    moves_w += ns * moves_per_sweep;
+    st += ns;
+    if (st % I == 0 && moves_w > 0.0) {
+      adapt(acc_w / moves_w);
+      acc_w = moves_w = 0.0;
+    }
+  }
+  const int64_t n = p->n_steps;
+  double mean = 0.0;
+  for (double v : energies) mean += v;
+  res->energy = mean / (double)n;
+  res->autocorrelation_time = qmc_autocorrelation_time(energies.data(), n);
+  res->error = qmc_blocking_error(energies.data(), n, res->autocorrelation_time);
+  res->acceptance_rate = acc_p / (moves_per_sweep * (double)n);
+  res->final_time_step = tau;
   if (energy_trace) std::copy(energies.begin(), energies.end(), energy_trace);
   return QMC_OK;
 }

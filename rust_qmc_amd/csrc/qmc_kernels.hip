@@ -62,6 +62,14 @@ __global__ void __launch_bounds__(64) sweep_kernel(const DevSys<T>* __restrict__
   sweep_block<NU, ND, Bs, Js, T>(*Sp, E, A, partial, el_trace, blockIdx.x, gridDim.x, threadIdx.x);
 }
 
+// drift-diffusion sweeps (dd_sweep_block in qmc_engine.hpp)
+template <int NU, int ND, class Bs, class Js, class T>
+__global__ void __launch_bounds__(64) dd_sweep_kernel(const DevSys<T>* __restrict__ Sp, EnsView<T> E,
+                                                      SweepArgs A, double* __restrict__ partial,
+                                                      double* __restrict__ el_trace) {
+  dd_sweep_block<NU, ND, Bs, Js, T>(*Sp, E, A, partial, el_trace, blockIdx.x, gridDim.x, threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // all-electron Metropolis (MCMCSimulation::metropolis_step, vmc.rs:95-119):
 // every electron moved at once, full recomputation of ln|psi|, E_L
@@ -246,6 +254,11 @@ struct Launch {
     sweep_kernel<NU, ND, Bs, Js, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<T>*)St, E, A, partial, trace);
     return hipGetLastError();
   }
+  static hipError_t dd(const void* St, EnsView<T> E, SweepArgs A, double* partial, double* trace,
+                       hipStream_t st) {
+    dd_sweep_kernel<NU, ND, Bs, JsRt, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<T>*)St, E, A, partial, trace);
+    return hipGetLastError();
+  }
   static hipError_t all_electron(const void* Sd, const void* St, EnsView<T> E, SweepArgs A, double* partial,
                                  double* trace, hipStream_t st) {
     all_electron_kernel<NU, ND, Bs, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<double>*)Sd,
@@ -279,6 +292,8 @@ static EngineOps make_ops(const char* name, bool (*match)(const DevSys<double>&)
   // specialised kernel agrees, and no cause has been found in the source.
   o.ae64 = &Launch<NU, ND, RtBasis, double>::all_electron;
   o.ae32 = &Launch<NU, ND, RtBasis, float>::all_electron;
+  o.dd64 = &Launch<NU, ND, Bs, double>::dd;
+  o.dd32 = &Launch<NU, ND, Bs, float>::dd;
   o.eval64 = &Launch<NU, ND, Bs, double>::eval;
   o.eval32 = &Launch<NU, ND, Bs, float>::eval;
   return o;

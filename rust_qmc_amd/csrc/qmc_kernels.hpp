@@ -25,8 +25,9 @@ struct SweepArgs {
   uint32_t step0;
   uint64_t seed;
   int64_t walker0;  // Philox walker index of lane 0
-  double step_size;
+  double step_size;  // Gaussian width (Metropolis) or time step tau (drift-diffusion)
   int64_t trace_ld;  // walker stride of the optional E_L trace
+  double max_drift;  // drift-diffusion: cap of |tau F| per electron
 };
 
 // Batched evaluation outputs; any pointer may be null.
@@ -52,6 +53,8 @@ struct EngineOps {
   // the same sweeps compiled for a Jastrow without Pade series terms
   hipError_t (*sweep64_ns)(const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*sweep32_ns)(const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
+  hipError_t (*dd64)(const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
+  hipError_t (*dd32)(const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*ae64)(const void*, const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*ae32)(const void*, const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*eval64)(const void*, const void*, int64_t, const double*, EvalOut, int*, hipStream_t);

@@ -1,0 +1,100 @@
+"""Drift-diffusion VMC (DriftDiffusionVMC, vmc.rs:195-535) on the GPU against
+the oracle's literal restatement (full recomputation of psi and of every
+drift at each trial configuration)."""
+import math
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import _close, _pair
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def Q():
+    import rust_qmc_amd as Q
+    Q.lib()
+    return Q
+
+
+@pytest.mark.parametrize("name", ["h2", "li_ref", "li_seeded", "ch4_sto3g", "ch4_631g"])
+def test_dd_trajectory_parity_fp64(Q, oracle, name):
+    """Same seed -> the Langevin single-electron moves (Sherman-Morrison
+    ratio, drift from the inverse column) follow the oracle's chain step for
+    step: identical decisions, positions and E_L."""
+    g, o = _pair(Q, oracle, name)
+    W, steps, seed, tau = 65, 4, 21, 0.05
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    es, e2, acc, tr = ens.sweeps(steps, tau, Q.QMC_MOVE_DRIFT, trace=True, refresh_interval=2, steps_per_launch=3,
+                                 max_drift=1.0)
+    pos_o, el_o, acc_o = o.dd_steps(p0, seed, 0, steps, tau, 1.0)
+    assert acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=1e-9)
+    assert _close(tr, el_o, 1e-9)
+
+
+def test_dd_large_step_and_drift_cap(Q, oracle):
+    """A time step large enough that tau F often exceeds max_drift (0.3) on
+    CH4: the capped drift of vmc.rs:313-321 in both Green's functions."""
+    g, o = _pair(Q, oracle, "ch4_sto3g")
+    W, steps, seed, tau, cap = 40, 3, 4, 0.4, 0.3
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    es, e2, acc, tr = ens.sweeps(steps, tau, Q.QMC_MOVE_DRIFT, trace=True, refresh_interval=0, max_drift=cap)
+    pos_o, el_o, acc_o = o.dd_steps(p0, seed, 0, steps, tau, cap)
+    assert acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=1e-9)
+    assert _close(tr, el_o, 1e-9)
+
+
+def test_dd_generic_kernels_match(Q, oracle):
+    g, o = _pair(Q, oracle, "ch4_631g")
+    out = []
+    for eng in ("auto", "generic"):
+        g.set_engine(eng)
+        ens = Q.Ensemble(g, 70, 2)
+        es, e2, acc = ens.sweeps(3, 0.03, Q.QMC_MOVE_DRIFT, refresh_interval=2, max_drift=1.0)
+        out.append((es, acc, ens.positions()))
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.allclose(out[0][0], out[1][0], rtol=1e-12, atol=0)
+    assert np.allclose(out[0][2], out[1][2], rtol=0, atol=1e-12)
+
+
+def test_dd_mixed_agrees_with_fp64(Q):
+    res = {}
+    for prec in ("fp64", "mixed"):
+        s = Q.MethaneGTO(1.5, 2.0, 2.06, "STO-3G", 0, precision=prec)
+        ens = Q.Ensemble(s, 4096, 3)
+        ens.sweeps(40, 0.03, Q.QMC_MOVE_DRIFT, refresh_interval=16, max_drift=1.0)
+        es, e2, acc = ens.sweeps(200, 0.03, Q.QMC_MOVE_DRIFT, refresh_interval=16, max_drift=1.0)
+        tr = es / 4096
+        assert np.isfinite(tr).all()
+        tau = Q.autocorrelation_time(tr)
+        res[prec] = (tr.mean(), max(Q.blocking_error(tr, tau), tr.std() / math.sqrt(len(tr))))
+    (a, sa), (b, sb) = res["fp64"], res["mixed"]
+    assert abs(a - b) < 3 * math.hypot(sa, sb) + 1e-4
+
+
+def test_ddvmc_run_matches_metropolis_vmc(Q):
+    """DriftDiffusionVMC::run and MCMCSimulation::run sample the same
+    |psi|^2: H2 energies agree; the adapted time step stays in the
+    reference's clamp [0.001, 0.05]."""
+    h2 = Q.H2MoleculeMO()
+    dd = Q.DriftDiffusionVMC(h2, Q.DDVMCParams(n_walkers=2048, n_steps=1500, n_burnin=300, time_step=0.02,
+                                               adaptation_interval=100), seed=5)
+    r = dd.run()
+    assert 0.001 <= r.final_time_step <= 0.05
+    assert 0.2 < r.acceptance_rate <= 1.0
+    vmc = Q.MCMCSimulation(h2, Q.MCMCParams(n_walkers=2048, n_steps=1800, initial_step_size=0.5, max_step_size=0.5,
+                                            min_step_size=0.5, n_burnin=300, move=Q.QMC_MOVE_SINGLE), seed=6)
+    rv = vmc.run()
+    assert abs(r.energy - rv.energy) < 3 * math.hypot(r.error, rv.error) + 2e-4
+
+
+def test_dd_not_available_on_the_wave_engine(Q):
+    s = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05)
+    ens = Q.Ensemble(s, 2, 0)
+    with pytest.raises(Q.QmcError):
+        ens.sweeps(1, 0.01, Q.QMC_MOVE_DRIFT)

@@ -509,3 +509,47 @@ def test_single_and_all_electron_samplers_agree(oracle):
     tau2 = oracle.autocorrelation_time(tr2)
     e2, s2 = tr2.mean(), oracle.blocking_error(tr2, tau2)
     assert abs(e1 - e2) < 2.5 * math.hypot(s1, s2)
+
+
+def _dd_energy(oracle, sys_, W, nsteps, tau, burn, seed, no_green=False):
+    pos = sys_.initialize_batch(seed, W)
+    pos, el, acc = sys_.dd_steps(pos, seed, 0, nsteps, tau, 1.0, no_green=no_green)
+    tr = el[burn:].mean(1)
+    t = oracle.autocorrelation_time(tr)
+    return tr.mean(), max(oracle.blocking_error(tr, t), tr.std() / math.sqrt(len(tr))), acc / (W * 2 * nsteps)
+
+
+def test_drift_diffusion_samples_psi_squared(oracle):
+    """DriftDiffusionVMC (vmc.rs:382-438) and the symmetric Metropolis sweep
+    both sample |psi|^2, so their H2 energies agree, also at a time step
+    (0.3) far above the reference's adaptive range where the Green's
+    function ratio carries a large correction; without that ratio the
+    Langevin sampler is measurably biased (the check can fail)."""
+    h2 = oracle.System.h2mo()
+    W = 256
+    pos = h2.initialize_batch(1, W)
+    pos, lp, el, acc = h2.single_electron_steps(pos, 1, 0, 1500, 0.7)
+    tr = el[200:].mean(1)
+    t = oracle.autocorrelation_time(tr)
+    e_m, s_m = tr.mean(), oracle.blocking_error(tr, t)
+    e_dd, s_dd, a_dd = _dd_energy(oracle, h2, W, 1500, 0.3, 200, 2)
+    assert 0.3 < a_dd < 0.99
+    assert abs(e_dd - e_m) < 3.0 * math.hypot(s_dd, s_m)
+    e_bad, s_bad, _ = _dd_energy(oracle, h2, W, 1500, 0.3, 200, 2, no_green=True)
+    assert abs(e_bad - e_m) > 5.0 * math.hypot(s_bad, s_m)
+
+
+def test_drift_diffusion_small_time_step_accepts_almost_everything(oracle):
+    # the Langevin proposal becomes exact as tau -> 0 (rejections O(tau^3/2))
+    h2 = oracle.System.h2mo()
+    _, _, a = _dd_energy(oracle, h2, 32, 40, 1e-4, 0, 5)
+    assert a > 0.995
+
+
+def test_drift_diffusion_zero_variance_hydrogenic(oracle):
+    # one electron in STO{c=1,p=0,zeta=3} about Z=3: E_L = -4.5 after every move
+    s = oracle.System.sto_system([([1.0], [0], [3.0])], [1], F=1.0, Z=3)
+    pos = s.initialize_batch(0, 8)
+    pos, el, acc = s.dd_steps(pos, 0, 0, 20, 0.05, 1.0)
+    assert acc > 0
+    assert np.allclose(el, -4.5, atol=1e-12)
