@@ -39,17 +39,18 @@ struct Row {
 
 // Pass 1 for the electron in slot 0 of sector A (combined index q):
 // determinant and Jastrow gradients, E_L terms, Omega, HF force.
-template <int NA, int NB, class T>
+template <int NA, int NB, class T, class Bs>
 __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, int q, const T ra[Cap<NA>::v][3],
                                                 const T ai[Cap<NA>::v][Cap<NA>::v], const T rb[Cap<NB>::v][3],
                                                 const DerivLayout& L, Row row, double& pot) {
   const T x = ra[0][0], y = ra[0][1], z = ra[0][2];
   T gdx, gdy, gdz, ld;
-  det_terms<NA>(S, spin, x, y, z, ai, gdx, gdy, gdz, ld, [&](int c, T sx, T sy, T sz) {
+  auto on_shell = [&](int c, T sx, T sy, T sz) {
     row[L.om(c, 0)] -= (double)sx;
     row[L.om(c, 1)] -= (double)sy;
     row[L.om(c, 2)] -= (double)sz;
-  });
+  };
+  det_terms<NA, T, decltype(on_shell), Bs>(S, spin, x, y, z, ai, gdx, gdy, gdz, ld, on_shell);
   T gux = 0, guy = 0, guz = 0, lu = 0, pairsq = 0;
   double pe = 0.0;
   auto ee = [&](T px, T py, T pz, bool same) {
@@ -72,7 +73,7 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
   for (int j = 1; j < NA; ++j) ee(ra[j][0], ra[j][1], ra[j][2], true);
 #pragma unroll
   for (int j = 0; j < NB; ++j) ee(rb[j][0], rb[j][1], rb[j][2], false);
-  for (int I = 0; I < S.n_nuc; ++I) {
+  Bs::nuclei(S, [&](int I) {
     T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
     T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
     T ir = M<T>::rcp(r);
@@ -98,7 +99,7 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
         row[L.om(I, 2)] -= (double)(f * dz);
       }
     }
-  }
+  });
   const T gs = S.grad_scale;
   T gjx = gs * gux, gjy = gs * guy, gjz = gs * guz;
   T lj = (S.flags & F_J2_LAP_QUIRK) ? (lu + pairsq) : (lu + gux * gux + guy * guy + guz * guz);
@@ -111,7 +112,7 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
 }
 
 // Pass 2: unique pairs -> O_k and d E_L / d p_k (Pade Jastrow only)
-template <int NU, int ND, class T>
+template <int NU, int ND, class T, class Bs>
 __device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>::v][3],
                                           const T rd[Cap<ND>::v][3], const DerivLayout& L, Row row) {
   constexpr int N = NU + ND;
@@ -154,7 +155,7 @@ __device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>
   const int Ke = S.k_en;
 #pragma unroll
   for (int i = 0; i < N; ++i)
-    for (int I = 0; I < S.n_nuc; ++I) {
+    Bs::nuclei(S, [&](int I) {
       T dx = pos(i, 0) - S.nuc[I][0], dy = pos(i, 1) - S.nuc[I][1], dz = pos(i, 2) - S.nuc[I][2];
       T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
       const int sp = S.species[I];
@@ -170,7 +171,7 @@ __device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>
           row[L.o(c0 + k)] += (double)(xk * pd.x * pd.x);
           xk *= pd.x;
         }
-        continue;
+        return;
       }
       const T ir = T(1) / r;
       const T dlt = (T)(row[L.g(i, 0)] * (double)(dx * ir) + row[L.g(i, 1)] * (double)(dy * ir) +
@@ -185,10 +186,10 @@ __device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>
         row[L.de(c0 + k)] += -(double)(T(0.5) * dl + dlt * du1);
         xk *= pd.x;
       }
-    }
+    });
 }
 
-template <int NU, int ND, class T>
+template <int NU, int ND, class Bs, class T>
 __global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __restrict__ Sd,
                                                     const DevSys<T>* __restrict__ St, DerivArgs A) {
   extern __shared__ double lds[];
@@ -220,8 +221,8 @@ __global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __rest
                              : (A.pos64 ? A.pos64[(e * 3 + c) * A.ld + wl] : (double)A.pos32[(e * 3 + c) * A.ld + wl]);
     }
   double lu, su, ldn, sd;
-  sector_inverse<NU>(*Sd, 0, D.ru, D.au, lu, su);
-  sector_inverse<ND>(*Sd, 1, D.rd, D.ad, ldn, sd);
+  sector_inverse<NU, double, Bs>(*Sd, 0, D.ru, D.au, lu, su);
+  sector_inverse<ND, double, Bs>(*Sd, 1, D.rd, D.ad, ldn, sd);
   WalkerRegs<NU, ND, T> R;
 #pragma unroll
   for (int k = 0; k < Cap<NU>::v; ++k) {
@@ -240,17 +241,17 @@ __global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __rest
   double kin = 0.0, pot = 0.0;
 #pragma unroll 1
   for (int k = 0; k < NU; ++k) {
-    kin += electron_pass<NU, ND>(S, 0, k, R.ru, R.au, R.rd, L, row, pot);
+    kin += electron_pass<NU, ND, T, Bs>(S, 0, k, R.ru, R.au, R.rd, L, row, pot);
     rotate<NU>(R.ru, R.au);
   }
 #pragma unroll 1
   for (int k = 0; k < ND; ++k) {
-    kin += electron_pass<ND, NU>(S, 1, NU + k, R.rd, R.ad, R.ru, L, row, pot);
+    kin += electron_pass<ND, NU, T, Bs>(S, 1, NU + k, R.rd, R.ad, R.ru, L, row, pot);
     rotate<ND>(R.rd, R.ad);
   }
   row[L.e()] = kin + pot + S.vnn;
   row[L.one()] = 1.0;
-  if (S.jas_kind == JAS_PADE) pair_pass<NU, ND>(S, R.ru, R.rd, L, row);
+  if (S.jas_kind == JAS_PADE) pair_pass<NU, ND, T, Bs>(S, R.ru, R.rd, L, row);
   if (live)
     for (int q = 0; q < A.Qp; ++q) A.Y[(int64_t)q * A.ldy + w] = row[q];
 }
@@ -314,40 +315,53 @@ __global__ void __launch_bounds__(256) gram_reduce_kernel(const double* __restri
   G[i] = s;
 }
 
-template <int NU, int ND, class T>
+template <int NU, int ND, class Bs, class T>
 static hipError_t launch_derivs(const void* Sd, const void* St, DerivArgs A, hipStream_t st) {
   const size_t shm = sizeof(double) * 64 * (A.Qp + 3 * (NU + ND));
   if (shm > 160 * 1024) return hipErrorInvalidValue;
   static bool attr_set = false;
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)derivs_kernel<NU, ND, T>,
+    hipError_t e = hipFuncSetAttribute((const void*)derivs_kernel<NU, ND, Bs, T>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  derivs_kernel<NU, ND, T><<<(unsigned)((A.W + 63) / 64), 64, shm, st>>>((const DevSys<double>*)Sd,
-                                                                          (const DevSys<T>*)St, A);
+  derivs_kernel<NU, ND, Bs, T><<<(unsigned)((A.W + 63) / 64), 64, shm, st>>>((const DevSys<double>*)Sd,
+                                                                              (const DevSys<T>*)St, A);
   return hipGetLastError();
 }
 
-template <int NU, int ND>
-static DerivOps make_dops() {
+template <int NU, int ND, class Bs>
+static DerivOps make_dops(bool (*match)(const DevSys<double>&)) {
   DerivOps o;
   o.nu = NU;
   o.nd = ND;
-  o.derivs64 = &launch_derivs<NU, ND, double>;
-  o.derivs32 = &launch_derivs<NU, ND, float>;
+  o.match = match;
+  o.derivs64 = &launch_derivs<NU, ND, Bs, double>;
+  o.derivs32 = &launch_derivs<NU, ND, Bs, float>;
   return o;
 }
 
+template <class Bd>
+static DerivOps compiled_dops() {
+  return make_dops<Bd::nu, Bd::nd, CtBasis<Bd>>(&basis_matches<Bd, double>);
+}
+
+// shell structures first (as the register engine), then generic shapes
 static const DerivOps kDOps[] = {
-    make_dops<1, 0>(), make_dops<1, 1>(), make_dops<2, 1>(), make_dops<2, 2>(), make_dops<3, 2>(),
-    make_dops<3, 3>(), make_dops<4, 4>(), make_dops<5, 5>(),
+    compiled_dops<BdH2>(),           compiled_dops<BdLiSeeded>(),    compiled_dops<BdLiRef>(),
+    compiled_dops<BdCH4Sto3g>(),     compiled_dops<BdCH4631g>(),     make_dops<1, 0, RtBasis>(nullptr),
+    make_dops<1, 1, RtBasis>(nullptr), make_dops<2, 1, RtBasis>(nullptr), make_dops<2, 2, RtBasis>(nullptr),
+    make_dops<3, 2, RtBasis>(nullptr), make_dops<3, 3, RtBasis>(nullptr), make_dops<4, 4, RtBasis>(nullptr),
+    make_dops<5, 5, RtBasis>(nullptr),
 };
 
-const DerivOps* find_derivs(int nu, int nd) {
-  for (const auto& o : kDOps)
-    if (o.nu == nu && o.nd == nd) return &o;
+const DerivOps* find_derivs(const DevSys<double>& D, bool generic_only) {
+  for (const auto& o : kDOps) {
+    if (o.nu != D.n_up || o.nd != D.n_dn) continue;
+    if (o.match && (generic_only || !o.match(D))) continue;
+    return &o;
+  }
   return nullptr;
 }
 

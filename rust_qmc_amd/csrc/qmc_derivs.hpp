@@ -17,13 +17,19 @@ struct DerivArgs {
   int64_t ldy;
 };
 
+template <class T>
+struct DevSys;
+
 struct DerivOps {
   int nu, nd;
+  bool (*match)(const DevSys<double>&);  // compiled shell structure, or null (generic)
   hipError_t (*derivs64)(const void*, const void*, DerivArgs, hipStream_t);
   hipError_t (*derivs32)(const void*, const void*, DerivArgs, hipStream_t);
 };
 
-const DerivOps* find_derivs(int nu, int nd);
+// derivative kernels for a system: its compiled shell structure when one
+// matches (unless generic_only), else the generic kernels of its shape
+const DerivOps* find_derivs(const DevSys<double>& D, bool generic_only);
 // G[Qp][Qp] (upper triangle valid) = Y Y^T over W walkers; partial holds
 // ceil(W/chunk) * Qp * Qp doubles.
 hipError_t launch_gram(const double* Y, int64_t ldy, int64_t W, int Qp, double* partial, int chunk, double* G,

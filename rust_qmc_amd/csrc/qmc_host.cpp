@@ -102,9 +102,14 @@ struct qmc_system {
   const EngineOps* ops = nullptr;      // register-engine kernels in use
   const EngineOps* ops_spec = nullptr; // compiled shell structure matching the system, if any
   const EngineOps* ops_gen = nullptr;  // generic kernels of the electron-count shape
-  const DerivOps* dops = nullptr;
+  const DerivOps* dops = nullptr;      // derivative kernels in use
+  const DerivOps* dops_spec = nullptr; // specialised to the shell structure (may equal dops_gen)
+  const DerivOps* dops_gen = nullptr;
   int32_t engine = QMC_ENGINE_AUTO;
-  void pick_ops() { ops = (engine == QMC_ENGINE_GENERIC || !ops_spec) ? ops_gen : ops_spec; }
+  void pick_ops() {
+    ops = (engine == QMC_ENGINE_GENERIC || !ops_spec) ? ops_gen : ops_spec;
+    dops = (engine == QMC_ENGINE_GENERIC || !dops_spec) ? dops_gen : dops_spec;
+  }
   bool dirty = true;
   uint64_t version = 0;  // bumped on every upload; ensembles refresh when it moves
   TilePlan tile;  // tile-engine plan (benzene-size sectors, mixed precision)
@@ -302,9 +307,10 @@ int upload(qmc_system* s) {
   s->ops_spec = find_engine(*d64, false);
   if (s->ops_spec && !s->ops_spec->match) s->ops_spec = nullptr;
   s->ops_gen = find_engine(*d64, true);
+  s->dops_spec = find_derivs(*d64, false);
+  s->dops_gen = find_derivs(*d64, true);
   s->pick_ops();
   tile_plan(*d32, s->tile);
-  s->dops = find_derivs(s->h.n_up(), s->h.n_dn());
   s->dirty = false;
   s->version++;
   return QMC_OK;

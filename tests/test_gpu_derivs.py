@@ -208,3 +208,24 @@ def test_sample_forces_bare_and_zero_variance(Q):
     Fz, ez, errz, fvz = Q.GeometryOptimizer(zero_variance=True, **common).sample_forces(g)
     assert eb == pytest.approx(ez, rel=1e-12)  # same seed, same walk
     assert np.isfinite(Fz).all() and np.isfinite(Fb).all()
+
+
+@pytest.mark.parametrize("name", ["ch4_sto3g", "ch4_631g", "h2"])
+def test_compiled_shell_derivatives_match_generic(Q, oracle, name):
+    """Derivative kernels specialised to the shell structure agree with the
+    run-time-loop kernels (O_k, dE_L/dp, Omega, HF force) with the Pade
+    series switched on."""
+    from test_gpu_parity import _configs, _pair
+    g, o = _pair(Q, oracle, name)
+    if name != "h2":
+        g.set_jastrow_series(3, 2)
+        p = g.get_params()
+        p[4:] = np.linspace(-0.05, 0.05, p.size - 4)
+        g.set_params(p)
+    pos = _configs(o, 70)
+    a = g.derivatives_batch(pos)
+    g.set_engine("generic")
+    b = g.derivatives_batch(pos)
+    for k in a:
+        if b[k].size:
+            assert np.allclose(a[k], b[k], rtol=1e-11, atol=1e-11 * (1 + np.abs(b[k]).max())), k
