@@ -37,8 +37,8 @@ METRIC = "walker-moves/s and SR-grad samples/s, mixed vs FP64 on 1 GPU at ≤0.1
 FLOP_PER_WM = 16338
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector FP64 / FP32
 # measured HBM traffic of one mixed-precision sweep launch at 65,536 walkers
-# and 16 sweeps per launch (22,680 KiB read + 23,296 KiB written)
-TRAFFIC_BYTES = (22680 + 23296) * 1024
+# and 16 sweeps per launch (22,532 KiB read + 23,296 KiB written)
+TRAFFIC_BYTES = (22532 + 23296) * 1024
 
 
 def run_mode(Q, precision, W, steps, warmup, spl, refresh):
