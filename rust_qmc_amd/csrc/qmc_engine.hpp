@@ -103,45 +103,169 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
   pot += 0.5 * (double)pee - (double)pen;
 }
 
+// shift a sector's per-electron vector one slot, as rotate() does
+template <int NA, class T>
+__device__ __forceinline__ void rotate_vec(T* v) {
+  if (NA <= 1) return;
+  T t = v[0];
+#pragma unroll
+  for (int k = 0; k + 1 < NA; ++k) v[k] = v[k + 1];
+  v[NA - 1] = t;
+}
+
 // Local energy E_L = -1/2 sum_i lap_i psi / psi + V, V = -sum Z/r + sum 1/r_ij + V_nn
-// (methane.rs:1154-1182).  Sectors are left in their canonical order.
-// When gout/lout are given, per-electron grad ln psi and lap psi / psi are
-// written to gout[(e*3+c)*ld], lout[e*ld] for global electron index e.
+// (methane.rs:1154-1182), summing the Jastrow and potential terms over unique
+// pairs first (each e-e pair once, with compile-time electron indices), which
+// gives every electron its Jastrow gradient and Laplacian terms; the
+// determinant terms then walk each sector with the rotation, the Jastrow
+// gradients rotating along.  Sectors are left in their canonical order.  The
+// per-electron arrays fit the register file in FP32 only.
+template <int NU, int ND, class T, class Bs, class Js>
+__device__ __forceinline__ double local_energy_pairs(const DevSys<T>& S, T ru[Cap<NU>::v][3],
+                                                     T au[Cap<NU>::v][Cap<NU>::v], T rd[Cap<ND>::v][3],
+                                                     T ad[Cap<ND>::v][Cap<ND>::v], double* gout, double* lout,
+                                                     int64_t ld) {
+  constexpr int N = NU + ND;
+  constexpr int NC = N > 0 ? N : 1;
+  T gx[NC], gy[NC], gz[NC], lu[NC];
+#pragma unroll
+  for (int i = 0; i < N; ++i) gx[i] = gy[i] = gz[i] = lu[i] = T(0);
+  // Jastrow2's Laplacian omits the cross terms (jastrow.rs:110-125): the
+  // squared pair derivatives replace |grad u|^2 under the flag
+  const bool quirk = S.flags & F_J2_LAP_QUIRK;
+  const T qw = quirk ? T(1) : T(0);
+  auto px = [&](int i, int c) -> T { return i < NU ? ru[i][c] : rd[i - NU][c]; };
+  // potential sums: FP32 within an electron's row, FP64 across rows
+  double pot = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    T pee = 0;
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) {
+      const T dx = px(i, 0) - px(j, 0), dy = px(i, 1) - px(j, 1), dz = px(i, 2) - px(j, 2);
+      const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+      const T ir = M<T>::rcp(r);
+      T u1, lp;
+      uee_d<T, Js>(S, r, ir, (i < NU) == (j < NU), u1, lp);
+      const bool ok = r >= T(1e-10);
+      const T f = ok ? u1 * ir : T(0);
+      const T fx = f * dx, fy = f * dy, fz = f * dz;
+      gx[i] += fx;
+      gy[i] += fy;
+      gz[i] += fz;
+      gx[j] -= fx;
+      gy[j] -= fy;
+      gz[j] -= fz;
+      const T l = ok ? lp + qw * u1 * u1 : T(0);
+      lu[i] += l;
+      lu[j] += l;
+      pee += ir;
+    }
+    pot += (double)pee;
+  }
+  const bool en_on = S.en_on;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const T x = px(i, 0), y = px(i, 1), z = px(i, 2);
+    T pen = 0;
+    Bs::nuclei(S, [&](int I) {
+      const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+      const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+      const T ir = M<T>::rcp(r);
+      pen += S.Z[I] * ir;
+      if (en_on) {
+        T u1, lp;
+        uen_d<T, Js>(S, r, ir, I, u1, lp);
+        const bool ok = r >= T(1e-10);
+        const T f = ok ? u1 * ir : T(0);
+        gx[i] += f * dx;
+        gy[i] += f * dy;
+        gz[i] += f * dz;
+        lu[i] += ok ? lp : T(0);
+      }
+    });
+    pot -= (double)pen;
+  }
+  // |grad u|^2 completes the Jastrow Laplacian; Jastrow2's gradient carries
+  // an extra factor F (jastrow.rs:102) under the flags
+  const T gs = S.grad_scale;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (!quirk) lu[i] += gx[i] * gx[i] + gy[i] * gy[i] + gz[i] * gz[i];
+    gx[i] *= gs;
+    gy[i] *= gs;
+    gz[i] *= gs;
+  }
+  double kin = 0.0;
+  auto sector = [&](auto na_tag, int spin, auto& ra, auto& ai, T* gxs, T* gys, T* gzs, T* ls) {
+    constexpr int NA = decltype(na_tag)::value;
+#pragma unroll 1
+    for (int k = 0; k < NA; ++k) {
+      T gdx, gdy, gdz, ldd;
+      det_terms<NA, T, NoShell, Bs>(S, spin, ra[0][0], ra[0][1], ra[0][2], ai, gdx, gdy, gdz, ldd, NoShell{});
+      const T lap = ldd + ls[0] + T(2) * (gdx * gxs[0] + gdy * gys[0] + gdz * gzs[0]);
+      kin += -0.5 * (double)lap;
+      if (gout) {
+        const int e = S.sector_elec[spin][k];
+        gout[(e * 3 + 0) * ld] = gdx + gxs[0];
+        gout[(e * 3 + 1) * ld] = gdy + gys[0];
+        gout[(e * 3 + 2) * ld] = gdz + gzs[0];
+        lout[e * ld] = lap;
+      }
+      rotate<NA>(ra, ai);
+      rotate_vec<NA>(gxs);
+      rotate_vec<NA>(gys);
+      rotate_vec<NA>(gzs);
+      rotate_vec<NA>(ls);
+    }
+  };
+  sector(std::integral_constant<int, NU>{}, 0, ru, au, gx, gy, gz, lu);
+  sector(std::integral_constant<int, ND>{}, 1, rd, ad, gx + NU, gy + NU, gz + NU, lu + NU);
+  return kin + pot + S.vnn;
+}
+
+// The same E_L electron by electron (each e-e pair visited from both ends),
+// for FP64 state whose per-electron arrays would not fit the register file.
+template <int NU, int ND, class T, class Bs, class Js>
+__device__ __forceinline__ double local_energy_rows(const DevSys<T>& S, T ru[Cap<NU>::v][3],
+                                                    T au[Cap<NU>::v][Cap<NU>::v], T rd[Cap<ND>::v][3],
+                                                    T ad[Cap<ND>::v][Cap<ND>::v], double* gout, double* lout,
+                                                    int64_t ld) {
+  double kin = 0.0, pot = 0.0;
+  auto sector = [&](auto na_tag, auto nb_tag, int spin, auto& ra, auto& ai, auto& rb) {
+    constexpr int NA = decltype(na_tag)::value, NB = decltype(nb_tag)::value;
+#pragma unroll 1
+    for (int k = 0; k < NA; ++k) {
+      T gx, gy, gz, lap;
+      electron_terms<NA, NB, T, Bs, Js>(S, spin, ra, ai, rb, gx, gy, gz, lap, pot);
+      kin += -0.5 * (double)lap;
+      if (gout) {
+        const int e = S.sector_elec[spin][k];
+        gout[(e * 3 + 0) * ld] = gx;
+        gout[(e * 3 + 1) * ld] = gy;
+        gout[(e * 3 + 2) * ld] = gz;
+        lout[e * ld] = lap;
+      }
+      rotate<NA>(ra, ai);
+    }
+  };
+  sector(std::integral_constant<int, NU>{}, std::integral_constant<int, ND>{}, 0, ru, au, rd);
+  sector(std::integral_constant<int, ND>{}, std::integral_constant<int, NU>{}, 1, rd, ad, ru);
+  return kin + pot + S.vnn;
+}
+
+// Local energy of the register state; when gout/lout are given, per-electron
+// grad ln psi and lap psi / psi are written to gout[(e*3+c)*ld], lout[e*ld]
+// for global electron index e.
 template <int NU, int ND, class T, class Bs = RtBasis, class Js = JsRt>
 __device__ __forceinline__ double local_energy(const DevSys<T>& S, T ru[Cap<NU>::v][3],
                                                T au[Cap<NU>::v][Cap<NU>::v], T rd[Cap<ND>::v][3],
                                                T ad[Cap<ND>::v][Cap<ND>::v], double* gout, double* lout,
                                                int64_t ld) {
-  double kin = 0.0, pot = 0.0;
-#pragma unroll 1
-  for (int k = 0; k < NU; ++k) {
-    T gx, gy, gz, lap;
-    electron_terms<NU, ND, T, Bs, Js>(S, 0, ru, au, rd, gx, gy, gz, lap, pot);
-    kin += -0.5 * (double)lap;
-    if (gout) {
-      const int e = S.sector_elec[0][k];
-      gout[(e * 3 + 0) * ld] = gx;
-      gout[(e * 3 + 1) * ld] = gy;
-      gout[(e * 3 + 2) * ld] = gz;
-      lout[e * ld] = lap;
-    }
-    rotate<NU>(ru, au);
-  }
-#pragma unroll 1
-  for (int k = 0; k < ND; ++k) {
-    T gx, gy, gz, lap;
-    electron_terms<ND, NU, T, Bs, Js>(S, 1, rd, ad, ru, gx, gy, gz, lap, pot);
-    kin += -0.5 * (double)lap;
-    if (gout) {
-      const int e = S.sector_elec[1][k];
-      gout[(e * 3 + 0) * ld] = gx;
-      gout[(e * 3 + 1) * ld] = gy;
-      gout[(e * 3 + 2) * ld] = gz;
-      lout[e * ld] = lap;
-    }
-    rotate<ND>(rd, ad);
-  }
-  return kin + pot + S.vnn;
+  if constexpr (sizeof(T) == 4)
+    return local_energy_pairs<NU, ND, T, Bs, Js>(S, ru, au, rd, ad, gout, lout, ld);
+  else
+    return local_energy_rows<NU, ND, T, Bs, Js>(S, ru, au, rd, ad, gout, lout, ld);
 }
 
 // Full Jastrow exponent ln J (FP64), including the pair constant.
