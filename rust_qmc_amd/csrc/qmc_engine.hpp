@@ -163,29 +163,36 @@ __device__ __forceinline__ double local_energy_pairs(const DevSys<T>& S, T ru[Ca
     }
     pot += (double)pee;
   }
-  const bool en_on = S.en_on;
+  // e-n pairs: one uniform branch on the e-n Jastrow for the whole pass
+  auto en_pass = [&](auto en_tag) {
+    constexpr bool EN = decltype(en_tag)::value;
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const T x = px(i, 0), y = px(i, 1), z = px(i, 2);
-    T pen = 0;
-    Bs::nuclei(S, [&](int I) {
-      const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-      const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-      const T ir = M<T>::rcp(r);
-      pen += S.Z[I] * ir;
-      if (en_on) {
-        T u1, lp;
-        uen_d<T, Js>(S, r, ir, I, u1, lp);
-        const bool ok = r >= T(1e-10);
-        const T f = ok ? u1 * ir : T(0);
-        gx[i] += f * dx;
-        gy[i] += f * dy;
-        gz[i] += f * dz;
-        lu[i] += ok ? lp : T(0);
-      }
-    });
-    pot -= (double)pen;
-  }
+    for (int i = 0; i < N; ++i) {
+      const T x = px(i, 0), y = px(i, 1), z = px(i, 2);
+      T pen = 0;
+      Bs::nuclei(S, [&](int I) {
+        const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+        const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+        const T ir = M<T>::rcp(r);
+        pen += S.Z[I] * ir;
+        if constexpr (EN) {
+          T u1, lp;
+          uen_d<T, Js>(S, r, ir, I, u1, lp);
+          const bool ok = r >= T(1e-10);
+          const T f = ok ? u1 * ir : T(0);
+          gx[i] += f * dx;
+          gy[i] += f * dy;
+          gz[i] += f * dz;
+          lu[i] += ok ? lp : T(0);
+        }
+      });
+      pot -= (double)pen;
+    }
+  };
+  if (S.en_on)
+    en_pass(std::true_type{});
+  else
+    en_pass(std::false_type{});
   // |grad u|^2 completes the Jastrow Laplacian; Jastrow2's gradient carries
   // an extra factor F (jastrow.rs:102) under the flags
   const T gs = S.grad_scale;
