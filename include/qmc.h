@@ -412,6 +412,57 @@ typedef struct {
 int qmc_ddvmc_run(const qmc_system* sys, const qmc_ddvmc_params* p, uint64_t seed, qmc_ddvmc_results* res,
                   double* energy_trace);
 
+/* ISDMCParams (is_dmc.rs:49-84); print_interval has no counterpart (the
+ * library prints nothing).  capacity bounds each population buffer (0: four
+ * times n_walkers, rounded up to 256; at most 1,048,576): clones past it are
+ * dropped and counted, where the reference's Vec grows without bound.
+ * refresh_interval rebuilds the FP64 inverses of a mixed-precision population
+ * every that many steps (0: never; FP64 systems ignore it).
+ * The reference's moves r' = r + tau F + sqrt(2 tau) eta with F = 2 grad psi/psi
+ * diffuse for an imaginary time 2 tau while its branching weight and E_ref
+ * feedback use tau (is_dmc.rs:215-222, 286, 487-489), which biases the energy
+ * (H2 at R = 1.4: -1.1646(17) against the exact -1.1745).  matched_branching = 1
+ * uses 2 tau in the weight and the feedback; 0 reproduces the reference. */
+typedef struct {
+  int64_t n_walkers;
+  int64_t n_steps;
+  int64_t n_burnin;
+  double time_step;
+  double max_drift;
+  double e_ref_damping;
+  int64_t n_vmc_equilibrate;
+  int64_t n_vmc_force_samples;
+  int64_t capacity;
+  int32_t refresh_interval;
+  int32_t matched_branching;
+} qmc_isdmc_params;
+/* ISDMCResults (is_dmc.rs:90-108) plus the population bookkeeping */
+typedef struct {
+  double energy;
+  double error;
+  double acceptance_rate;
+  double avg_population;
+  int64_t collapses;  /* re-initialisations after the population died out (is_dmc.rs:424-433) */
+  int64_t dropped;    /* clones dropped at the capacity */
+  int64_t capacity;   /* walkers per population buffer */
+  double step_ms;     /* device time of the burn-in and production steps */
+} qmc_isdmc_results;
+/* ImportanceSampledDMC::run (is_dmc.rs:376-546) on the register engine:
+ * VMC forces from min(n_walkers, 50) all-electron walkers (step 0.5,
+ * is_dmc.rs:305-370; Philox key seed ^ 0xD1B54A32D192ED03), then n_burnin +
+ * n_steps steps of one drift-diffusion sweep (is_dmc.rs:198-265), branching
+ * with at most 3 copies (is_dmc.rs:272-300) and the E_ref feedback.  Walkers
+ * start from Philox counters (w, 0, e, INIT) under key seed, re-initialised
+ * populations from walker ids collapses * n_walkers + w; the sweep of step t
+ * uses counters (slot, t, ...) and the branching (slot, t, 0, 6).
+ * forces_* are [M*3]; energy_trace and population_trace (host, optional)
+ * receive the [n_steps] production series. */
+int qmc_isdmc_run(const qmc_system* sys, const qmc_isdmc_params* p, uint64_t seed, qmc_isdmc_results* res,
+                  double* forces_mixed, double* forces_vmc, double* forces_extrapolated, double* energy_trace,
+                  double* population_trace);
+/* ImportanceSampledDMC::block_error (is_dmc.rs:555-607) on a host series */
+double qmc_isdmc_block_error(const double* e, int64_t n);
+
 /* vmc.rs:141-163 and vmc.rs:166-188 on a host series */
 double qmc_autocorrelation_time(const double* e, int64_t n);
 double qmc_blocking_error(const double* e, int64_t n, double tau);

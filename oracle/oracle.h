@@ -48,6 +48,7 @@
 #define ORC_RNG_ALL 3u
 #define ORC_RNG_SYS 4u
 #define ORC_RNG_DRIFT 5u
+#define ORC_RNG_BRANCH 6u
 
 /* ---- slater.rs:10-42 ---- */
 typedef struct { double alpha; double center[3]; } orc_slater1s;
@@ -224,6 +225,23 @@ void orc_dd_steps(const orc_system* s, int64_t W, double* pos, uint64_t seed, ui
 void orc_all_electron_steps(const orc_system* s, int64_t W, double* pos, double* logpsi,
                             double* eloc, uint64_t seed, uint32_t step0, int nsteps, double step,
                             int64_t* accepted);
+
+/* ImportanceSampledDMC (is_dmc.rs:176-607), orc_dmc.c */
+typedef struct {
+    int64_t n_walkers, n_steps, n_burnin;
+    double time_step, max_drift, e_ref_damping;
+    int64_t n_vmc_equilibrate, n_vmc_force_samples, capacity;
+    int32_t matched_branching; /* 1: weights and E_ref feedback over 2 tau */
+} orc_isdmc_params;
+typedef struct {
+    double energy, error, acceptance_rate, avg_population;
+    int64_t collapses, dropped;
+} orc_isdmc_results;
+void orc_isdmc_branch(int64_t n, const double* el, const double* el_prev, double e_ref, double tau, uint64_t seed,
+                      uint32_t step, int* copies);
+double orc_isdmc_block_error(const double* e, int64_t n);
+void orc_isdmc_run(const orc_system* s, const orc_isdmc_params* p, uint64_t seed, orc_isdmc_results* res,
+                   double* f_mixed, double* f_vmc, double* f_extrap, double* e_trace, double* pop_trace);
 
 /* SR (sr_optimize.rs:176-246) and ZV forces (geometry_opt.rs:198-226) */
 int orc_sr_update(int64_t ns, int P, const double* E, const double* O, double eps, double* dp,

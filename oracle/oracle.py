@@ -48,6 +48,17 @@ class McmcResults(C.Structure):
                 ("acceptance", _d), ("final_step", _d)]
 
 
+class IsdmcParams(C.Structure):
+    _fields_ = [("n_walkers", _i64), ("n_steps", _i64), ("n_burnin", _i64), ("time_step", _d),
+                ("max_drift", _d), ("e_ref_damping", _d), ("n_vmc_equilibrate", _i64),
+                ("n_vmc_force_samples", _i64), ("capacity", _i64), ("matched_branching", _i)]
+
+
+class IsdmcResults(C.Structure):
+    _fields_ = [("energy", _d), ("error", _d), ("acceptance_rate", _d), ("avg_population", _d),
+                ("collapses", _i64), ("dropped", _i64)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -103,6 +114,9 @@ def lib():
             "orc_slater_matrix": (_i, [_p, _p, _i, _p]),
             "orc_jastrow_u": (_d, [_p, _p]),
             "orc_sto_system_new": (_p, [_i, _p, _p, _p, _p, _p, _d, _i, _i]),
+            "orc_isdmc_branch": (None, [_i64, _p, _p, _d, _d, _u64, _u32, _p]),
+            "orc_isdmc_block_error": (_d, [_p, _i64]),
+            "orc_isdmc_run": (None, [_p, _p, _u64, _p, _p, _p, _p, _p, _p]),
             "orc_j1_eval": (_d, [_p, _p]),
             "orc_j1_deriv": (None, [_p, _p, _p]),
             "orc_j1_lap": (None, [_p, _p, _p]),
@@ -351,6 +365,24 @@ class System:
                            C.byref(acc))
         return pos, el, acc.value
 
+    def isdmc_run(self, seed, n_walkers, n_steps, n_burnin, time_step, max_drift=1.0, e_ref_damping=0.5,
+                  n_vmc_equilibrate=500, n_vmc_force_samples=2000, capacity=0, matched_branching=False):
+        """ImportanceSampledDMC::run (is_dmc.rs:376-546) with the GPU's
+        counters; capacity 0 means 4 x n_walkers rounded up to 256."""
+        if capacity <= 0:
+            capacity = (4 * n_walkers + 255) // 256 * 256
+        p = IsdmcParams(n_walkers, n_steps, n_burnin, time_step, max_drift, e_ref_damping, n_vmc_equilibrate,
+                        n_vmc_force_samples, capacity, int(matched_branching))
+        r = IsdmcResults()
+        M = self.n_nuc
+        fm, fv, fx = np.zeros((M, 3)), np.zeros((M, 3)), np.zeros((M, 3))
+        et, pt = np.zeros(n_steps), np.zeros(n_steps)
+        lib().orc_isdmc_run(self._h, C.byref(p), seed, C.byref(r), _ptr(fm), _ptr(fv), _ptr(fx), _ptr(et),
+                            _ptr(pt))
+        return dict(energy=r.energy, error=r.error, acceptance_rate=r.acceptance_rate,
+                    avg_population=r.avg_population, collapses=r.collapses, dropped=r.dropped,
+                    forces_mixed=fm, forces_vmc=fv, forces_extrapolated=fx, energies=et, populations=pt)
+
     def all_electron_steps(self, pos, logpsi, eloc, seed, step0, nsteps, step):
         pos = _f64(pos).copy()
         lp = _f64(logpsi).copy()
@@ -359,6 +391,19 @@ class System:
         lib().orc_all_electron_steps(self._h, pos.shape[0], _ptr(pos), _ptr(lp), _ptr(el), seed,
                                      step0, nsteps, step, C.byref(acc))
         return pos, lp, el, acc.value
+
+
+def isdmc_branch(el, el_prev, e_ref, tau, seed, step):
+    """Copies per walker of the IS-DMC branching (is_dmc.rs:284-289)."""
+    el, el_prev = _f64(el), _f64(el_prev)
+    out = np.zeros(el.size, dtype=np.int32)
+    lib().orc_isdmc_branch(el.size, _ptr(el), _ptr(el_prev), e_ref, tau, seed, step, _ptr(out))
+    return out
+
+
+def isdmc_block_error(e):
+    e = _f64(e)
+    return lib().orc_isdmc_block_error(_ptr(e), e.size)
 
 
 def autocorrelation_time(e):

@@ -34,6 +34,7 @@ __all__ = [
     "MCMCParams", "MCMCResults", "MCMCSimulation", "autocorrelation_time", "blocking_error", "QMC_FP64",
     "QMC_MIXED", "QMC_MOVE_SINGLE", "QMC_MOVE_ALL", "QMC_MOVE_DRIFT", "QMC_FLAGS_REFERENCE", "Batch",
     "DDVMCParams", "DDVMCResults", "DriftDiffusionVMC",
+    "ISDMCParams", "ISDMCResults", "ImportanceSampledDMC", "isdmc_block_error",
 ]
 
 _PREC = {"fp64": QMC_FP64, "mixed": QMC_MIXED, QMC_FP64: QMC_FP64, QMC_MIXED: QMC_MIXED}
@@ -598,6 +599,76 @@ class DriftDiffusionVMC:
         self.energies = np.zeros(p.n_steps)
         check(lib().qmc_ddvmc_run(self.wavefunction._h, C.byref(c), self.seed, C.byref(r), _ptr(self.energies)))
         return DDVMCResults(r.energy, r.error, r.autocorrelation_time, r.acceptance_rate, r.final_time_step)
+
+
+@dataclass
+class ISDMCParams:
+    """is_dmc.rs:49-84 (defaults as Default for ISDMCParams; print_interval has
+    no counterpart) plus the population capacity (0: 4 x n_walkers), the
+    FP64 inverse refresh interval of a mixed-precision population, and
+    matched_branching: weights and E_ref feedback over the 2 tau the moves
+    diffuse for, instead of the reference's tau (is_dmc.rs:215-222, 286)."""
+    n_walkers: int = 200
+    n_steps: int = 5000
+    n_burnin: int = 1000
+    time_step: float = 0.005
+    max_drift: float = 1.0
+    e_ref_damping: float = 0.5
+    n_vmc_equilibrate: int = 500
+    n_vmc_force_samples: int = 2000
+    capacity: int = 0
+    refresh_interval: int = 64
+    matched_branching: bool = False
+
+
+@dataclass
+class ISDMCResults:
+    """is_dmc.rs:90-108; forces are [M][3] arrays.  collapses, dropped,
+    capacity and step_ms describe the population run."""
+    energy: float
+    error: float
+    forces_mixed: np.ndarray
+    forces_vmc: np.ndarray
+    forces_extrapolated: np.ndarray
+    acceptance_rate: float
+    avg_population: float
+    collapses: int = 0
+    dropped: int = 0
+    capacity: int = 0
+    step_ms: float = 0.0
+
+
+class ImportanceSampledDMC:
+    """is_dmc.rs:176-607 on the GPU (qmc_isdmc_run); register-engine systems.
+    After run(), energies and populations hold the production series."""
+
+    def __init__(self, wfn, params: ISDMCParams, seed: int = 0):
+        self.wfn = wfn
+        self.params = params
+        self.seed = seed
+        self.energies = None
+        self.populations = None
+
+    def run(self) -> ISDMCResults:
+        p = self.params
+        c = _lib.QmcIsdmcParams(p.n_walkers, p.n_steps, p.n_burnin, p.time_step, p.max_drift, p.e_ref_damping,
+                                p.n_vmc_equilibrate, p.n_vmc_force_samples, p.capacity, p.refresh_interval,
+                                int(p.matched_branching))
+        r = _lib.QmcIsdmcResults()
+        M = self.wfn.num_nuclei()
+        fm, fv, fx = np.zeros((M, 3)), np.zeros((M, 3)), np.zeros((M, 3))
+        self.energies = np.zeros(p.n_steps)
+        self.populations = np.zeros(p.n_steps)
+        check(lib().qmc_isdmc_run(self.wfn._h, C.byref(c), self.seed, C.byref(r), _ptr(fm), _ptr(fv), _ptr(fx),
+                                  _ptr(self.energies), _ptr(self.populations)))
+        return ISDMCResults(r.energy, r.error, fm, fv, fx, r.acceptance_rate, r.avg_population, r.collapses,
+                            r.dropped, r.capacity, r.step_ms)
+
+
+def isdmc_block_error(e) -> float:
+    """ImportanceSampledDMC::block_error (is_dmc.rs:555-607)."""
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    return lib().qmc_isdmc_block_error(_ptr(e), e.size)
 
 
 _BATCH_MODES = {"fused": 0, "streams": 1, "sequential": 2}

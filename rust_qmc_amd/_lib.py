@@ -46,7 +46,7 @@ EXPORTED = [
     "qmc_ensemble_timing", "qmc_vmc_run", "qmc_derivatives", "qmc_log_derivs", "qmc_dEL_dp", "qmc_hf_force",
     "qmc_nuc_grad_log", "qmc_sr_solve", "qmc_ensemble_measure", "qmc_ensemble_moments", "qmc_sr_optimize",
     "qmc_sample_forces",
-    "qmc_ddvmc_run",
+    "qmc_ddvmc_run", "qmc_isdmc_run", "qmc_isdmc_block_error",
     "qmc_batch_create", "qmc_batch_destroy", "qmc_batch_fusable", "qmc_batch_set_config", "qmc_batch_get_config",
     "qmc_batch_sweeps", "qmc_batch_timing", "qmc_batch_tune",
     "qmc_autocorrelation_time", "qmc_blocking_error", "qmc_last_error", "qmc_version",
@@ -122,6 +122,17 @@ class QmcDdvmcResults(C.Structure):
                 ("final_time_step", _d)]
 
 
+class QmcIsdmcParams(C.Structure):
+    _fields_ = [("n_walkers", _i64), ("n_steps", _i64), ("n_burnin", _i64), ("time_step", _d), ("max_drift", _d),
+                ("e_ref_damping", _d), ("n_vmc_equilibrate", _i64), ("n_vmc_force_samples", _i64),
+                ("capacity", _i64), ("refresh_interval", _i32), ("matched_branching", _i32)]
+
+
+class QmcIsdmcResults(C.Structure):
+    _fields_ = [("energy", _d), ("error", _d), ("acceptance_rate", _d), ("avg_population", _d),
+                ("collapses", _i64), ("dropped", _i64), ("capacity", _i64), ("step_ms", _d)]
+
+
 class QmcVmcResults(C.Structure):
     _fields_ = [("energy", _d), ("error", _d), ("autocorrelation_time", _d), ("acceptance", _d),
                 ("final_step_size", _d)]
@@ -174,6 +185,8 @@ _SIGS = {
     "qmc_sr_optimize": (_i32, [_p, _p, _u64, _p, _p, _p, _p]),
     "qmc_sample_forces": (_i32, [_p, _p, _u64, _p, _p, _p, _p]),
     "qmc_ddvmc_run": (_i32, [_p, _p, _u64, _p, _p]),
+    "qmc_isdmc_run": (_i32, [_p, _p, _u64, _p, _p, _p, _p, _p, _p]),
+    "qmc_isdmc_block_error": (_d, [_p, _i64]),
     "qmc_batch_create": (_i32, [_p, _i32, _pp]),
     "qmc_batch_destroy": (_i32, [_p]),
     "qmc_batch_fusable": (_i32, [_p, _p]),

@@ -16,7 +16,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libqmc_hip.so")
 
-SOURCES = ["qmc_kernels.hip", "qmc_derivs.hip", "qmc_wave.hip", "qmc_tile.hip", "qmc_batch.hip", "qmc_host.cpp"]
+SOURCES = [
+    "qmc_kernels.hip", "qmc_derivs.hip", "qmc_wave.hip", "qmc_tile.hip", "qmc_batch.hip", "qmc_dmc.hip", "qmc_host.cpp",
+]
 ARCH = os.environ.get("QMC_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     f"--offload-arch={ARCH}",

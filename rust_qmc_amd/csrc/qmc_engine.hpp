@@ -663,9 +663,16 @@ template <int NU, int ND, class Bs, class Js, class T>
 __device__ __forceinline__ void dd_sweep_block(const DevSys<T>& S, EnsView<T> E, SweepArgs A,
                                                double* __restrict__ partial, double* __restrict__ el_trace,
                                                int64_t blk, int64_t nblk, int lane) {
+  const int64_t W = E.n_live ? *E.n_live : E.W;
+  if (blk * 64 >= W) {  // block past the live population: zero partial sums only
+    if (lane == 0)
+      for (int s = 0; s < A.n_steps; ++s)
+        for (int q = 0; q < 3; ++q) partial[((int64_t)s * nblk + blk) * 3 + q] = 0.0;
+    return;
+  }
   const int64_t w = blk * 64 + lane;
-  const bool live = w < E.W;
-  const int64_t wl = live ? w : E.W - 1;
+  const bool live = w < W;
+  const int64_t wl = live ? w : W - 1;
   WalkerRegs<NU, ND, T> R;
   load_positions<NU, ND>(S, E.pos, E.ld, wl, R);
   load_inverse<NU, ND>(E.ainv, E.ld, wl, R);

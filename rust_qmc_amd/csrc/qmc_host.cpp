@@ -17,6 +17,7 @@
 
 #include "qmc_derivs.hpp"
 #include "qmc_device.hpp"
+#include "qmc_dmc.hpp"
 #include "qmc_kernels.hpp"
 #include "qmc_batch.hpp"
 #include "qmc_tile.hpp"
@@ -2050,3 +2051,279 @@ int qmc_batch_tune(qmc_batch* b, int32_t trial_steps, const double* step_size, i
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// importance-sampled DMC (ImportanceSampledDMC, is_dmc.rs:176-607)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr uint64_t ISDMC_VMC_KEY = 0xD1B54A32D192ED03ull;
+
+// device allocations of one run, freed together
+struct DevArena {
+  std::vector<void*> ptrs;
+  ~DevArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <class P>
+  int get(P** p, size_t bytes) {
+    void* q = nullptr;
+    HIP_TRY(hipMalloc(&q, bytes > 0 ? bytes : 1));
+    ptrs.push_back(q);
+    *p = (P*)q;
+    return QMC_OK;
+  }
+};
+
+struct DmcHost {
+  double step_ms = 0.0;
+  int64_t collapses = 0;
+  std::vector<double> e_trace, pop_trace, acc;  // [n_burnin + n_steps]
+  std::vector<char> fresh;                      // population (re)initialised before the step
+  DmcCtl ctl{};
+};
+
+// phases 2-4 of is_dmc.rs:376-546 on a population of capacity cap
+template <class T>
+int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, int64_t cap, DmcHost* out) {
+  const int N = s->h.n_elec(), M = s->h.n_nuc(), nu = s->h.n_up(), nd = s->h.n_dn();
+  const int nainv = nu * nu + nd * nd;
+  const int64_t T_all = p->n_burnin + p->n_steps;
+  const int nb = (int)(cap / DMC_BLOCK), nblk64 = (int)(cap / 64);
+  const bool f64 = sizeof(T) == 8;
+  hipStream_t st = nullptr;
+  DevArena A;
+  int rc;
+  T* pos[2];
+  T* ainv[2];
+  double *logdet[2], *sign[2], *ju[2], *eloc[2], *eprev[2];
+  for (int b = 0; b < 2; ++b) {
+    if ((rc = A.get(&pos[b], sizeof(T) * 3 * N * cap)) || (rc = A.get(&ainv[b], sizeof(T) * std::max(1, nainv) * cap)))
+      return rc;
+    for (double** q : {&logdet[b], &sign[b], &ju[b], &eloc[b], &eprev[b]})
+      if ((rc = A.get(q, sizeof(double) * cap))) return rc;
+  }
+  DmcBufs B{};
+  double *partial = nullptr, *sums = nullptr;
+  if ((rc = A.get(&B.ctl, sizeof(DmcCtl))) || (rc = A.get(&B.copies, sizeof(int) * cap)) ||
+      (rc = A.get(&B.loff, sizeof(int) * cap)) || (rc = A.get(&B.bsum, sizeof(int64_t) * nb)) ||
+      (rc = A.get(&B.part, sizeof(double) * nb * (1 + 3 * M))) || (rc = A.get(&B.e_trace, sizeof(double) * T_all)) ||
+      (rc = A.get(&B.pop_trace, sizeof(double) * T_all)) || (rc = A.get(&partial, sizeof(double) * 3 * nblk64)) ||
+      (rc = A.get(&sums, sizeof(double) * 3 * T_all)))
+    return rc;
+  HIP_TRY(hipMemsetAsync(B.ctl, 0, sizeof(DmcCtl), st));
+  EnsView<T> V[2];
+  for (int b = 0; b < 2; ++b)
+    V[b] = EnsView<T>{cap, cap, pos[b], ainv[b], logdet[b], sign[b], ju[b], eloc[b], &B.ctl->n_live[b]};
+  const double tau_b = p->matched_branching ? 2.0 * p->time_step : p->time_step;
+  const DmcArgs a{3 * N, nainv, M, cap, tau_b, p->e_ref_damping, (double)p->n_walkers, seed};
+  // a fresh population of n_walkers in buffer b from walker ids w0 + w
+  auto init_pop = [&](int b, int64_t w0) -> int {
+    EnsView<T> E = V[b];
+    E.W = p->n_walkers;
+    E.n_live = nullptr;
+    if constexpr (sizeof(T) == 8) {
+      HIP_TRY(launch_init64(s->d64, E, seed, w0, st));
+      HIP_TRY(s->ops->build64(s->d64, s->d64, E, 1, nullptr, st));
+    } else {
+      HIP_TRY(launch_init32(s->d64, E, seed, w0, st));
+      HIP_TRY(s->ops->build32(s->d64, s->d32, E, 1, nullptr, st));
+    }
+    HIP_TRY(dmc_reset(B.ctl, b, p->n_walkers, eloc[b], eprev[b], st));
+    return QMC_OK;
+  };
+  out->e_trace.assign(T_all, 0.0);
+  out->pop_trace.assign(T_all, 0.0);
+  out->fresh.assign(T_all, 0);
+  std::vector<int> cur_after(T_all);
+  int cur = 0;
+  if ((rc = init_pop(cur, 0))) return rc;
+  out->fresh[0] = 1;
+  const int R = f64 ? 0 : std::max(0, p->refresh_interval);
+  int since = 0;
+  hipEvent_t ev0, ev1;
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  HIP_TRY(hipEventRecord(ev0, st));
+  const int64_t chunk = 64;
+  int64_t t = 0;
+  while (t < T_all) {
+    const int64_t t_end = std::min(T_all, t + chunk);
+    for (int64_t u = t; u < t_end; ++u) {
+      const SweepArgs sa{1, (uint32_t)u, seed, 0, p->time_step, 0, p->max_drift};
+      if constexpr (sizeof(T) == 8) {
+        HIP_TRY(s->ops->dd64(s->d64, V[cur], sa, partial, nullptr, st));
+      } else {
+        if (R > 0 && since >= R) {
+          HIP_TRY(s->ops->build32(s->d64, s->d32, V[cur], 0, nullptr, st));
+          since = 0;
+        }
+        HIP_TRY(s->ops->dd32(s->d32, V[cur], sa, partial, nullptr, st));
+      }
+      HIP_TRY(launch_reduce(partial, 1, nblk64, 3, sums + 3 * u, st));
+      const int nxt = 1 - cur;
+      HIP_TRY(dmc_branch_step<T>(s->d64, a, B, cur, V[cur], eprev[cur], V[nxt], eprev[nxt], (uint32_t)u, u,
+                                 u >= p->n_burnin ? 1 : 0, st));
+      cur = nxt;
+      cur_after[u] = cur;
+      ++since;
+    }
+    HIP_TRY(hipMemcpyAsync(out->pop_trace.data() + t, B.pop_trace + t, sizeof(double) * (t_end - t),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    int64_t dead = -1;
+    for (int64_t u = t; u < t_end && dead < 0; ++u)
+      if (out->pop_trace[u] == 0.0) dead = u;
+    if (dead < 0) {
+      t = t_end;
+      continue;
+    }
+    // the population died out at step dead: start again from fresh walkers
+    // (is_dmc.rs:424-433, 492-501) and redo the steps queued after it
+    ++out->collapses;
+    cur = cur_after[dead];
+    if ((rc = init_pop(cur, out->collapses * p->n_walkers))) return rc;
+    since = 0;
+    t = dead + 1;
+    if (t < T_all) out->fresh[t] = 1;
+  }
+  HIP_TRY(hipEventRecord(ev1, st));
+  HIP_TRY(hipMemcpyAsync(out->e_trace.data(), B.e_trace, sizeof(double) * T_all, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(out->pop_trace.data(), B.pop_trace, sizeof(double) * T_all, hipMemcpyDeviceToHost, st));
+  std::vector<double> sh(3 * T_all);
+  HIP_TRY(hipMemcpyAsync(sh.data(), sums, sizeof(double) * 3 * T_all, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(&out->ctl, B.ctl, sizeof(DmcCtl), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  out->step_ms = ms;
+  out->acc.resize(T_all);
+  for (int64_t u = 0; u < T_all; ++u) out->acc[u] = sh[3 * u + 2];
+  return QMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ImportanceSampledDMC::block_error (is_dmc.rs:555-607)
+double qmc_isdmc_block_error(const double* e, int64_t n) {
+  if (!e || n < 20) return 0.0;
+  double mean = 0.0;
+  for (int64_t i = 0; i < n; ++i) mean += e[i];
+  mean /= (double)n;
+  double var = 0.0;
+  for (int64_t i = 0; i < n; ++i) var += (e[i] - mean) * (e[i] - mean);
+  var /= (double)n;
+  if (var < 1e-30) return 0.0;
+  const double ac = qmc_autocorrelation_time(e, n);  // the same positive-sequence sum (is_dmc.rs:569-580)
+  const int64_t bs = (int64_t)std::ceil(2.0 * ac);
+  const int64_t nb = n / bs;
+  if (nb < 2) return std::sqrt(var * ac / (double)n);
+  std::vector<double> bm(nb);
+  for (int64_t b = 0; b < nb; ++b) {
+    double sum = 0.0;
+    for (int64_t i = b * bs; i < b * bs + bs; ++i) sum += e[i];
+    bm[b] = sum / (double)bs;
+  }
+  double bmean = 0.0;
+  for (double v : bm) bmean += v;
+  bmean /= (double)nb;
+  double bvar = 0.0;
+  for (double v : bm) bvar += (v - bmean) * (v - bmean);
+  bvar /= (double)(nb - 1);
+  return std::sqrt(bvar / (double)nb);
+}
+
+int qmc_isdmc_run(const qmc_system* sys, const qmc_isdmc_params* p, uint64_t seed, qmc_isdmc_results* res,
+                  double* forces_mixed, double* forces_vmc, double* forces_extrapolated, double* energy_trace,
+                  double* population_trace) {
+  if (!sys || !p || !res || p->n_walkers < 1 || p->n_steps < 1 || p->n_burnin < 0 || !(p->time_step > 0.0) ||
+      !(p->max_drift > 0.0) || p->n_vmc_equilibrate < 0 || p->n_vmc_force_samples < 0 || p->capacity < 0 ||
+      p->refresh_interval < 0 || p->n_burnin + p->n_steps > INT32_MAX || p->n_vmc_equilibrate > INT32_MAX ||
+      (p->matched_branching != 0 && p->matched_branching != 1))
+    return fail(QMC_ERR_INVALID, "bad arguments");
+  int rc = need_engine(sys);
+  if (rc) return rc;
+  if (sys->wave() || !sys->ops)
+    return fail(QMC_ERR_UNSUPPORTED, "IS-DMC needs drift-diffusion moves, which run on the register engine");
+  const int M = sys->h.n_nuc();
+  if (M > MAXN) return fail(QMC_ERR_UNSUPPORTED, "too many nuclei");
+  int64_t cap = p->capacity > 0 ? p->capacity : 4 * p->n_walkers;
+  cap = (cap + DMC_BLOCK - 1) / DMC_BLOCK * DMC_BLOCK;
+  if ((p->capacity > 0 && p->capacity < p->n_walkers) || cap > (int64_t)DMC_BLOCK * DMC_MAX_BLOCKS)
+    return fail(QMC_ERR_INVALID, "capacity must hold n_walkers and be at most 1048576");
+  const std::vector<double> fnn = hf_nn(sys->h);
+  // phase 1: VMC forces (is_dmc.rs:305-370)
+  std::vector<double> fv(3 * M, 0.0);
+  {
+    const int64_t nv = std::min<int64_t>(p->n_walkers, 50);
+    qmc_ensemble* ev = nullptr;
+    if ((rc = qmc_ensemble_create(sys, nv, seed ^ ISDMC_VMC_KEY, &ev))) return rc;
+    std::unique_ptr<qmc_ensemble> guard(ev);
+    qmc_sweep_params sp{QMC_MOVE_ALL, (int32_t)p->n_vmc_equilibrate, 0.5, 0, 0, 0.0};
+    if (p->n_vmc_equilibrate > 0 && (rc = qmc_ensemble_sweeps(ev, &sp, nullptr, nullptr, nullptr, nullptr)))
+      return rc;
+    const int64_t nfb = std::max<int64_t>(p->n_vmc_force_samples / nv, 1);
+    DevArena A;
+    double *part = nullptr, *vout = nullptr;
+    if ((rc = A.get(&part, sizeof(double) * 3 * M)) || (rc = A.get(&vout, sizeof(double) * 3 * M * nfb))) return rc;
+    sp.n_steps = 5;  // decorrelation sweeps between force samples (is_dmc.rs:334-353)
+    for (int64_t it = 0; it < nfb; ++it) {
+      if ((rc = qmc_ensemble_sweeps(ev, &sp, nullptr, nullptr, nullptr, nullptr))) return rc;
+      int nblk = 0;
+      if (ev->prec == QMC_FP64)
+        HIP_TRY(hf_partials<double>(sys->d64, ev->view<double>(), nv, 3 * sys->h.n_elec(), M, part, &nblk, ev->st));
+      else
+        HIP_TRY(hf_partials<float>(sys->d64, ev->view<float>(), nv, 3 * sys->h.n_elec(), M, part, &nblk, ev->st));
+      HIP_TRY(launch_reduce(part, 1, nblk, 3 * M, vout + 3 * M * it, ev->st));
+    }
+    std::vector<double> h(3 * M * nfb);
+    HIP_TRY(hipMemcpyAsync(h.data(), vout, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ev->st));
+    HIP_TRY(hipStreamSynchronize(ev->st));
+    for (int64_t it = 0; it < nfb; ++it)
+      for (int q = 0; q < 3 * M; ++q) fv[q] += h[3 * M * it + q];
+    for (int q = 0; q < 3 * M; ++q) fv[q] = fv[q] / (double)(nv * nfb) + fnn[q];
+  }
+  // phases 2-4: the branching population
+  DmcHost H;
+  rc = sys->precision == QMC_FP64 ? isdmc_steps<double>(sys, p, seed, cap, &H)
+                                  : isdmc_steps<float>(sys, p, seed, cap, &H);
+  if (rc) return rc;
+  // phase 5: results (is_dmc.rs:510-545)
+  const int64_t n = p->n_steps, b0 = p->n_burnin;
+  const double* E = H.e_trace.data() + b0;
+  double sum_e = 0.0, sum_pop = 0.0, acc = 0.0, moves = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    sum_e += E[i];
+    sum_pop += H.pop_trace[b0 + i];
+  }
+  const double n_e = (double)sys->h.n_elec();
+  for (int64_t u = b0; u < b0 + n; ++u) {
+    const double before = H.fresh[u] ? (double)p->n_walkers : H.pop_trace[u - 1];
+    acc += H.acc[u];
+    moves += before * n_e;
+  }
+  res->energy = sum_e / (double)n;
+  res->error = qmc_isdmc_block_error(E, n);
+  res->acceptance_rate = moves > 0.0 ? acc / moves : 0.0;
+  res->avg_population = sum_pop / (double)n;
+  res->collapses = H.collapses;
+  res->dropped = H.ctl.dropped;
+  res->capacity = cap;
+  res->step_ms = H.step_ms;
+  std::vector<double> fm(3 * M);
+  for (int q = 0; q < 3 * M; ++q) fm[q] = H.ctl.force_sum[q] / (double)H.ctl.n_force + fnn[q];
+  for (int q = 0; q < 3 * M; ++q) {
+    if (forces_mixed) forces_mixed[q] = fm[q];
+    if (forces_vmc) forces_vmc[q] = fv[q];
+    if (forces_extrapolated) forces_extrapolated[q] = 2.0 * fm[q] - fv[q];
+  }
+  if (energy_trace) std::copy(E, E + n, energy_trace);
+  if (population_trace) std::copy(H.pop_trace.begin() + b0, H.pop_trace.end(), population_trace);
+  return QMC_OK;
+}
+
+}  // extern "C"

@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(64) build_state_kernel(const DevSys<double>* _
                                                          const DevSys<T>* __restrict__ St, EnsView<T> E,
                                                          int want_eloc, int* singular_count) {
   const int64_t w = (int64_t)blockIdx.x * 64 + threadIdx.x;
-  if (w >= E.W) return;
+  if (w >= (E.n_live ? *E.n_live : E.W)) return;
   WalkerRegs<NU, ND, T> R;
   load_positions<NU, ND>(*St, E.pos, E.ld, w, R);
   double lu, su, ldn, sd;

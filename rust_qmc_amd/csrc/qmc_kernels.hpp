@@ -18,6 +18,9 @@ struct EnsView {
   double* sign;    // [ld]
   double* ju;      // [ld] Jastrow exponent ln J
   double* eloc;    // [ld] last local energy
+  // live walker count in device memory (a branching population), or null
+  // for W; read by the drift-diffusion sweep and the state build
+  const int64_t* n_live = nullptr;
 };
 
 struct SweepArgs {

@@ -21,7 +21,7 @@ enum : int { SH_S = 0, SH_P = 1, SH_SP = 2, SH_P1 = 3, SH_STO = 4, SH_SLATER1S =
 // Jastrow kinds
 enum : int { JAS_NONE = 0, JAS_PAIR = 1, JAS_PADE = 2 };
 // Philox streams (fourth counter word)
-enum : uint32_t { RNG_INIT = 1u, RNG_SINGLE = 2u, RNG_ALL = 3u, RNG_SYS = 4u, RNG_DRIFT = 5u };
+enum : uint32_t { RNG_INIT = 1u, RNG_SINGLE = 2u, RNG_ALL = 3u, RNG_SYS = 4u, RNG_DRIFT = 5u, RNG_BRANCH = 6u };
 
 // flags (mirrors include/qmc.h)
 enum : uint32_t {
