@@ -132,6 +132,22 @@ def run_batch(Q, sweeps, trial):
     return out
 
 
+def run_isdmc(Q, precision, W, n_steps):
+    """Importance-sampled DMC on CH4 STO-3G: one drift-diffusion sweep plus
+    the branching, compaction and E_ref feedback per step, all on the
+    device.  Throughput counts the walkers each step actually sweeps."""
+    g = Q.MethaneGTO(1.5, 2.0, 2.06, "STO-3G", 0, precision=precision)
+    P = Q.ISDMCParams(n_walkers=W, n_steps=n_steps, n_burnin=0, time_step=0.005, n_vmc_equilibrate=20,
+                      n_vmc_force_samples=100, refresh_interval=64)
+    dmc = Q.ImportanceSampledDMC(g, P, seed=11)
+    r = dmc.run()
+    pops = dmc.populations
+    swept = float(W + pops[:-1].sum())  # walkers entering each step's sweep
+    return {"walker_steps_per_s": swept / (r.step_ms / 1e3), "walker_moves_per_s": 10 * swept / (r.step_ms / 1e3),
+            "ms_per_step": r.step_ms / n_steps, "avg_population": r.avg_population, "collapses": r.collapses,
+            "dropped": r.dropped, "acceptance": r.acceptance_rate}
+
+
 def cpu_baseline(seconds_target):
     """The reference algorithm (vmc.rs:95-119: all-electron proposal, full
     recomputation of psi, E_L with 8 sector LU per accepted move) on the CH4
@@ -170,6 +186,7 @@ def main():
     ap.add_argument("--sr-calls", type=int, default=20, help="timed SR-grad measurement calls (W samples each)")
     ap.add_argument("--batch-sweeps", type=int, default=32,
                     help="timed sweeps of the benzene + CH4 + H2 batch (0: skip it)")
+    ap.add_argument("--dmc-steps", type=int, default=100, help="IS-DMC steps per precision (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 8 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -191,6 +208,7 @@ def main():
     res = {m: run_mode(Q, m, W, K, args.warmup, args.steps_per_launch, args.refresh) for m in ("fp64", "mixed")}
     sr = {m: run_sr(Q, m, W, args.sr_calls, 16) for m in ("fp64", "mixed")}
     batch = run_batch(Q, args.batch_sweeps, 4) if args.batch_sweeps > 0 else None
+    dmc = {m: run_isdmc(Q, m, W, args.dmc_steps) for m in ("mixed", "fp64")} if args.dmc_steps > 0 else None
     mx, fp = res["mixed"], res["fp64"]
     # roofline of the dominant kernel (the mixed-precision sweep)
     avg_launch_s = mx["kernel_ms"] / 1e3 / mx["launches"]
@@ -248,6 +266,7 @@ def main():
                                   "ms_per_65536_samples": {"mixed": sr["mixed"]["ms_per_call"],
                                                            "fp64": sr["fp64"]["ms_per_call"]}},
         "heterogeneous_batch": batch,
+        "isdmc": dmc,
         "roofline": {
             "bound": "mfma",
             "engine": "vector ALU: the sweep has no GEMM-shaped work; 157.3 TFLOP/s is MI355X's FP32 peak "

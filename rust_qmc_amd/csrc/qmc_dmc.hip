@@ -10,7 +10,8 @@
 //   dmc_copy_kernel     writes every survivor and clone into buffer nxt in
 //                       source order (clones adjacent) with per-workgroup sums
 //                       of E_L and of the HF force over the new population
-//   dmc_finalize_kernel mean E_L, population, force sums and the E_ref feedback
+//   dmc_finalize_kernel mean E_L, population, force sums, the sweep's accepted
+//                       moves and the E_ref feedback
 //                       E_ref = <E_L> - damping ln(N / N_target) / tau
 // The live counts, E_ref and the sums stay in device memory.
 #include <hip/hip_runtime.h>
@@ -45,10 +46,10 @@ __device__ __forceinline__ void hf_electronic(const DevSys<double>& S, const T* 
         const double dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
         const double d = sqrt(dx * dx + dy * dy + dz * dz);
         if (d > 1e-10) {
-          const double d3 = d * d * d;
-          f[I][0] += S.Z[I] * dx / d3;
-          f[I][1] += S.Z[I] * dy / d3;
-          f[I][2] += S.Z[I] * dz / d3;
+          const double s = S.Z[I] / (d * d * d);
+          f[I][0] += s * dx;
+          f[I][1] += s * dy;
+          f[I][2] += s * dz;
         }
       }
     }
@@ -194,33 +195,49 @@ __global__ void __launch_bounds__(DMC_BLOCK) dmc_copy_kernel(const DevSys<double
   }
 }
 
+// One workgroup: sums the copy kernel's partials (E_L, HF force) and the
+// sweep's accepted-move partials in a single pass, then updates the control
+// block.  acc_out receives the step's accepted moves.
 __global__ void __launch_bounds__(256) dmc_finalize_kernel(DmcArgs a, DmcBufs b, int nxt, int nb, int64_t ts,
-                                                           int with_force) {
+                                                           int with_force, const double* __restrict__ sweep_part,
+                                                           int sweep_nblk, double* __restrict__ acc_out) {
+  constexpr int QM = 2 + 3 * MAXN;  // E_L, accepted moves, forces
   const int Q = 1 + 3 * a.n_nuc;
-  __shared__ double sh[256];
-  __shared__ double tot[1 + 3 * MAXN];
-  for (int q = 0; q < Q; ++q) {
-    double v = 0.0;
-    for (int k = threadIdx.x; k < nb; k += 256) v += b.part[(int64_t)k * Q + q];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) tot[q] = sh[0];
-    __syncthreads();
+  const int nq = with_force ? Q : 1;
+  double v[QM];
+#pragma unroll
+  for (int q = 0; q < QM; ++q) v[q] = 0.0;
+  for (int k = threadIdx.x; k < nb; k += 256) {
+    const double* p = b.part + (int64_t)k * Q;
+    v[0] += p[0];
+#pragma unroll
+    for (int q = 1; q < 1 + 3 * MAXN; ++q)
+      if (q < nq) v[q + 1] += p[q];
   }
+  for (int k = threadIdx.x; k < sweep_nblk; k += 256) v[1] += sweep_part[(int64_t)k * 3 + 2];
+  __shared__ double sh[256 / 64][QM];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    if (q < nq + 1) {
+      const double x = dmc_wave_sum(v[q]);
+      if (lane == 0) sh[wv][q] = x;
+    }
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double tot[QM];
+    for (int q = 0; q < nq + 1; ++q) tot[q] = sh[0][q] + sh[1][q] + sh[2][q] + sh[3][q];
     DmcCtl* c = b.ctl;
     const int64_t N = c->n_live[nxt];
     const double e_mean = tot[0] / (double)N;  // 0/0 = NaN when the population died, as is_dmc.rs:474-475
     if (ts >= 0) {
       b.e_trace[ts] = e_mean;
       b.pop_trace[ts] = (double)N;
+      acc_out[ts] = tot[1];
     }
     if (with_force) {
-      for (int q = 0; q < 3 * a.n_nuc; ++q) c->force_sum[q] += tot[1 + q];
+      for (int q = 0; q < 3 * a.n_nuc; ++q) c->force_sum[q] += tot[2 + q];
       c->n_force += N;
     }
     c->e_ref = e_mean - a.damping * log((double)N / a.n_target) / a.tau_b;  // is_dmc.rs:487-489
@@ -271,7 +288,8 @@ __global__ void __launch_bounds__(DMC_BLOCK) hf_partials_kernel(const DevSys<dou
 template <class T>
 hipError_t dmc_branch_step(const void* Sd, const DmcArgs& a, const DmcBufs& b, int cur, EnsView<T> src,
                            const double* eprev_src, EnsView<T> dst, double* eprev_dst, uint32_t t, int64_t ts,
-                           int with_force, hipStream_t st) {
+                           int with_force, const double* sweep_part, int sweep_nblk, double* acc_out,
+                           hipStream_t st) {
   if (a.cap <= 0 || a.cap % DMC_BLOCK || a.cap / DMC_BLOCK > DMC_MAX_BLOCKS || a.n_nuc > MAXN || src.ld < a.cap ||
       dst.ld < a.cap)
     return hipErrorInvalidValue;
@@ -279,7 +297,7 @@ hipError_t dmc_branch_step(const void* Sd, const DmcArgs& a, const DmcBufs& b, i
   dmc_branch_kernel<<<nb, DMC_BLOCK, 0, st>>>(a, b, cur, src.eloc, eprev_src, t);
   dmc_scan_kernel<<<1, 1024, 0, st>>>(a, b, nxt, nb);
   dmc_copy_kernel<T><<<nb, DMC_BLOCK, 0, st>>>((const DevSys<double>*)Sd, a, b, cur, src, dst, eprev_dst, with_force);
-  dmc_finalize_kernel<<<1, 256, 0, st>>>(a, b, nxt, nb, ts, with_force);
+  dmc_finalize_kernel<<<1, 256, 0, st>>>(a, b, nxt, nb, ts, with_force, sweep_part, sweep_nblk, acc_out);
   return hipGetLastError();
 }
 
@@ -301,10 +319,10 @@ hipError_t hf_partials(const void* Sd, EnsView<T> E, int64_t W, int n3, int n_nu
 
 template hipError_t dmc_branch_step<double>(const void*, const DmcArgs&, const DmcBufs&, int, EnsView<double>,
                                             const double*, EnsView<double>, double*, uint32_t, int64_t, int,
-                                            hipStream_t);
+                                            const double*, int, double*, hipStream_t);
 template hipError_t dmc_branch_step<float>(const void*, const DmcArgs&, const DmcBufs&, int, EnsView<float>,
                                            const double*, EnsView<float>, double*, uint32_t, int64_t, int,
-                                           hipStream_t);
+                                           const double*, int, double*, hipStream_t);
 template hipError_t hf_partials<double>(const void*, EnsView<double>, int64_t, int, int, double*, int*, hipStream_t);
 template hipError_t hf_partials<float>(const void*, EnsView<float>, int64_t, int, int, double*, int*, hipStream_t);
 

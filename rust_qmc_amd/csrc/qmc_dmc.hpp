@@ -44,11 +44,14 @@ struct DmcBufs {
 // Branch the population of buffer cur (post-sweep E_L in src.eloc, pre-sweep
 // in eprev_src) into buffer nxt and update E_ref; step t is the global step
 // (Philox counter), trace slot ts its row in the traces (-1: none), and
-// with_force adds the new population's HF force to ctl->force_sum.
+// with_force adds the new population's HF force to ctl->force_sum.  The
+// sweep's per-block partials [sweep_nblk][3] (accepted moves in column 2)
+// are summed into acc_out[ts].
 template <class T>
 hipError_t dmc_branch_step(const void* Sd, const DmcArgs& a, const DmcBufs& b, int cur, EnsView<T> src,
                            const double* eprev_src, EnsView<T> dst, double* eprev_dst, uint32_t t, int64_t ts,
-                           int with_force, hipStream_t st);
+                           int with_force, const double* sweep_part, int sweep_nblk, double* acc_out,
+                           hipStream_t st);
 // a fresh population of n0 walkers in buffer b0 (positions, state and E_L
 // already built): live count, pre-sweep E_L = E_L, E_ref = mean E_L
 hipError_t dmc_reset(DmcCtl* ctl, int b0, int64_t n0, const double* eloc, double* eprev, hipStream_t st);

@@ -2104,12 +2104,12 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
       if ((rc = A.get(q, sizeof(double) * cap))) return rc;
   }
   DmcBufs B{};
-  double *partial = nullptr, *sums = nullptr;
+  double *partial = nullptr, *acc_dev = nullptr;
   if ((rc = A.get(&B.ctl, sizeof(DmcCtl))) || (rc = A.get(&B.copies, sizeof(int) * cap)) ||
       (rc = A.get(&B.loff, sizeof(int) * cap)) || (rc = A.get(&B.bsum, sizeof(int64_t) * nb)) ||
       (rc = A.get(&B.part, sizeof(double) * nb * (1 + 3 * M))) || (rc = A.get(&B.e_trace, sizeof(double) * T_all)) ||
       (rc = A.get(&B.pop_trace, sizeof(double) * T_all)) || (rc = A.get(&partial, sizeof(double) * 3 * nblk64)) ||
-      (rc = A.get(&sums, sizeof(double) * 3 * T_all)))
+      (rc = A.get(&acc_dev, sizeof(double) * T_all)))
     return rc;
   HIP_TRY(hipMemsetAsync(B.ctl, 0, sizeof(DmcCtl), st));
   EnsView<T> V[2];
@@ -2160,10 +2160,9 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
         }
         HIP_TRY(s->ops->dd32(s->d32, V[cur], sa, partial, nullptr, st));
       }
-      HIP_TRY(launch_reduce(partial, 1, nblk64, 3, sums + 3 * u, st));
       const int nxt = 1 - cur;
       HIP_TRY(dmc_branch_step<T>(s->d64, a, B, cur, V[cur], eprev[cur], V[nxt], eprev[nxt], (uint32_t)u, u,
-                                 u >= p->n_burnin ? 1 : 0, st));
+                                 u >= p->n_burnin ? 1 : 0, partial, nblk64, acc_dev, st));
       cur = nxt;
       cur_after[u] = cur;
       ++since;
@@ -2190,8 +2189,8 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
   HIP_TRY(hipEventRecord(ev1, st));
   HIP_TRY(hipMemcpyAsync(out->e_trace.data(), B.e_trace, sizeof(double) * T_all, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(out->pop_trace.data(), B.pop_trace, sizeof(double) * T_all, hipMemcpyDeviceToHost, st));
-  std::vector<double> sh(3 * T_all);
-  HIP_TRY(hipMemcpyAsync(sh.data(), sums, sizeof(double) * 3 * T_all, hipMemcpyDeviceToHost, st));
+  out->acc.resize(T_all);
+  HIP_TRY(hipMemcpyAsync(out->acc.data(), acc_dev, sizeof(double) * T_all, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(&out->ctl, B.ctl, sizeof(DmcCtl), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   float ms = 0.f;
@@ -2199,8 +2198,6 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   out->step_ms = ms;
-  out->acc.resize(T_all);
-  for (int64_t u = 0; u < T_all; ++u) out->acc[u] = sh[3 * u + 2];
   return QMC_OK;
 }
 
