@@ -315,6 +315,32 @@ typedef struct {
 int qmc_sample_forces(const qmc_system* sys, const qmc_force_params* p, uint64_t seed, double* forces,
                       double* mean_energy, double* error, double* force_variance);
 
+/* GeometryOptimizer::optimize (geometry_opt.rs:233-347): steepest descent
+ * R_I += alpha (F_I - F_com) with F_com the charge-weighted mean force and
+ * each displacement capped at max_displacement (0.02 bohr in the reference,
+ * geometry_opt.rs:296-303), until max_I |F_I| < force_threshold or
+ * max_iterations force samplings; then one more sampling for the final
+ * energy.  Sampling k uses Philox key seed + k * 0x9E3779B97F4A7C15 (the
+ * reference draws fresh walkers each call).  The system's nuclei are moved
+ * (qmc_set_nuclei).  Histories (optional, host): energy and max |F| per
+ * sampling [max_iterations], geometries [max_iterations + 1][M][3], of
+ * which res->iterations (+1) entries are filled. */
+typedef struct {
+  double step_size;        /* alpha, bohr^2 / Ha */
+  int64_t max_iterations;
+  double force_threshold;  /* Ha / bohr */
+  double max_displacement; /* bohr per nucleus and step */
+} qmc_geomopt_params;
+typedef struct {
+  double final_energy;
+  double final_max_force;
+  int64_t iterations;  /* force samplings before the stop */
+  int32_t converged;
+} qmc_geomopt_results;
+int qmc_geometry_optimize(qmc_system* sys, const qmc_force_params* fp, const qmc_geomopt_params* gp, uint64_t seed,
+                          qmc_geomopt_results* res, double* energy_history, double* force_history,
+                          double* geometry_history);
+
 /* ---- heterogeneous batches: several ensembles in one persistent launch ----
  * No reference counterpart (the reference runs one system at a time); the
  * batch serves ensembles of different molecules sampled together.  In

@@ -34,7 +34,7 @@ __all__ = [
     "MCMCParams", "MCMCResults", "MCMCSimulation", "autocorrelation_time", "blocking_error", "QMC_FP64",
     "QMC_MIXED", "QMC_MOVE_SINGLE", "QMC_MOVE_ALL", "QMC_MOVE_DRIFT", "QMC_FLAGS_REFERENCE", "Batch",
     "DDVMCParams", "DDVMCResults", "DriftDiffusionVMC",
-    "ISDMCParams", "ISDMCResults", "ImportanceSampledDMC", "isdmc_block_error",
+    "ISDMCParams", "ISDMCResults", "ImportanceSampledDMC", "isdmc_block_error", "GeometryOptResult",
 ]
 
 _PREC = {"fp64": QMC_FP64, "mixed": QMC_MIXED, QMC_FP64: QMC_FP64, QMC_MIXED: QMC_MIXED}
@@ -535,28 +535,65 @@ class SROptimizer:
 
 
 @dataclass
+class GeometryOptResult:
+    """geometry_opt.rs:351-368; histories cover the force samplings made."""
+    final_nuclei: np.ndarray
+    final_energy: float
+    final_max_force: float
+    energy_history: np.ndarray
+    force_history: np.ndarray
+    geometry_history: np.ndarray
+    zero_variance: bool
+    converged: bool
+
+
+@dataclass
 class GeometryOptimizer:
-    """Sampling settings of geometry_opt.rs:16-53 (sample_forces only)."""
+    """geometry_opt.rs:16-53 (defaults as Default for GeometryOptimizer; the
+    verbose printing has no counterpart) plus the sampler's move kind,
+    decorrelation sweeps, refresh interval, the displacement cap the
+    reference hard-codes (geometry_opt.rs:298) and the Philox seed."""
     n_samples: int = 5000
     n_walkers: int = 20
     n_equilibrate: int = 500
+    step_size: float = 0.005
+    max_iterations: int = 50
+    force_threshold: float = 0.05
     mc_step_size: float = 0.5
     zero_variance: bool = False
     decorrelation: int = 5
     move: int = QMC_MOVE_ALL
     refresh_interval: int = 64
+    max_displacement: float = 0.02
     seed: int = 0
+
+    def _force_params(self):
+        return _lib.QmcForceParams(self.n_samples, self.n_walkers, self.n_equilibrate, self.mc_step_size,
+                                   int(self.zero_variance), self.decorrelation, self.move, self.refresh_interval)
 
     def sample_forces(self, wfn: System):
         """geometry_opt.rs:105-227: (mean_forces [M][3], mean_energy, error, force_variance [M])."""
-        p = _lib.QmcForceParams(self.n_samples, self.n_walkers, self.n_equilibrate, self.mc_step_size,
-                                int(self.zero_variance), self.decorrelation, self.move, self.refresh_interval)
+        p = self._force_params()
         M = wfn.n_nuc
         F = np.zeros((M, 3))
         fv = np.zeros(M)
         me, err = C.c_double(), C.c_double()
         check(lib().qmc_sample_forces(wfn._h, C.byref(p), self.seed, _ptr(F), C.byref(me), C.byref(err), _ptr(fv)))
         return F, me.value, err.value, fv
+
+    def optimize(self, wfn: System) -> GeometryOptResult:
+        """geometry_opt.rs:233-347 on the GPU (qmc_geometry_optimize); moves wfn's nuclei."""
+        fp = self._force_params()
+        gp = _lib.QmcGeomoptParams(self.step_size, self.max_iterations, self.force_threshold, self.max_displacement)
+        res = _lib.QmcGeomoptResults()
+        M, K = wfn.n_nuc, self.max_iterations
+        eh, fh, gh = np.zeros(max(K, 1)), np.zeros(max(K, 1)), np.zeros((K + 1, M, 3))
+        check(lib().qmc_geometry_optimize(wfn._h, C.byref(fp), C.byref(gp), self.seed, C.byref(res), _ptr(eh),
+                                          _ptr(fh), _ptr(gh)))
+        n = res.iterations
+        steps = n - 1 if res.converged else n  # geometries after each displacement
+        return GeometryOptResult(wfn.get_nuclei(), res.final_energy, res.final_max_force, eh[:n], fh[:n],
+                                 gh[: steps + 1], self.zero_variance, bool(res.converged))
 
 
 @dataclass

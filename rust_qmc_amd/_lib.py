@@ -45,7 +45,7 @@ EXPORTED = [
     "qmc_ensemble_get_state", "qmc_ensemble_refresh", "qmc_ensemble_sweeps", "qmc_ensemble_set_timing",
     "qmc_ensemble_timing", "qmc_vmc_run", "qmc_derivatives", "qmc_log_derivs", "qmc_dEL_dp", "qmc_hf_force",
     "qmc_nuc_grad_log", "qmc_sr_solve", "qmc_ensemble_measure", "qmc_ensemble_moments", "qmc_sr_optimize",
-    "qmc_sample_forces",
+    "qmc_sample_forces", "qmc_geometry_optimize",
     "qmc_ddvmc_run", "qmc_isdmc_run", "qmc_isdmc_block_error",
     "qmc_batch_create", "qmc_batch_destroy", "qmc_batch_fusable", "qmc_batch_set_config", "qmc_batch_get_config",
     "qmc_batch_sweeps", "qmc_batch_timing", "qmc_batch_tune",
@@ -122,6 +122,14 @@ class QmcDdvmcResults(C.Structure):
                 ("final_time_step", _d)]
 
 
+class QmcGeomoptParams(C.Structure):
+    _fields_ = [("step_size", _d), ("max_iterations", _i64), ("force_threshold", _d), ("max_displacement", _d)]
+
+
+class QmcGeomoptResults(C.Structure):
+    _fields_ = [("final_energy", _d), ("final_max_force", _d), ("iterations", _i64), ("converged", _i32)]
+
+
 class QmcIsdmcParams(C.Structure):
     _fields_ = [("n_walkers", _i64), ("n_steps", _i64), ("n_burnin", _i64), ("time_step", _d), ("max_drift", _d),
                 ("e_ref_damping", _d), ("n_vmc_equilibrate", _i64), ("n_vmc_force_samples", _i64),
@@ -184,6 +192,7 @@ _SIGS = {
     "qmc_ensemble_moments": (_i32, [_p, _p]),
     "qmc_sr_optimize": (_i32, [_p, _p, _u64, _p, _p, _p, _p]),
     "qmc_sample_forces": (_i32, [_p, _p, _u64, _p, _p, _p, _p]),
+    "qmc_geometry_optimize": (_i32, [_p, _p, _p, _u64, _p, _p, _p, _p]),
     "qmc_ddvmc_run": (_i32, [_p, _p, _u64, _p, _p]),
     "qmc_isdmc_run": (_i32, [_p, _p, _u64, _p, _p, _p, _p, _p, _p]),
     "qmc_isdmc_block_error": (_d, [_p, _i64]),

@@ -1586,6 +1586,59 @@ int qmc_sample_forces(const qmc_system* sys, const qmc_force_params* p, uint64_t
   return QMC_OK;
 }
 
+int qmc_geometry_optimize(qmc_system* sys, const qmc_force_params* fp, const qmc_geomopt_params* gp, uint64_t seed,
+                          qmc_geomopt_results* res, double* energy_history, double* force_history,
+                          double* geometry_history) {
+  if (!sys || !fp || !gp || !res || gp->max_iterations < 0 || !(gp->max_displacement > 0.0) ||
+      !(gp->force_threshold >= 0.0))
+    return fail(QMC_ERR_INVALID, "bad arguments");
+  const int M = sys->h.n_nuc();
+  double qsum = 0.0;
+  for (int I = 0; I < M; ++I) qsum += sys->h.Z[I];
+  std::vector<double> R(sys->h.nuc), F(3 * M), fv(M);
+  if (geometry_history) std::copy(R.begin(), R.end(), geometry_history);
+  auto key = [&](int64_t k) { return seed + (uint64_t)k * 0x9E3779B97F4A7C15ull; };
+  int rc;
+  double e = 0.0, err = 0.0, fmax = 0.0;
+  int64_t it = 0;
+  res->converged = 0;
+  for (; it < gp->max_iterations; ++it) {
+    if ((rc = qmc_sample_forces(sys, fp, key(it), F.data(), &e, &err, fv.data()))) return rc;
+    fmax = 0.0;
+    for (int I = 0; I < M; ++I)
+      fmax = std::max(fmax, std::sqrt(F[3 * I] * F[3 * I] + F[3 * I + 1] * F[3 * I + 1] + F[3 * I + 2] * F[3 * I + 2]));
+    if (energy_history) energy_history[it] = e;
+    if (force_history) force_history[it] = fmax;
+    if (fmax < gp->force_threshold) {
+      res->converged = 1;
+      ++it;
+      break;
+    }
+    // charge-weighted mean force (the charges serve as masses, geometry_opt.rs:283-289)
+    double com[3] = {0.0, 0.0, 0.0};
+    for (int I = 0; I < M; ++I)
+      for (int c = 0; c < 3; ++c) com[c] += F[3 * I + c] * sys->h.Z[I];
+    for (int c = 0; c < 3; ++c) com[c] /= qsum;
+    for (int I = 0; I < M; ++I) {
+      double d[3], n2 = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        d[c] = gp->step_size * (F[3 * I + c] - com[c]);
+        n2 += d[c] * d[c];
+      }
+      const double n = std::sqrt(n2), sc = n > gp->max_displacement ? gp->max_displacement / n : 1.0;
+      for (int c = 0; c < 3; ++c) R[3 * I + c] += d[c] * sc;
+    }
+    if ((rc = qmc_set_nuclei(sys, R.data(), M))) return rc;
+    if (geometry_history) std::copy(R.begin(), R.end(), geometry_history + 3 * M * (it + 1));
+  }
+  res->iterations = it;
+  // final energy at the final geometry (geometry_opt.rs:316-317)
+  if ((rc = qmc_sample_forces(sys, fp, key(it), F.data(), &e, &err, fv.data()))) return rc;
+  res->final_energy = e;
+  res->final_max_force = it > 0 ? fmax : 0.0;
+  return QMC_OK;
+}
+
 int qmc_ensemble_set_timing(qmc_ensemble* e, int32_t on) {
   if (!e) return fail(QMC_ERR_INVALID, "null ensemble");
   e->timing = on != 0;
