@@ -139,6 +139,33 @@ int qmc_methane_gto_create(double b_ee, double b_en, double ch_bond, int32_t bas
 int qmc_benzene_gto_create(double b_ee, double b_en, double cc_bond, double ch_bond, int32_t precision,
                            qmc_system** out);
 
+/* System descriptions as JSON text, replacing the YAML loaders of
+ * io/config.rs:20-35 (which read H2 only).  The document is one object in
+ * one of three forms:
+ *  - the serde layout of H2MoleculeMO (hydrogen.rs:27-35):
+ *      {"orbital1": {"alpha": a, "center": [x, y, z]}, "orbital2": {...},
+ *       "jastrow": {"cusp_param": F}}
+ *    (conf/h2.yaml's keys H1/H2 {alpha, R} and J {F} are read the same way);
+ *  - a named constructor: {"system": "MethaneGTO", "b_ee", "b_en",
+ *    "ch_bond", "basis": "6-31G" | "STO-3G", "mo_seed"}, {"system":
+ *    "BenzeneGTO", "b_ee", "b_en", "cc_bond", "ch_bond"}, {"system":
+ *    "LithiumSeeded", "seed"} (missing numbers take the constructors'
+ *    reference values);
+ *  - any molecule, the fields of qmc_system_desc: "nuclei" [[x,y,z]...],
+ *    "charges", "spins" (+1/-1), "aos" [{"kind": "gto"|"sto"|"slater1s",
+ *    "center", "l", "m", "exp": [...], "coef": [...], "pow": [...]}],
+ *    "mo_up" [[C_ma]...] (rows = MOs), optional "mo_dn", "jastrow" {"kind":
+ *    "none"|"pair"|"pade", "F", "b_ee", "b_en", "a_anti", "a_para",
+ *    "c_anti", "c_para", "species", "d_en" [[...] per species]},
+ *    "init_nuc", "init_sigma".
+ * "flags" is an integer or a list of "j2_grad_quirk", "j2_lap_quirk",
+ * "hf_nn_refsign", "singular_identity", "reference".  len = 0 means a
+ * NUL-terminated text.  Errors name the field or the byte offset. */
+int qmc_system_from_json(const char* text, int64_t len, int32_t precision, qmc_system** out);
+/* The system in the generic form above (exact: %.17g numbers).  Call with
+ * buf = NULL, len = 0 to get the size (with the NUL) in *needed. */
+int qmc_system_to_json(const qmc_system* sys, char* buf, int64_t len, int64_t* needed);
+
 int qmc_set_precision(qmc_system* sys, int32_t precision);
 /* Engine selection.  The register engine keeps one walker per lane in
  * VGPRs (compiled electron counts up to 5 + 5); the wave engine keeps one

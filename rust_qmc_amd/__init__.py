@@ -106,6 +106,40 @@ class System:
         h = cls._new(lib().qmc_system_create, C.byref(d), _PREC[precision])
         return cls(h, precision)
 
+    # ---- descriptions as JSON / YAML (io/config.rs:20-35) ------------------------
+    @classmethod
+    def from_json(cls, text, precision="fp64"):
+        """A system from a JSON description (the forms qmc_system_from_json
+        reads: the H2MoleculeMO serde layout, a named molecule, or the
+        generic description)."""
+        raw = text.encode() if isinstance(text, str) else bytes(text)
+        h = cls._new(lib().qmc_system_from_json, raw, len(raw), _PREC[precision])
+        return cls(h, precision)
+
+    @classmethod
+    def from_file(cls, path, precision="fp64"):
+        """A system from a .json file, or a .yaml/.yml file with the same
+        structure (read with yaml.safe_load)."""
+        import json
+        with open(path, "r", encoding="utf-8") as f:
+            text = f.read()
+        if str(path).lower().endswith((".yaml", ".yml")):
+            import yaml
+            text = json.dumps(yaml.safe_load(text))
+        return cls.from_json(text, precision)
+
+    def to_json(self) -> str:
+        """The generic description of this system (reloads exactly)."""
+        n = C.c_int64()
+        check(lib().qmc_system_to_json(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(lib().qmc_system_to_json(self._h, buf, n.value, C.byref(n)))
+        return buf.value.decode()
+
+    def save(self, path):
+        with open(path, "w", encoding="utf-8") as f:
+            f.write(self.to_json())
+
     # ---- configuration -------------------------------------------------------
     def set_precision(self, precision):
         check(lib().qmc_set_precision(self._h, _PREC[precision]))

@@ -46,7 +46,7 @@ EXPORTED = [
     "qmc_ensemble_timing", "qmc_vmc_run", "qmc_derivatives", "qmc_log_derivs", "qmc_dEL_dp", "qmc_hf_force",
     "qmc_nuc_grad_log", "qmc_sr_solve", "qmc_ensemble_measure", "qmc_ensemble_moments", "qmc_sr_optimize",
     "qmc_sample_forces", "qmc_geometry_optimize",
-    "qmc_ddvmc_run", "qmc_isdmc_run", "qmc_isdmc_block_error",
+    "qmc_ddvmc_run", "qmc_isdmc_run", "qmc_isdmc_block_error", "qmc_system_from_json", "qmc_system_to_json",
     "qmc_batch_create", "qmc_batch_destroy", "qmc_batch_fusable", "qmc_batch_set_config", "qmc_batch_get_config",
     "qmc_batch_sweeps", "qmc_batch_timing", "qmc_batch_tune",
     "qmc_autocorrelation_time", "qmc_blocking_error", "qmc_last_error", "qmc_version",
@@ -196,6 +196,8 @@ _SIGS = {
     "qmc_ddvmc_run": (_i32, [_p, _p, _u64, _p, _p]),
     "qmc_isdmc_run": (_i32, [_p, _p, _u64, _p, _p, _p, _p, _p, _p]),
     "qmc_isdmc_block_error": (_d, [_p, _i64]),
+    "qmc_system_from_json": (_i32, [C.c_char_p, _i64, _i32, _pp]),
+    "qmc_system_to_json": (_i32, [_p, _p, _i64, _p]),
     "qmc_batch_create": (_i32, [_p, _i32, _pp]),
     "qmc_batch_destroy": (_i32, [_p]),
     "qmc_batch_fusable": (_i32, [_p, _p]),

@@ -29,6 +29,14 @@ namespace {
 
 thread_local std::string g_err;
 
+int fail(int code, const std::string& msg);
+}  // namespace
+
+// error hook of the JSON reader (qmc_io.cpp)
+__attribute__((visibility("hidden"))) int qmc_io_fail(int code, const char* msg) { return fail(code, msg); }
+
+namespace {
+
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -1583,6 +1591,97 @@ int qmc_sample_forces(const qmc_system* sys, const qmc_force_params* p, uint64_t
   if (mean_energy) *mean_energy = m.energy;
   if (error) *error = std::sqrt(m.variance / (double)m.n_samples);  // geometry_opt.rs:198-201
   if (force_variance) std::copy(fv.begin(), fv.end(), force_variance);
+  return QMC_OK;
+}
+
+int qmc_system_to_json(const qmc_system* s, char* buf, int64_t len, int64_t* needed) {
+  if (!s || len < 0 || (len > 0 && !buf)) return fail(QMC_ERR_INVALID, "bad arguments");
+  const HostSys& h = s->h;
+  std::string o;
+  char tmp[40];
+  auto num = [&](double v) {
+    std::snprintf(tmp, sizeof tmp, "%.17g", v);
+    o += tmp;
+  };
+  auto nums = [&](const double* v, size_t n) {
+    o += '[';
+    for (size_t i = 0; i < n; ++i) {
+      if (i) o += ", ";
+      num(v[i]);
+    }
+    o += ']';
+  };
+  auto ints = [&](const int32_t* v, size_t n) {
+    o += '[';
+    for (size_t i = 0; i < n; ++i) o += (i ? ", " : "") + std::to_string(v[i]);
+    o += ']';
+  };
+  auto rows = [&](const double* v, int nr, int nc) {
+    o += '[';
+    for (int r = 0; r < nr; ++r) {
+      if (r) o += ", ";
+      nums(v + (size_t)r * nc, nc);
+    }
+    o += ']';
+  };
+  const int A = h.n_ao();
+  o += "{\"format\": \"rust_qmc_amd.system/1\", \"nuclei\": ";
+  rows(h.nuc.data(), h.n_nuc(), 3);
+  o += ", \"charges\": ";
+  nums(h.Z.data(), h.Z.size());
+  o += ", \"spins\": ";
+  ints(h.spins.data(), h.spins.size());
+  o += ", \"aos\": [";
+  for (int a = 0; a < A; ++a) {
+    const qmc_ao& q = h.aos[a];
+    o += a ? ", " : "";
+    o += std::string("{\"kind\": \"") +
+         (q.kind == QMC_AO_GTO ? "gto" : q.kind == QMC_AO_STO ? "sto" : "slater1s") + "\", \"center\": " +
+         std::to_string(q.center) + ", \"l\": " + std::to_string(q.l) + ", \"m\": " + std::to_string(q.m) +
+         ", \"exp\": ";
+    nums(h.pexp.data() + q.prim0, q.nprim);
+    o += ", \"coef\": ";
+    nums(h.pcoef.data() + q.prim0, q.nprim);
+    o += ", \"pow\": ";
+    ints(h.ppow.data() + q.prim0, q.nprim);
+    o += '}';
+  }
+  o += "], \"mo_up\": ";
+  rows(h.mo_up.data(), h.n_mo, A);
+  o += ", \"mo_dn\": ";
+  rows(h.mo_dn.data(), h.n_mo, A);
+  o += std::string(", \"jastrow\": {\"kind\": \"") +
+       (h.jas_kind == QMC_JAS_PAIR ? "pair" : h.jas_kind == QMC_JAS_PADE ? "pade" : "none") + "\", \"F\": ";
+  num(h.F);
+  o += ", \"b_ee\": ";
+  num(h.b_ee);
+  o += ", \"b_en\": ";
+  num(h.b_en);
+  o += ", \"a_anti\": ";
+  num(h.a_anti);
+  o += ", \"a_para\": ";
+  num(h.a_para);
+  if (h.k_ee > 0) {
+    o += ", \"c_anti\": ";
+    nums(h.c_anti.data(), h.k_ee);
+    o += ", \"c_para\": ";
+    nums(h.c_para.data(), h.k_ee);
+  }
+  if (h.k_en > 0) {
+    o += ", \"species\": ";
+    ints(h.species.data(), h.species.size());
+    o += ", \"d_en\": ";
+    rows(h.d_en.data(), h.n_species, h.k_en);
+  }
+  o += "}, \"init_nuc\": ";
+  ints(h.init_nuc.data(), h.init_nuc.size());
+  o += ", \"init_sigma\": ";
+  num(h.init_sigma);
+  o += ", \"flags\": " + std::to_string(h.flags) + "}";
+  if (needed) *needed = (int64_t)o.size() + 1;
+  if (len == 0) return QMC_OK;  // size query
+  if ((int64_t)o.size() + 1 > len) return fail(QMC_ERR_INVALID, "buffer too small");
+  std::memcpy(buf, o.c_str(), o.size() + 1);
   return QMC_OK;
 }
 
