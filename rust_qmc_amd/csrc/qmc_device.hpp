@@ -389,12 +389,14 @@ struct PairDerivs {
 // Pade series lengths: JsRt reads them from DevSys at run time; JsNone
 // compiles the series out (the reference Jastrow3 has none).
 struct JsRt {
+  static constexpr bool no_series = false;
   template <class T>
   static __device__ __forceinline__ int kee(const DevSys<T>& S) { return S.k_ee; }
   template <class T>
   static __device__ __forceinline__ int ken(const DevSys<T>& S) { return S.k_en; }
 };
 struct JsNone {
+  static constexpr bool no_series = true;
   template <class T>
   static __host__ __device__ constexpr int kee(const DevSys<T>&) { return 0; }
   template <class T>
@@ -428,6 +430,29 @@ __device__ __forceinline__ T uen_val(const DevSys<T>& S, T r, int I) {
     v += s;
   }
   return v;
+}
+
+// u(r_new) - u(r_old) of an e-e pair.  Without a series the Pade terms
+// combine over one reciprocal: a r'/(1+b r') - a r/(1+b r) =
+// a (r' - r) / ((1+b r')(1+b r)).
+template <class T, class Js = JsRt>
+__device__ __forceinline__ T uee_diff(const DevSys<T>& S, T rn, T ro, bool same) {
+  if constexpr (Js::no_series) {
+    const T a = same ? S.a_para : S.a_anti;
+    return a * (rn - ro) * M<T>::rcp((T(1) + S.b_ee * rn) * (T(1) + S.b_ee * ro));
+  } else {
+    return uee_val<T, Js>(S, rn, same) - uee_val<T, Js>(S, ro, same);
+  }
+}
+
+// the same for the e-n pair with nucleus I: -Z (r' - r) / ((1+b r')(1+b r))
+template <class T, class Js = JsRt>
+__device__ __forceinline__ T uen_diff(const DevSys<T>& S, T rn, T ro, int I) {
+  if constexpr (Js::no_series) {
+    return -S.Z[I] * (rn - ro) * M<T>::rcp((T(1) + S.b_en * rn) * (T(1) + S.b_en * ro));
+  } else {
+    return uen_val<T, Js>(S, rn, I) - uen_val<T, Js>(S, ro, I);
+  }
 }
 
 // u'(r) and u''(r) + 2 u'(r)/r for an e-e pair

@@ -360,8 +360,7 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
     auto pair = [&](T px, T py, T pz, bool same) {
       T ax = nx - px, ay = ny - py, az = nz - pz;
       T bx = ox - px, by = oy - py, bz = oz - pz;
-      du += uee_val<T, Js>(S, M<T>::sq(ax * ax + ay * ay + az * az), same) -
-            uee_val<T, Js>(S, M<T>::sq(bx * bx + by * by + bz * bz), same);
+      du += uee_diff<T, Js>(S, M<T>::sq(ax * ax + ay * ay + az * az), M<T>::sq(bx * bx + by * by + bz * bz), same);
     };
 #pragma unroll
     for (int j = 1; j < NA; ++j) pair(ra[j][0], ra[j][1], ra[j][2], true);
@@ -371,8 +370,7 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
       Bs::nuclei(S, [&](int I) {
         T ax = nx - S.nuc[I][0], ay = ny - S.nuc[I][1], az = nz - S.nuc[I][2];
         T bx = ox - S.nuc[I][0], by = oy - S.nuc[I][1], bz = oz - S.nuc[I][2];
-        du += uen_val<T, Js>(S, M<T>::sq(ax * ax + ay * ay + az * az), I) -
-              uen_val<T, Js>(S, M<T>::sq(bx * bx + by * by + bz * bz), I);
+        du += uen_diff<T, Js>(S, M<T>::sq(ax * ax + ay * ay + az * az), M<T>::sq(bx * bx + by * by + bz * bz), I);
       });
     const T p = R * R * M<T>::ex(T(2) * du);
     if (U < p) {

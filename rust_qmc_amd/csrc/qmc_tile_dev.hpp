@@ -250,14 +250,17 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<float>& S, EnsView
               const bool same = spin_of[p] == spin;
               const float ax = nx - px, ay = ny - py, az = nz - pz;
               const float bx = ox - px, by = oy - py, bz = oz - pz;
-              du += uee_val<float, Js>(S, M<float>::sq(ax * ax + ay * ay + az * az), same) -
-                    uee_val<float, Js>(S, M<float>::sq(bx * bx + by * by + bz * bz), same);
+              du += uee_diff<float, Js>(S, M<float>::sq(ax * ax + ay * ay + az * az),
+                                        M<float>::sq(bx * bx + by * by + bz * bz), same);
             }
           } else if (en_on && p < N + M_) {
             const float ax = nx - pnx[t], ay = ny - pny[t], az = nz - pnz[t];
             const float bx = ox - pnx[t], by = oy - pny[t], bz = oz - pnz[t];
-            du += tile_uen<Js>(S, M<float>::sq(ax * ax + ay * ay + az * az), pnZ[t], p - N) -
-                  tile_uen<Js>(S, M<float>::sq(bx * bx + by * by + bz * bz), pnZ[t], p - N);
+            const float rn = M<float>::sq(ax * ax + ay * ay + az * az), ro = M<float>::sq(bx * bx + by * by + bz * bz);
+            if constexpr (Js::no_series)  // -Z (r' - r) / ((1+b r')(1+b r)), one reciprocal
+              du += -pnZ[t] * (rn - ro) * M<float>::rcp((1.f + S.b_en * rn) * (1.f + S.b_en * ro));
+            else
+              du += tile_uen<Js>(S, rn, pnZ[t], p - N) - tile_uen<Js>(S, ro, pnZ[t], p - N);
           }
         }
         du = gsum(du);
