@@ -334,14 +334,29 @@ __device__ __forceinline__ bool sector_inverse(const DevSys<double>& S, int spin
   return sing;
 }
 
+// e-n Jastrow sum sum_I u_en(|r - R_I|) of a position
+template <class T, class Bs, class Js>
+__device__ __forceinline__ T en_sum(const DevSys<T>& S, T x, T y, T z) {
+  T u = 0;
+  Bs::nuclei(S, [&](int I) {
+    const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+    u += uen_val<T, Js>(S, M<T>::sq(dx * dx + dy * dy + dz * dz), I);
+  });
+  return u;
+}
+
 // One sector's single-electron Metropolis moves (Gaussian proposal, ratio
 // from the inverse column, Jastrow difference from the moved electron's
-// pairs, Sherman-Morrison row update on acceptance).
+// pairs, Sherman-Morrison row update on acceptance).  uen holds each
+// sector electron's e-n Jastrow sum at its current position (slot order,
+// rotating with the sector), so a move evaluates the e-n terms at the
+// proposal only.
 template <int NA, int NB, class T, class Bs = RtBasis, class Js = JsRt>
 __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[Cap<NA>::v][3],
                                              T ai[Cap<NA>::v][Cap<NA>::v], const T rb[Cap<NB>::v][3],
-                                             uint32_t w, uint32_t step, uint32_t k0, uint32_t k1, T sigma,
-                                             double& logdet, double& sgn, double& ju, int& acc) {
+                                             T uen[Cap<NA>::v], uint32_t w, uint32_t step, uint32_t k0,
+                                             uint32_t k1, T sigma, double& logdet, double& sgn, double& ju,
+                                             int& acc) {
 #pragma unroll 1
   for (int k = 0; k < NA; ++k) {
     const uint32_t e = (uint32_t)S.sector_elec[spin][k];
@@ -366,12 +381,11 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
     for (int j = 1; j < NA; ++j) pair(ra[j][0], ra[j][1], ra[j][2], true);
 #pragma unroll
     for (int j = 0; j < NB; ++j) pair(rb[j][0], rb[j][1], rb[j][2], false);
-    if (S.en_on)
-      Bs::nuclei(S, [&](int I) {
-        T ax = nx - S.nuc[I][0], ay = ny - S.nuc[I][1], az = nz - S.nuc[I][2];
-        T bx = ox - S.nuc[I][0], by = oy - S.nuc[I][1], bz = oz - S.nuc[I][2];
-        du += uen_diff<T, Js>(S, M<T>::sq(ax * ax + ay * ay + az * az), M<T>::sq(bx * bx + by * by + bz * bz), I);
-      });
+    T un = 0;
+    if (S.en_on) {
+      un = en_sum<T, Bs, Js>(S, nx, ny, nz);
+      du += un - uen[0];
+    }
     const T p = R * R * M<T>::ex(T(2) * du);
     if (U < p) {
       // Sherman-Morrison: A'^-1 = A^-1 - (A^-1 e_0)(phi^T A^-1 - e_0^T)/R
@@ -397,8 +411,10 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
       if (R < T(0)) sgn = -sgn;
       ju += (double)du;
       ++acc;
+      uen[0] = un;
     }
     rotate<NA>(ra, ai);
+    rotate_vec<NA>(uen);
   }
 }
 
@@ -491,11 +507,17 @@ __device__ __forceinline__ void sweep_block(const DevSys<T>& S, EnsView<T> E, Sw
   const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
   const uint32_t wid = (uint32_t)(A.walker0 + wl);
   const T sigma = (T)A.step_size;
+  T uen_u[Cap<NU>::v], uen_d[Cap<ND>::v];
+  const bool en_on = S.en_on;
+#pragma unroll
+  for (int k = 0; k < NU; ++k) uen_u[k] = en_on ? en_sum<T, Bs, Js>(S, R.ru[k][0], R.ru[k][1], R.ru[k][2]) : T(0);
+#pragma unroll
+  for (int k = 0; k < ND; ++k) uen_d[k] = en_on ? en_sum<T, Bs, Js>(S, R.rd[k][0], R.rd[k][1], R.rd[k][2]) : T(0);
   for (int s = 0; s < A.n_steps; ++s) {
     const uint32_t step = A.step0 + (uint32_t)s;
     int acc = 0;
-    sector_moves<NU, ND, T, Bs, Js>(S, 0, R.ru, R.au, R.rd, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
-    sector_moves<ND, NU, T, Bs, Js>(S, 1, R.rd, R.ad, R.ru, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
+    sector_moves<NU, ND, T, Bs, Js>(S, 0, R.ru, R.au, R.rd, uen_u, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
+    sector_moves<ND, NU, T, Bs, Js>(S, 1, R.rd, R.ad, R.ru, uen_d, wid, step, k0, k1, sigma, logdet, sgn, ju, acc);
     el = local_energy<NU, ND, T, Bs, Js>(S, R.ru, R.au, R.rd, R.ad, nullptr, nullptr, 0);
     if (el_trace && live) el_trace[(int64_t)s * A.trace_ld + w] = el;
     double e1 = live ? el : 0.0, e2 = live ? el * el : 0.0, a = live ? (double)acc : 0.0;
