@@ -128,6 +128,7 @@ extern "C" {
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double orc_u01(uint32_t w);
 void orc_normals3(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, double z[3]);
+void orc_move_draw(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, double z[3], double* u);
 
 /* linear algebra */
 int orc_lu_det_inv(int n, const double* a, double* det, double* inv, double* logabs, double* sgn);

@@ -54,6 +54,26 @@ void orc_normals3(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t
     z[2] = rad2 * cos(twopi * u4);
 }
 
+/* One single-electron move from one Philox block (the GPU's Rng::move):
+ * Box-Muller normals from the top 24 bits of the four words, the acceptance
+ * uniform from their low bytes. */
+void orc_move_draw(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, double z[3], double* u) {
+    uint32_t ctr[4] = {c0, c1, c2, c3};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    const double twopi = 6.283185307179586;
+    double v[4];
+    for (int i = 0; i < 4; ++i) v[i] = ((double)(o[i] >> 8) + 0.5) * 5.9604644775390625e-08;
+    double rad1 = sqrt(-2.0 * log(v[0]));
+    double rad2 = sqrt(-2.0 * log(v[2]));
+    z[0] = rad1 * cos(twopi * v[1]);
+    z[1] = rad1 * sin(twopi * v[1]);
+    z[2] = rad2 * cos(twopi * v[3]);
+    const uint32_t low = (o[0] & 0xFFu) | ((o[1] & 0xFFu) << 8) | ((o[2] & 0xFFu) << 16) | ((o[3] & 0xFFu) << 24);
+    *u = orc_u01(low);
+}
+
 /* ------------------------------------------------------------------------
  * LU with partial pivoting (row-major a[n*n]).  Stands in for nalgebra's
  * DMatrix::determinant and try_inverse (methane.rs:828-829,

@@ -174,13 +174,12 @@ void orc_single_electron_steps(const orc_system* s, int64_t W, double* pos, doub
             uint32_t st = step0 + (uint32_t)k;
             for (int qq = 0; qq < n; ++qq) {
                 const int e = order[qq];
-                double z[3], old[3];
-                orc_normals3(seed, (uint32_t)w, st, (uint32_t)(2 * e), ORC_RNG_SINGLE, z);
+                double z[3], old[3], u;
+                orc_move_draw(seed, (uint32_t)w, st, (uint32_t)e, ORC_RNG_SINGLE, z, &u);
                 for (int q = 0; q < 3; ++q) { old[q] = r[e][q]; r[e][q] += step * z[q]; }
                 double lpn, sgn;
                 orc_log_psi(s, (const double(*)[3])r, &lpn, &sgn);
                 double ratio = exp(2.0 * (lpn - lp));
-                double u = orc_u01(philox_w0(seed, (uint32_t)w, st, (uint32_t)(2 * e + 1), ORC_RNG_SINGLE));
                 if (u < ratio) {
                     lp = lpn;
                     ++acc;
@@ -248,8 +247,8 @@ void orc_dd_steps(const orc_system* s, int64_t W, double* pos, uint64_t seed, ui
             uint32_t st = step0 + (uint32_t)k;
             for (int qq = 0; qq < n; ++qq) {
                 const int e = order[qq];
-                double z[3], d[3];
-                orc_normals3(seed, (uint32_t)w, st, (uint32_t)(2 * e), ORC_RNG_DRIFT, z);
+                double z[3], d[3], u;
+                orc_move_draw(seed, (uint32_t)w, st, (uint32_t)e, ORC_RNG_DRIFT, z, &u);
                 dd_cap(F[e], tau, max_drift, d);
                 memcpy(t, r, sizeof(double) * 3 * n);
                 for (int c = 0; c < 3; ++c) t[e][c] = r[e][c] + d[c] + s2t * z[c];
@@ -268,7 +267,6 @@ void orc_dd_steps(const orc_system* s, int64_t W, double* pos, uint64_t seed, ui
                 const double ratio = psin / psi;
                 double p = ratio * ratio * (no_green ? 1.0 : exp(gb - gf));
                 if (p > 1.0) p = 1.0;
-                const double u = orc_u01(philox_w0(seed, (uint32_t)w, st, (uint32_t)(2 * e + 1), ORC_RNG_DRIFT));
                 if (u < p) {
                     memcpy(r, t, sizeof(double) * 3 * n);
                     psi = psin;

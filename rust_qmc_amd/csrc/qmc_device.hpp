@@ -58,6 +58,11 @@ __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint
 }
 
 __device__ __forceinline__ double u01d(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; }
+__device__ __forceinline__ double u24d(uint32_t w) { return ((double)(w >> 8) + 0.5) * 5.9604644775390625e-08; }
+// the low bytes of a Philox block, as one 32-bit word
+__device__ __forceinline__ uint32_t low_bytes(U4 o) {
+  return (o.x & 0xFFu) | ((o.y & 0xFFu) << 8) | ((o.z & 0xFFu) << 16) | ((o.w & 0xFFu) << 24);
+}
 __device__ __forceinline__ float u01f(uint32_t w) { return ((float)(w >> 8) + 0.5f) * 5.9604644775390625e-08f; }
 
 template <class T>
@@ -75,6 +80,20 @@ struct Rng<double> {
     z[2] = r2 * cos(twopi * u4);
   }
   static __device__ __forceinline__ double uniform(uint32_t w) { return u01d(w); }
+  // One single-electron move from one Philox block: three Box-Muller normals
+  // from the top 24 bits of the four words and the acceptance uniform from
+  // their low bytes (independent bits), as the FP32 path does.
+  static __device__ __forceinline__ void move(U4 o, double z[3], double& U) {
+    const double twopi = 6.283185307179586;
+    double u1 = u24d(o.x), u2 = u24d(o.y), u3 = u24d(o.z), u4 = u24d(o.w);
+    double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    double s, c;
+    sincos(twopi * u2, &s, &c);
+    z[0] = r1 * c;
+    z[1] = r1 * s;
+    z[2] = r2 * cos(twopi * u4);
+    U = u01d(low_bytes(o));
+  }
 };
 template <>
 struct Rng<float> {
@@ -87,6 +106,10 @@ struct Rng<float> {
     z[2] = r2 * __cosf(twopi * u4);
   }
   static __device__ __forceinline__ float uniform(uint32_t w) { return u01f(w); }
+  static __device__ __forceinline__ void move(U4 o, float z[3], float& U) {
+    normals3(o, z);  // u01f already takes the top 24 bits
+    U = u01f(low_bytes(o));
+  }
 };
 
 // ---------------------------------------------------------------------------

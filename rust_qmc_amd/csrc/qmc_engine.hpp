@@ -360,9 +360,8 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
 #pragma unroll 1
   for (int k = 0; k < NA; ++k) {
     const uint32_t e = (uint32_t)S.sector_elec[spin][k];
-    T z[3];
-    Rng<T>::normals3(philox(w, step, 2u * e, RNG_SINGLE, k0, k1), z);
-    const T U = Rng<T>::uniform(philox(w, step, 2u * e + 1u, RNG_SINGLE, k0, k1).x);
+    T z[3], U;
+    Rng<T>::move(philox(w, step, e, RNG_SINGLE, k0, k1), z, U);
     const T ox = ra[0][0], oy = ra[0][1], oz = ra[0][2];
     const T nx = ox + sigma * z[0], ny = oy + sigma * z[1], nz = oz + sigma * z[2];
     T phi[Cap<NA>::v];
@@ -627,8 +626,8 @@ __device__ __forceinline__ void sector_moves_dd(const DevSys<T>& S, int spin, T 
     cap_drift(T(2) * (gdx + gs * gux), T(2) * (gdy + gs * guy), T(2) * (gdz + gs * guz), tau, max_drift, dold);
     // Langevin proposal
     T z[3];
-    Rng<T>::normals3(philox(w, step, 2u * e, RNG_DRIFT, k0, k1), z);
-    const T U = Rng<T>::uniform(philox(w, step, 2u * e + 1u, RNG_DRIFT, k0, k1).x);
+    T U;
+    Rng<T>::move(philox(w, step, e, RNG_DRIFT, k0, k1), z, U);
     const T nx = ox + dold[0] + s2t * z[0], ny = oy + dold[1] + s2t * z[1], nz = oz + dold[2] + s2t * z[2];
     T phi[Cap<NA>::v];
     mo_values<NA, T, Bs>(S, spin, nx, ny, nz, phi);

@@ -190,8 +190,8 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<float>& S, EnsView
         const int e = S.sector_elec[spin][k];
         // A: proposal
         float z[3];
-        Rng<float>::normals3(philox(wid, step, 2u * (uint32_t)e, RNG_SINGLE, k0, k1), z);
-        const float U = Rng<float>::uniform(philox(wid, step, 2u * (uint32_t)e + 1u, RNG_SINGLE, k0, k1).x);
+        float U;
+        Rng<float>::move(philox(wid, step, (uint32_t)e, RNG_SINGLE, k0, k1), z, U);
         const float ox = pw[3 * e], oy = pw[3 * e + 1], oz = pw[3 * e + 2];
         const float nx = ox + sigma * z[0], ny = oy + sigma * z[1], nz = oz + sigma * z[2];
         // B: AO values of the proposal

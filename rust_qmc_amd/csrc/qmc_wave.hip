@@ -305,8 +305,8 @@ __global__ void __launch_bounds__(64) wave_sweep_kernel(const DevSys<T>* __restr
       for (int k = 0; k < n; ++k) {
         const int e = S.sector_elec[spin][k];
         T z[3];
-        Rng<T>::normals3(philox(wid, step, 2u * (uint32_t)e, RNG_SINGLE, k0, k1), z);
-        const T U = Rng<T>::uniform(philox(wid, step, 2u * (uint32_t)e + 1u, RNG_SINGLE, k0, k1).x);
+        T U;
+        Rng<T>::move(philox(wid, step, (uint32_t)e, RNG_SINGLE, k0, k1), z, U);
         const T ox = pos[3 * e], oy = pos[3 * e + 1], oz = pos[3 * e + 2];
         const T nx = ox + sigma * z[0], ny = oy + sigma * z[1], nz = oz + sigma * z[2];
         // AO values of the proposal: one shell per lane

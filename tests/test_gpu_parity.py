@@ -254,7 +254,10 @@ def test_compiled_shell_kernels_match_generic(Q, oracle, name, precision):
     for k in ("logabs", "eloc"):
         assert _close(spec[k], gen[k], tol), k
     assert np.allclose(pos_spec, ens2.positions(), rtol=0, atol=1e-9 if precision == "fp64" else 1e-5)
-    assert _close(tr_spec, tr_gen, 1e-10 if precision == "fp64" else 1e-4)
+    # FP32 E_L is a sum of terms of order 10 Ha that can cancel to ~0: compare
+    # it to 1e-4 of max(|E_L|, 1 Ha)
+    assert _close(tr_spec, tr_gen, 1e-10 if precision == "fp64" else 1e-4,
+                  scale=None if precision == "fp64" else np.maximum(np.abs(tr_gen), 1.0))
 
 
 @pytest.mark.parametrize("name", SYSTEMS)

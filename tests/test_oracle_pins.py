@@ -78,6 +78,41 @@ def test_box_muller_normals_statistics(oracle):
     assert np.all(np.abs(c - np.eye(3)) < 0.03)
 
 
+def test_move_draw_from_one_philox_block(oracle):
+    """One single-electron move per Philox block: Box-Muller normals from
+    the top 24 bits of the four words, the acceptance uniform from their
+    low bytes -- recomputed from the words, with the statistics of three
+    independent N(0,1) draws and an independent U(0,1)."""
+    L = oracle.lib()
+    import ctypes as C
+    L.orc_move_draw.restype = None
+    L.orc_move_draw.argtypes = [C.c_uint64] + [C.c_uint32] * 4 + [C.c_void_p, C.c_void_p]
+    z = np.zeros(3)
+    u = C.c_double()
+    o = _philox_py([123, 3, 5, 2], [7, 0])
+    v = [((x >> 8) + 0.5) / 2**24 for x in o]
+    low = (o[0] & 255) | ((o[1] & 255) << 8) | ((o[2] & 255) << 16) | ((o[3] & 255) << 24)
+    L.orc_move_draw(7, 123, 3, 5, 2, z.ctypes.data_as(C.c_void_p), C.byref(u))
+    r1, r2 = math.sqrt(-2 * math.log(v[0])), math.sqrt(-2 * math.log(v[2]))
+    assert z[0] == pytest.approx(r1 * math.cos(2 * math.pi * v[1]), rel=1e-14)
+    assert z[1] == pytest.approx(r1 * math.sin(2 * math.pi * v[1]), rel=1e-14)
+    assert z[2] == pytest.approx(r2 * math.cos(2 * math.pi * v[3]), rel=1e-14)
+    assert u.value == (low + 0.5) / 2**32
+    zs, us = [], []
+    for w in range(30000):
+        L.orc_move_draw(9, w, 4, 1, 2, z.ctypes.data_as(C.c_void_p), C.byref(u))
+        zs.append(z.copy())
+        us.append(u.value)
+    zs, us = np.array(zs), np.array(us)
+    assert abs(zs.mean()) < 0.02 and abs(zs.var() - 1.0) < 0.03
+    assert abs(us.mean() - 0.5) < 0.01 and abs(us.var() - 1 / 12) < 0.005
+    c = np.corrcoef(np.column_stack([zs, us]).T)
+    assert np.all(np.abs(c - np.eye(4)) < 0.03)
+    # uniform in every bin of 20 (chi-square, 19 degrees of freedom, p ~ 1e-4 bound)
+    h, _ = np.histogram(us, bins=20, range=(0, 1))
+    assert ((h - 1500.0) ** 2 / 1500.0).sum() < 50.0
+
+
 # --------------------------------------------------------------------------
 # LU (nalgebra determinant / try_inverse / lu().solve stand-in) vs LAPACK
 # --------------------------------------------------------------------------
