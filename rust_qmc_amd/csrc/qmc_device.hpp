@@ -22,6 +22,7 @@ struct M<double> {
   static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
   static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
   static __device__ __forceinline__ double lg(double x) { return log(x); }
+  static __device__ __forceinline__ double rsq(double x) { return 1.0 / sqrt(x); }
 };
 template <>
 struct M<float> {
@@ -29,6 +30,7 @@ struct M<float> {
   static __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
   static __device__ __forceinline__ float lg(float x) { return __logf(x); }
+  static __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 };
 
 // ---------------------------------------------------------------------------

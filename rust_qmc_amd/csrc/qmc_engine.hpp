@@ -143,8 +143,11 @@ __device__ __forceinline__ double local_energy_pairs(const DevSys<T>& S, T ru[Ca
 #pragma unroll
     for (int j = i + 1; j < N; ++j) {
       const T dx = px(i, 0) - px(j, 0), dy = px(i, 1) - px(j, 1), dz = px(i, 2) - px(j, 2);
-      const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-      const T ir = M<T>::rcp(r);
+      // 1/r from one reciprocal square root; r = 0 gives r = NaN, which the
+      // r >= 1e-10 guards reject like the coincident pair they are
+      const T d2 = dx * dx + dy * dy + dz * dz;
+      const T ir = M<T>::rsq(d2);
+      const T r = d2 * ir;
       T u1, lp;
       uee_d<T, Js>(S, r, ir, (i < NU) == (j < NU), u1, lp);
       const bool ok = r >= T(1e-10);
@@ -172,8 +175,9 @@ __device__ __forceinline__ double local_energy_pairs(const DevSys<T>& S, T ru[Ca
       T pen = 0;
       Bs::nuclei(S, [&](int I) {
         const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-        const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-        const T ir = M<T>::rcp(r);
+        const T d2 = dx * dx + dy * dy + dz * dz;
+        const T ir = M<T>::rsq(d2);
+        const T r = d2 * ir;
         pen += S.Z[I] * ir;
         if constexpr (EN) {
           T u1, lp;
