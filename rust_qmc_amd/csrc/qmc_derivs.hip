@@ -31,10 +31,13 @@ struct DerivLayout {
   __device__ __forceinline__ int g(int q, int c) const { return Qp + 3 * q + c; }
 };
 
-// per-thread accumulator rows in LDS: element q of lane t at lds[q*64 + t]
+// per-thread accumulator rows in LDS: element q of lane t at lds[q*64 + t],
+// in the arithmetic type (FP32 rows take half the LDS, so twice the
+// workgroups fit a CU); E_L is kept in FP64 outside the rows
+template <class T>
 struct Row {
-  double* base;
-  __device__ __forceinline__ double& operator[](int q) const { return base[q * 64]; }
+  T* base;
+  __device__ __forceinline__ T& operator[](int q) const { return base[q * 64]; }
 };
 
 // Pass 1 for the electron in slot 0 of sector A (combined index q):
@@ -42,13 +45,13 @@ struct Row {
 template <int NA, int NB, class T, class Bs>
 __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, int q, const T ra[Cap<NA>::v][3],
                                                 const T ai[Cap<NA>::v][Cap<NA>::v], const T rb[Cap<NB>::v][3],
-                                                const DerivLayout& L, Row row, double& pot) {
+                                                const DerivLayout& L, Row<T> row, double& pot) {
   const T x = ra[0][0], y = ra[0][1], z = ra[0][2];
   T gdx, gdy, gdz, ld;
   auto on_shell = [&](int c, T sx, T sy, T sz) {
-    row[L.om(c, 0)] -= (double)sx;
-    row[L.om(c, 1)] -= (double)sy;
-    row[L.om(c, 2)] -= (double)sz;
+    row[L.om(c, 0)] -= sx;
+    row[L.om(c, 1)] -= sy;
+    row[L.om(c, 2)] -= sz;
   };
   det_terms<NA, T, decltype(on_shell), Bs>(S, spin, x, y, z, ai, gdx, gdy, gdz, ld, on_shell);
   T gux = 0, guy = 0, guz = 0, lu = 0, pairsq = 0;
@@ -79,10 +82,10 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
     T ir = M<T>::rcp(r);
     pe -= (double)(S.Z[I] * ir);
     if (r > T(1e-10)) {  // methane.rs:1253
-      const double f3 = (double)(S.Z[I] * ir * ir * ir);
-      row[L.f(I, 0)] += f3 * (double)dx;
-      row[L.f(I, 1)] += f3 * (double)dy;
-      row[L.f(I, 2)] += f3 * (double)dz;
+      const T f3 = S.Z[I] * ir * ir * ir;
+      row[L.f(I, 0)] += f3 * dx;
+      row[L.f(I, 1)] += f3 * dy;
+      row[L.f(I, 2)] += f3 * dz;
     }
     if (S.en_on) {
       T u1, lp;
@@ -94,9 +97,9 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
         guz += f * dz;
         lu += lp;
         // d u_en / d R_I = -u1 (r_i - R_I)/r   (methane.rs:1071-1090)
-        row[L.om(I, 0)] -= (double)(f * dx);
-        row[L.om(I, 1)] -= (double)(f * dy);
-        row[L.om(I, 2)] -= (double)(f * dz);
+        row[L.om(I, 0)] -= f * dx;
+        row[L.om(I, 1)] -= f * dy;
+        row[L.om(I, 2)] -= f * dz;
       }
     }
   });
@@ -104,9 +107,9 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
   T gjx = gs * gux, gjy = gs * guy, gjz = gs * guz;
   T lj = (S.flags & F_J2_LAP_QUIRK) ? (lu + pairsq) : (lu + gux * gux + guy * guy + guz * guz);
   const T lap = ld + lj + T(2) * (gdx * gjx + gdy * gjy + gdz * gjz);
-  row[L.g(q, 0)] = (double)(gdx + gux);
-  row[L.g(q, 1)] = (double)(gdy + guy);
-  row[L.g(q, 2)] = (double)(gdz + guz);
+  row[L.g(q, 0)] = gdx + gux;
+  row[L.g(q, 1)] = gdy + guy;
+  row[L.g(q, 2)] = gdz + guz;
   pot += pe;
   return -0.5 * (double)lap;
 }
@@ -114,7 +117,7 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
 // Pass 2: unique pairs -> O_k and d E_L / d p_k (Pade Jastrow only)
 template <int NU, int ND, class T, class Bs>
 __device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>::v][3],
-                                          const T rd[Cap<ND>::v][3], const DerivLayout& L, Row row) {
+                                          const T rd[Cap<ND>::v][3], const DerivLayout& L, Row<T> row) {
   constexpr int N = NU + ND;
   auto pos = [&](int q, int c) -> T { return q < NU ? ru[q][c] : rd[q - NU][c]; };
   const int K = S.k_ee;
@@ -131,24 +134,23 @@ __device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>
       pd.init(r, S.b_ee, a, K, c);
       const int ia = same ? 3 : 2, c0 = 4 + (same ? K : 0);
       // O: d u / d b_ee = -(a + s') x^2, d u / d a = x, d u / d c_k = x^(k+2)
-      row[L.o(0)] += (double)(-pd.A1 * pd.x * pd.x);
-      row[L.o(ia)] += (double)pd.x;
+      row[L.o(0)] += -pd.A1 * pd.x * pd.x;
+      row[L.o(ia)] += pd.x;
       if (r < T(1e-10)) continue;
       const T ir = T(1) / r;
       // Delta = (G_i - G_j) . rhat_ij
-      const T dlt = (T)((row[L.g(i, 0)] - row[L.g(j, 0)]) * (double)(dx * ir) +
-                        (row[L.g(i, 1)] - row[L.g(j, 1)]) * (double)(dy * ir) +
-                        (row[L.g(i, 2)] - row[L.g(j, 2)]) * (double)(dz * ir));
+      const T dlt = (row[L.g(i, 0)] - row[L.g(j, 0)]) * (dx * ir) + (row[L.g(i, 1)] - row[L.g(j, 1)]) * (dy * ir) +
+                    (row[L.g(i, 2)] - row[L.g(j, 2)]) * (dz * ir);
       T du1, dl;
       pd.d_b(r, S.b_ee, du1, dl);
-      row[L.de(0)] += -(double)(dl + dlt * du1);
+      row[L.de(0)] += -(dl + dlt * du1);
       pd.d_a(r, S.b_ee, du1, dl);
-      row[L.de(ia)] += -(double)(dl + dlt * du1);
+      row[L.de(ia)] += -(dl + dlt * du1);
       T xk = T(1);
       for (int k = 0; k < K; ++k) {
-        row[L.o(c0 + k)] += (double)(xk * pd.x * pd.x);
+        row[L.o(c0 + k)] += xk * pd.x * pd.x;
         pd.d_c(k, r, S.b_ee, xk, du1, dl);
-        row[L.de(c0 + k)] += -(double)(dl + dlt * du1);
+        row[L.de(c0 + k)] += -(dl + dlt * du1);
         xk *= pd.x;
       }
     }
@@ -163,27 +165,26 @@ __device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>
       PairDerivs<T> pd;
       pd.init(r, S.b_en, -S.Z[I], Ke, c);
       // d u_en / d b_en = (Z - s') y^2 = -A1 y^2
-      row[L.o(1)] += (double)(-pd.A1 * pd.x * pd.x);
+      row[L.o(1)] += -pd.A1 * pd.x * pd.x;
       const int c0 = 4 + 2 * S.k_ee + sp * Ke;
       if (r < T(1e-10)) {
         T xk = T(1);
         for (int k = 0; k < Ke; ++k) {
-          row[L.o(c0 + k)] += (double)(xk * pd.x * pd.x);
+          row[L.o(c0 + k)] += xk * pd.x * pd.x;
           xk *= pd.x;
         }
         return;
       }
       const T ir = T(1) / r;
-      const T dlt = (T)(row[L.g(i, 0)] * (double)(dx * ir) + row[L.g(i, 1)] * (double)(dy * ir) +
-                        row[L.g(i, 2)] * (double)(dz * ir));
+      const T dlt = row[L.g(i, 0)] * (dx * ir) + row[L.g(i, 1)] * (dy * ir) + row[L.g(i, 2)] * (dz * ir);
       T du1, dl;
       pd.d_b(r, S.b_en, du1, dl);
-      row[L.de(1)] += -(double)(T(0.5) * dl + dlt * du1);
+      row[L.de(1)] += -(T(0.5) * dl + dlt * du1);
       T xk = T(1);
       for (int k = 0; k < Ke; ++k) {
-        row[L.o(c0 + k)] += (double)(xk * pd.x * pd.x);
+        row[L.o(c0 + k)] += xk * pd.x * pd.x;
         pd.d_c(k, r, S.b_en, xk, du1, dl);
-        row[L.de(c0 + k)] += -(double)(T(0.5) * dl + dlt * du1);
+        row[L.de(c0 + k)] += -(T(0.5) * dl + dlt * du1);
         xk *= pd.x;
       }
     });
@@ -192,15 +193,16 @@ __device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>
 template <int NU, int ND, class Bs, class T>
 __global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __restrict__ Sd,
                                                     const DevSys<T>* __restrict__ St, DerivArgs A) {
-  extern __shared__ double lds[];
+  extern __shared__ double lds_raw[];
+  T* lds = (T*)lds_raw;
   const DevSys<T>& S = *St;
   const int64_t w = (int64_t)blockIdx.x * 64 + threadIdx.x;
   const bool live = w < A.W;
   const int64_t wl = live ? w : A.W - 1;
   DerivLayout L{A.P, A.M, NU + ND, A.Qp};
-  Row row{lds + threadIdx.x};
+  Row<T> row{lds + threadIdx.x};
   const int nq = A.Qp + 3 * (NU + ND);
-  for (int q = 0; q < nq; ++q) row[q] = 0.0;
+  for (int q = 0; q < nq; ++q) row[q] = T(0);
   // positions in FP64, from AoS input or from the ensemble's SoA state
   WalkerRegs<NU, ND, double> D;
   constexpr int N = NU + ND;
@@ -249,11 +251,11 @@ __global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __rest
     kin += electron_pass<ND, NU, T, Bs>(S, 1, NU + k, R.rd, R.ad, R.ru, L, row, pot);
     rotate<ND>(R.rd, R.ad);
   }
-  row[L.e()] = kin + pot + S.vnn;
-  row[L.one()] = 1.0;
+  const double el = kin + pot + S.vnn;
+  row[L.one()] = T(1);
   if (S.jas_kind == JAS_PADE) pair_pass<NU, ND, T, Bs>(S, R.ru, R.rd, L, row);
   if (live)
-    for (int q = 0; q < A.Qp; ++q) A.Y[(int64_t)q * A.ldy + w] = row[q];
+    for (int q = 0; q < A.Qp; ++q) A.Y[(int64_t)q * A.ldy + w] = q == L.e() ? el : (double)row[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -317,7 +319,7 @@ __global__ void __launch_bounds__(256) gram_reduce_kernel(const double* __restri
 
 template <int NU, int ND, class Bs, class T>
 static hipError_t launch_derivs(const void* Sd, const void* St, DerivArgs A, hipStream_t st) {
-  const size_t shm = sizeof(double) * 64 * (A.Qp + 3 * (NU + ND));
+  const size_t shm = sizeof(T) * 64 * (A.Qp + 3 * (NU + ND));
   if (shm > 160 * 1024) return hipErrorInvalidValue;
   static bool attr_set = false;
   if (!attr_set) {
