@@ -279,11 +279,18 @@ __global__ void __launch_bounds__(64) gram_kernel(const double* __restrict__ Y, 
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
   const int fr = lane & 15, kk = lane >> 4;
+  // the next step's fragments are loaded before this step's MFMAs, so the
+  // loads overlap the matrix work instead of waiting in front of it
+  double fn[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) fn[b] = w0 + kk < w1 ? Y[(int64_t)(16 * b + fr) * ldy + w0 + kk] : 0.0;
   for (int64_t k0 = w0; k0 < w1; k0 += 4) {
-    const int64_t wk = k0 + kk;
     double f[NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) f[b] = wk < w1 ? Y[(int64_t)(16 * b + fr) * ldy + wk] : 0.0;
+    for (int b = 0; b < NB; ++b) f[b] = fn[b];
+    const int64_t wn = k0 + 4 + kk;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) fn[b] = wn < w1 ? Y[(int64_t)(16 * b + fr) * ldy + wn] : 0.0;
     int t = 0;
 #pragma unroll
     for (int I = 0; I < NB; ++I)
@@ -313,7 +320,8 @@ __global__ void __launch_bounds__(256) gram_reduce_kernel(const double* __restri
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= (int64_t)Qp * Qp) return;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * Qp * Qp + i];
+#pragma unroll 16
+  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * Qp * Qp + i];  // loads in flight together, same order
   G[i] = s;
 }
 
