@@ -24,7 +24,7 @@
 namespace qmc {
 
 template <int NU, int ND, int KS, class Js>
-__global__ void __launch_bounds__(TILE_T) tile_sweep_kernel(const DevSys<float>* __restrict__ Sp, EnsView<float> E,
+__global__ void __launch_bounds__(TILE_T, 2) tile_sweep_kernel(const DevSys<float>* __restrict__ Sp, EnsView<float> E,
                                                             TileArgs TA, SweepArgs A, double* __restrict__ partial,
                                                             double* __restrict__ el_trace) {
   extern __shared__ float lds_f[];
