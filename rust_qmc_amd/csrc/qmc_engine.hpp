@@ -210,7 +210,8 @@ __device__ __forceinline__ double local_energy_pairs(const DevSys<T>& S, T ru[Ca
   double kin = 0.0;
   auto sector = [&](auto na_tag, int spin, auto& ra, auto& ai, T* gxs, T* gys, T* gzs, T* ls) {
     constexpr int NA = decltype(na_tag)::value;
-#pragma unroll 1
+    // unrolled: the rotations become register renames
+#pragma unroll
     for (int k = 0; k < NA; ++k) {
       T gdx, gdy, gdz, ldd;
       det_terms<NA, T, NoShell, Bs>(S, spin, ra[0][0], ra[0][1], ra[0][2], ai, gdx, gdy, gdz, ldd, NoShell{});
