@@ -50,6 +50,16 @@ def test_isdmc_population_parity_fp64(Q, oracle, name, W):
     assert r.capacity == 256
 
 
+@pytest.mark.parametrize("name", ["h2", "ch4_sto3g"])
+def test_isdmc_matched_branching_parity_fp64(Q, oracle, name):
+    """The branching over 2 tau follows the oracle as the reference form does."""
+    g, o = _pair(Q, oracle, name)
+    dmc, r, ro = _run_both(Q, g, o, 5, 48, 10, 2, 0.01, matched=True)
+    assert np.array_equal(dmc.populations, ro["populations"])
+    assert _close(dmc.energies, ro["energies"], 1e-9)
+    assert r.acceptance_rate == pytest.approx(ro["acceptance_rate"], rel=1e-12)
+
+
 def test_isdmc_capacity_drops_match(Q, oracle):
     """Negative damping fills the capacity; the clones past it are dropped
     in source order exactly as the oracle drops them."""
