@@ -38,7 +38,7 @@ FLOP_PER_WM = 16338
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector FP64 / FP32
 # measured HBM traffic of one mixed-precision sweep launch at 65,536 walkers
 # and 16 sweeps per launch (22,532 KiB read + 23,296 KiB written)
-TRAFFIC_BYTES = (2 * 11356 + 23296) * 1024  # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB per launch
+TRAFFIC_BYTES = int((2 * 11485.5 + 23296) * 1024)  # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB per launch
 
 
 def run_mode(Q, precision, W, steps, warmup, spl, refresh):
