@@ -148,10 +148,23 @@ def run_isdmc(Q, precision, W, n_steps):
             "dropped": r.dropped, "acceptance": r.acceptance_rate}
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(seconds_target):
     """The reference algorithm (vmc.rs:95-119: all-electron proposal, full
     recomputation of psi, E_L with 8 sector LU per accepted move) on the CH4
-    workload, one host thread, timed on a bounded sample of walkers."""
+    workload, timed on a bounded sample of walkers: one host thread (the
+    reference is single-threaded on this path), and again with the walkers
+    spread over the host threads this process may use (at most 16)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
@@ -163,14 +176,26 @@ def cpu_baseline(seconds_target):
     W = 256
     steps = max(2, int(seconds_target / (per_wm * W)))
     sec, _ = o.mcmc_time_steps(W, 2, steps, 0.3, 0)
-    return {
+    cpu = _cpu_model()
+    out = {
         "value": W * steps / sec,
         "unit": "walker-moves/s",
         "cores": 1,
         "kind": "port",
         "sample": f"{W} walkers x {steps} steps (after 2 warm-up), CH4 STO-3G, oracle in reference-algorithm "
-                  f"mode, {platform.processor() or platform.machine()}, {sec:.1f} s",
+                  f"mode, {cpu}, {sec:.1f} s",
     }
+    T = min(16, len(os.sched_getaffinity(0)))
+    if T > 1:
+        Wm = 64 * T
+        sm = max(2, int(seconds_target * T / (per_wm * Wm) / 2))
+        secm, _ = o.mcmc_time_steps_mt(Wm, 2, sm, 0.3, 0, T)
+        out["all_cores"] = {
+            "value": Wm * sm / secm,
+            "cores": T,
+            "sample": f"{Wm} walkers x {sm} steps (after 2 warm-up) on {T} threads, {secm:.1f} s",
+        }
+    return out
 
 
 def main():

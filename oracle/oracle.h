@@ -218,6 +218,8 @@ void orc_mcmc_run(const orc_system* s, const orc_mcmc_params* p, uint64_t seed, 
                   orc_mcmc_results* res);
 void orc_mcmc_time_steps(const orc_system* s, int64_t W, int64_t warm, int64_t steps, double step,
                          uint64_t seed, double* seconds, double* mean_e);
+void orc_mcmc_time_steps_mt(const orc_system* s, int64_t W, int64_t warm, int64_t steps, double step,
+                            uint64_t seed, int threads, double* seconds, double* mean_e);
 void orc_single_electron_steps(const orc_system* s, int64_t W, double* pos, double* logpsi,
                                uint64_t seed, uint32_t step0, int nsteps, double step,
                                double* eloc, int64_t* accepted);

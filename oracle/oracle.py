@@ -105,6 +105,7 @@ def lib():
             "orc_blocking_error": (_d, [_p, _i64, _d]),
             "orc_mcmc_run": (None, [_p, _p, _u64, _p, _p]),
             "orc_mcmc_time_steps": (None, [_p, _i64, _i64, _i64, _d, _u64, _p, _p]),
+            "orc_mcmc_time_steps_mt": (None, [_p, _i64, _i64, _i64, _d, _u64, _i, _p, _p]),
             "orc_single_electron_steps": (None, [_p, _i64, _p, _p, _u64, _u32, _i, _d, _p, _p]),
             "orc_dd_steps": (None, [_p, _i64, _p, _u64, _u32, _i, _d, _d, _i, _p, _p]),
             "orc_all_electron_steps": (None, [_p, _i64, _p, _p, _p, _u64, _u32, _i, _d, _p]),
@@ -343,6 +344,12 @@ class System:
         sec = C.c_double()
         me = C.c_double()
         lib().orc_mcmc_time_steps(self._h, W, warm, steps, step, seed, C.byref(sec), C.byref(me))
+        return sec.value, me.value
+
+    def mcmc_time_steps_mt(self, W, warm, steps, step, seed=0, threads=1):
+        sec = C.c_double()
+        me = C.c_double()
+        lib().orc_mcmc_time_steps_mt(self._h, W, warm, steps, step, seed, threads, C.byref(sec), C.byref(me))
         return sec.value, me.value
 
     def single_electron_steps(self, pos, seed, step0, nsteps, step):

@@ -149,6 +149,40 @@ void orc_mcmc_time_steps(const orc_system* s, int64_t W, int64_t warm, int64_t s
     free(st);
 }
 
+/* The same workload on `threads` host threads: the walkers' chains are
+ * independent (every draw is keyed by walker and step), so each thread runs
+ * whole chains of its share of the walkers, warm-up included; the timed
+ * region is the chains after the warm-up, bounded by barriers. */
+void orc_mcmc_time_steps_mt(const orc_system* s, int64_t W, int64_t warm, int64_t steps, double step,
+                            uint64_t seed, int threads, double* seconds, double* mean_e) {
+    mcmc_state* st = (mcmc_state*)malloc(sizeof(mcmc_state) * W);
+    struct timespec t0, t1;
+    double m = 0.0;
+#pragma omp parallel num_threads(threads)
+    {
+#pragma omp for schedule(static)
+        for (int64_t w = 0; w < W; ++w) {
+            orc_initialize(s, seed, (uint32_t)w, st[w].r);
+            st[w].psi = orc_evaluate(s, (const double(*)[3])st[w].r);
+            st[w].energy = orc_local_energy(s, (const double(*)[3])st[w].r);
+            for (int64_t k = 0; k < warm; ++k) metropolis_step(s, &st[w], step, seed, (uint32_t)w, (uint32_t)k);
+        }
+#pragma omp single
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+#pragma omp for schedule(static) reduction(+ : m)
+        for (int64_t w = 0; w < W; ++w)
+            for (int64_t k = 0; k < steps; ++k) {
+                metropolis_step(s, &st[w], step, seed, (uint32_t)w, (uint32_t)(warm + k));
+                m += st[w].energy;
+            }
+#pragma omp single
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+    }
+    *seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+    *mean_e = m / (double)(W * steps);
+    free(st);
+}
+
 /* Single-electron Metropolis sweeps, the move scheme of the GPU sampler:
  * for each electron e in sector order (spin-up electrons by increasing
  * index, then spin-down), r_e' = r_e + step * z with z from

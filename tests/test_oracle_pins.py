@@ -235,6 +235,17 @@ def test_h2_vmc_energy_matches_reference_readme(oracle):
     assert abs(eq.mean() - e_ref) < 3 * err + 5e-4
 
 
+def test_threaded_timing_runs_the_same_chains(oracle):
+    """The CPU baseline's threaded leg runs the same per-walker chains as the
+    one-thread leg (draws keyed by walker and step), so the mean E_L over
+    the timed steps agrees up to the order of the sum."""
+    o = oracle.System.methane_sto3g(1.5, 2.0, 2.06, 0)
+    _, m1 = o.mcmc_time_steps(12, 2, 3, 0.3, 7)
+    _, m4 = o.mcmc_time_steps_mt(12, 2, 3, 0.3, 7, 4)
+    assert math.isfinite(m1) and m1 < 0
+    assert abs(m1 - m4) <= 1e-12 * abs(m1)
+
+
 # --------------------------------------------------------------------------
 # Finite-difference checks (traits.rs:62-90 method), tightened
 # --------------------------------------------------------------------------
