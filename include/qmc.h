@@ -61,6 +61,15 @@ extern "C" {
 #define QMC_FLAGS_REFERENCE \
   (QMC_FLAG_J2_GRAD_QUIRK | QMC_FLAG_J2_LAP_QUIRK | QMC_FLAG_HF_NN_REFSIGN | QMC_FLAG_SINGULAR_IDENTITY)
 
+/* ---- MO coefficients that follow the geometry (qmc_set_nuclei) ----
+ * QMC_MO_BENZENE_CH: MOs 9-14 of BenzeneGTO are the C-H sigma bonds, whose
+ * C 2p part points from carbon n to hydrogen n (benzene.rs:345-381); they
+ * are rebuilt whenever the nuclei move, as BenzeneGTO::set_nuclei does
+ * (benzene.rs:786-809).  The nuclear gradient Omega includes the
+ * coefficients' dependence on the geometry. */
+#define QMC_MO_FIXED 0
+#define QMC_MO_BENZENE_CH 1
+
 /* ---- system description ---- */
 #define QMC_AO_GTO 0      /* CGTO, methane.rs:488-591 (unnormalised primitives) */
 #define QMC_AO_STO 1      /* STO radial sum c r^p exp(-zeta r), slater.rs:48-145 */
@@ -104,6 +113,7 @@ typedef struct {
   const int32_t* init_nuc;  /* [n_elec] nucleus each electron is initialised around */
   double init_sigma;        /* width of the initial Gaussian cloud */
   uint32_t flags;           /* QMC_FLAG_* */
+  int32_t mo_rule;          /* QMC_MO_*: rows the rule owns are recomputed from nuc_pos */
 } qmc_system_desc;
 
 typedef struct qmc_system qmc_system;
@@ -157,7 +167,8 @@ int qmc_benzene_gto_create(double b_ee, double b_en, double cc_bond, double ch_b
  *    "mo_up" [[C_ma]...] (rows = MOs), optional "mo_dn", "jastrow" {"kind":
  *    "none"|"pair"|"pade", "F", "b_ee", "b_en", "a_anti", "a_para",
  *    "c_anti", "c_para", "species", "d_en" [[...] per species]},
- *    "init_nuc", "init_sigma".
+ *    "init_nuc", "init_sigma", "mo_rule": "fixed"|"benzene_ch" (the MO rows
+ *    the rule owns are recomputed from "nuclei").
  * "flags" is an integer or a list of "j2_grad_quirk", "j2_lap_quirk",
  * "hf_nn_refsign", "singular_identity", "reference".  len = 0 means a
  * NUL-terminated text.  Errors name the field or the byte offset. */

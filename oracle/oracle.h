@@ -118,7 +118,12 @@ typedef struct {
     int n_mo;
     double mo[2][ORC_MAXMO][ORC_MAXAO];  /* [spin: 0 up, 1 down][mo][ao] */
     orc_jastrow3 j3;
+    /* MO coefficients that follow the geometry on set_nuclei:
+     * ORC_MO_BENZENE_CH = the C-H sigma MOs of BenzeneGTO (benzene.rs:350, :797) */
+    int mo_rule;
 } orc_system;
+
+enum { ORC_MO_FIXED = 0, ORC_MO_BENZENE_CH = 1 };
 
 #ifdef __cplusplus
 extern "C" {

@@ -325,6 +325,27 @@ orc_system* orc_methane_sto3g_new(double b_ee, double b_en, double ch_bond, uint
     return s;
 }
 
+/* benzene.rs:345-381: MOs 9-14, C-H sigma bonds; the C 2p part points from
+ * carbon n to hydrogen n, so it is rebuilt whenever the nuclei move
+ * (BenzeneGTO::set_nuclei, benzene.rs:786-809). */
+static void benzene_ch_mos(double mo[21][66], const orc_system* s) {
+    for (int n = 0; n < 6; ++n) {
+        int m = 9 + n;
+        double dir[3] = {s->nuc[6 + n][0] - s->nuc[n][0], s->nuc[6 + n][1] - s->nuc[n][1],
+                         s->nuc[6 + n][2] - s->nuc[n][2]};
+        double dn = norm3(dir);
+        for (int k = 0; k < 3; ++k) dir[k] /= dn;
+        mo[m][9 * n + 1] = 0.15;
+        mo[m][9 * n + 5] = 0.30;
+        for (int k = 0; k < 3; ++k) {
+            mo[m][9 * n + 2 + k] = 0.20 * dir[k];
+            mo[m][9 * n + 6 + k] = 0.55 * dir[k];
+        }
+        mo[m][54 + 2 * n] = 0.25;
+        mo[m][54 + 2 * n + 1] = 0.15;
+    }
+}
+
 /* benzene.rs:71-112, geometry :119-140, basis :152-245, MOs :262-470 */
 orc_system* orc_benzene_new(double b_ee, double b_en, double cc_bond, double ch_bond) {
     orc_system* s = sys_alloc(ORC_SYS_GTO);
@@ -387,22 +408,8 @@ orc_system* orc_benzene_new(double b_ee, double b_en, double cc_bond, double ch_
         mo[m][CO(n) + 7] = 0.55 * w;
     }
     ++m;
-    for (int n = 0; n < 6; ++n, ++m) {
-        double dir[3] = {s->nuc[6 + n][0] - s->nuc[n][0], s->nuc[6 + n][1] - s->nuc[n][1],
-                         s->nuc[6 + n][2] - s->nuc[n][2]};
-        double dn = norm3(dir);
-        for (int k = 0; k < 3; ++k) dir[k] /= dn;
-        mo[m][CO(n) + 1] = 0.15;
-        mo[m][CO(n) + 5] = 0.30;
-        mo[m][CO(n) + 2] = 0.20 * dir[0];
-        mo[m][CO(n) + 3] = 0.20 * dir[1];
-        mo[m][CO(n) + 4] = 0.20 * dir[2];
-        mo[m][CO(n) + 6] = 0.55 * dir[0];
-        mo[m][CO(n) + 7] = 0.55 * dir[1];
-        mo[m][CO(n) + 8] = 0.55 * dir[2];
-        mo[m][HO(n)] = 0.25;
-        mo[m][HO(n) + 1] = 0.15;
-    }
+    benzene_ch_mos(mo, s);
+    m += 6;
     {
         double cpz = 1.0 / sqrt(6.0);
         for (int n = 0; n < 6; ++n) { mo[m][CO(n) + 4] = 0.30 * cpz; mo[m][CO(n) + 8] = 0.70 * cpz; }
@@ -450,6 +457,7 @@ orc_system* orc_benzene_new(double b_ee, double b_en, double cc_bond, double ch_
         for (int q = 0; q < 21; ++q)
             for (int a = 0; a < 66; ++a) s->mo[sp][q][a] = mo[q][a];
     jastrow3_init(s, b_ee, b_en);
+    s->mo_rule = ORC_MO_BENZENE_CH;
     return s;
 }
 
@@ -470,6 +478,15 @@ void orc_system_set_nuclei(orc_system* s, const double* R) {
             for (int k = 0; k < 3; ++k) s->ao[a].center[k] = R[3 * s->ao_nuc[a] + k];
         for (int I = 0; I < s->n_nuc; ++I)
             for (int k = 0; k < 3; ++k) s->j3.nuc[I][k] = R[3 * I + k];
+        if (s->mo_rule == ORC_MO_BENZENE_CH) {
+            double mo[21][66];
+            for (int q = 0; q < 21; ++q)
+                for (int a = 0; a < 66; ++a) mo[q][a] = s->mo[0][q][a];
+            benzene_ch_mos(mo, s);
+            for (int sp = 0; sp < 2; ++sp)
+                for (int q = 9; q < 15; ++q)
+                    for (int a = 0; a < 66; ++a) s->mo[sp][q][a] = mo[q][a];
+        }
     }
 }
 
@@ -908,7 +925,7 @@ void orc_hf_force(const orc_system* s, const double (*r)[3], double (*F)[3]) {
 }
 
 void orc_wfn_nuclear_gradient(const orc_system* s, const double (*r)[3], double (*Om)[3]) {
-    if (s->kind == ORC_SYS_GTO) { /* methane.rs:1096-1101 */
+    if (s->kind == ORC_SYS_GTO && s->mo_rule == ORC_MO_FIXED) { /* methane.rs:1096-1101 */
         for (int I = 0; I < s->n_nuc; ++I) {
             double a[3], b[3];
             gto_slater_nuclear_gradient(s, r, I, a);
@@ -917,7 +934,9 @@ void orc_wfn_nuclear_gradient(const orc_system* s, const double (*r)[3], double 
         }
         return;
     }
-    /* hydrogen.rs:260-290: central differences of ln|psi|, h = 1e-5 */
+    /* hydrogen.rs:260-290: central differences of ln|psi|, h = 1e-5.  Also the
+     * definition used for BenzeneGTO, whose MOs follow the geometry and for
+     * which the reference has no nuclear gradient (the trait default panics). */
     const double h = 1e-5;
     orc_system* w = (orc_system*)malloc(sizeof(orc_system));
     double R0[3 * ORC_MAXN];
@@ -930,13 +949,14 @@ void orc_wfn_nuclear_gradient(const orc_system* s, const double (*r)[3], double 
             memcpy(R, R0, sizeof(R0));
             R[3 * I + k] += h;
             orc_system_set_nuclei(w, R);
-            double pf = fabs(orc_evaluate(w, r));
+            double lf, sf, lb, sb;
+            orc_log_psi(w, r, &lf, &sf); /* ln|psi|: benzene's psi underflows a double */
             memcpy(w, s, sizeof(orc_system));
             memcpy(R, R0, sizeof(R0));
             R[3 * I + k] -= h;
             orc_system_set_nuclei(w, R);
-            double pb = fabs(orc_evaluate(w, r));
-            Om[I][k] = (pf > 1e-30 && pb > 1e-30) ? (log(pf) - log(pb)) / (2.0 * h) : 0.0;
+            orc_log_psi(w, r, &lb, &sb);
+            Om[I][k] = (sf != 0.0 && sb != 0.0 && lf > -1e300 && lb > -1e300) ? (lf - lb) / (2.0 * h) : 0.0;
         }
     free(w);
 }

@@ -102,7 +102,7 @@ class System:
             b_en=desc.get("b_en", 0.0), a_anti=desc.get("a_anti", 0.5), a_para=desc.get("a_para", 0.25),
             k_ee=0, k_en=0, c_anti=None, c_para=None, n_species=0, species=None, d_en=None,
             init_nuc=arr(desc.get("init_nuc", [0] * len(desc["spins"])), np.int32),
-            init_sigma=desc.get("init_sigma", 1.0), flags=desc.get("flags", 0))
+            init_sigma=desc.get("init_sigma", 1.0), flags=desc.get("flags", 0), mo_rule=desc.get("mo_rule", 0))
         h = cls._new(lib().qmc_system_create, C.byref(d), _PREC[precision])
         return cls(h, precision)
 

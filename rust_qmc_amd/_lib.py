@@ -72,7 +72,7 @@ class QmcSystemDesc(C.Structure):
         ("n_mo", _i32), ("mo_up", _p), ("mo_dn", _p), ("jastrow_kind", _i32), ("jas_F", _d),
         ("b_ee", _d), ("b_en", _d), ("a_anti", _d), ("a_para", _d), ("k_ee", _i32), ("k_en", _i32),
         ("c_anti", _p), ("c_para", _p), ("n_species", _i32), ("species", _p), ("d_en", _p),
-        ("init_nuc", _p), ("init_sigma", _d), ("flags", _u32),
+        ("init_nuc", _p), ("init_sigma", _d), ("flags", _u32), ("mo_rule", _i32),
     ]
 
 

@@ -17,6 +17,8 @@
 #include "qmc_derivs.hpp"
 
 #include "qmc_engine.hpp"
+#include "qmc_wave.hpp"
+#include "qmc_wave_dev.hpp"
 
 namespace qmc {
 
@@ -114,79 +116,93 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
   return -0.5 * (double)lap;
 }
 
+// O_k and d E_L / d p_k of one electron pair (Pade Jastrow only); (dx, dy,
+// dz) = r_i - r_j and gd(c) = G_i[c] - G_j[c], the difference of the two
+// electrons' d ln psi / d r (read only when the pair is apart).
+template <class T, class RowT, class GD>
+__device__ __forceinline__ void ee_pair_derivs(const DevSys<T>& S, T dx, T dy, T dz, bool same, const DerivLayout& L,
+                                               RowT row, GD gd) {
+  const int K = S.k_ee;
+  T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+  const T a = same ? S.a_para : S.a_anti;
+  const T* c = same ? S.c_para : S.c_anti;
+  PairDerivs<T> pd;
+  pd.init(r, S.b_ee, a, K, c);
+  const int ia = same ? 3 : 2, c0 = 4 + (same ? K : 0);
+  // O: d u / d b_ee = -(a + s') x^2, d u / d a = x, d u / d c_k = x^(k+2)
+  row[L.o(0)] += -pd.A1 * pd.x * pd.x;
+  row[L.o(ia)] += pd.x;
+  if (r < T(1e-10)) return;
+  const T ir = T(1) / r;
+  // Delta = (G_i - G_j) . rhat_ij
+  const T dlt = gd(0) * (dx * ir) + gd(1) * (dy * ir) + gd(2) * (dz * ir);
+  T du1, dl;
+  pd.d_b(r, S.b_ee, du1, dl);
+  row[L.de(0)] += -(dl + dlt * du1);
+  pd.d_a(r, S.b_ee, du1, dl);
+  row[L.de(ia)] += -(dl + dlt * du1);
+  T xk = T(1);
+  for (int k = 0; k < K; ++k) {
+    row[L.o(c0 + k)] += xk * pd.x * pd.x;
+    pd.d_c(k, r, S.b_ee, xk, du1, dl);
+    row[L.de(c0 + k)] += -(dl + dlt * du1);
+    xk *= pd.x;
+  }
+}
+
+// The same for electron i and nucleus I; (dx, dy, dz) = r_i - R_I and
+// g(c) = G_i[c].
+template <class T, class RowT, class GI>
+__device__ __forceinline__ void en_pair_derivs(const DevSys<T>& S, int I, T dx, T dy, T dz, const DerivLayout& L,
+                                               RowT row, GI g) {
+  const int Ke = S.k_en;
+  T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+  const int sp = S.species[I];
+  const T* c = S.d_en[sp];
+  PairDerivs<T> pd;
+  pd.init(r, S.b_en, -S.Z[I], Ke, c);
+  // d u_en / d b_en = (Z - s') y^2 = -A1 y^2
+  row[L.o(1)] += -pd.A1 * pd.x * pd.x;
+  const int c0 = 4 + 2 * S.k_ee + sp * Ke;
+  if (r < T(1e-10)) {
+    T xk = T(1);
+    for (int k = 0; k < Ke; ++k) {
+      row[L.o(c0 + k)] += xk * pd.x * pd.x;
+      xk *= pd.x;
+    }
+    return;
+  }
+  const T ir = T(1) / r;
+  const T dlt = g(0) * (dx * ir) + g(1) * (dy * ir) + g(2) * (dz * ir);
+  T du1, dl;
+  pd.d_b(r, S.b_en, du1, dl);
+  row[L.de(1)] += -(T(0.5) * dl + dlt * du1);
+  T xk = T(1);
+  for (int k = 0; k < Ke; ++k) {
+    row[L.o(c0 + k)] += xk * pd.x * pd.x;
+    pd.d_c(k, r, S.b_en, xk, du1, dl);
+    row[L.de(c0 + k)] += -(T(0.5) * dl + dlt * du1);
+    xk *= pd.x;
+  }
+}
+
 // Pass 2: unique pairs -> O_k and d E_L / d p_k (Pade Jastrow only)
 template <int NU, int ND, class T, class Bs>
 __device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>::v][3],
                                           const T rd[Cap<ND>::v][3], const DerivLayout& L, Row<T> row) {
   constexpr int N = NU + ND;
   auto pos = [&](int q, int c) -> T { return q < NU ? ru[q][c] : rd[q - NU][c]; };
-  const int K = S.k_ee;
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = i + 1; j < N; ++j) {
-      const bool same = (i < NU) == (j < NU);
-      T dx = pos(i, 0) - pos(j, 0), dy = pos(i, 1) - pos(j, 1), dz = pos(i, 2) - pos(j, 2);
-      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-      const T a = same ? S.a_para : S.a_anti;
-      const T* c = same ? S.c_para : S.c_anti;
-      PairDerivs<T> pd;
-      pd.init(r, S.b_ee, a, K, c);
-      const int ia = same ? 3 : 2, c0 = 4 + (same ? K : 0);
-      // O: d u / d b_ee = -(a + s') x^2, d u / d a = x, d u / d c_k = x^(k+2)
-      row[L.o(0)] += -pd.A1 * pd.x * pd.x;
-      row[L.o(ia)] += pd.x;
-      if (r < T(1e-10)) continue;
-      const T ir = T(1) / r;
-      // Delta = (G_i - G_j) . rhat_ij
-      const T dlt = (row[L.g(i, 0)] - row[L.g(j, 0)]) * (dx * ir) + (row[L.g(i, 1)] - row[L.g(j, 1)]) * (dy * ir) +
-                    (row[L.g(i, 2)] - row[L.g(j, 2)]) * (dz * ir);
-      T du1, dl;
-      pd.d_b(r, S.b_ee, du1, dl);
-      row[L.de(0)] += -(dl + dlt * du1);
-      pd.d_a(r, S.b_ee, du1, dl);
-      row[L.de(ia)] += -(dl + dlt * du1);
-      T xk = T(1);
-      for (int k = 0; k < K; ++k) {
-        row[L.o(c0 + k)] += xk * pd.x * pd.x;
-        pd.d_c(k, r, S.b_ee, xk, du1, dl);
-        row[L.de(c0 + k)] += -(dl + dlt * du1);
-        xk *= pd.x;
-      }
-    }
-  const int Ke = S.k_en;
+    for (int j = i + 1; j < N; ++j)
+      ee_pair_derivs(S, pos(i, 0) - pos(j, 0), pos(i, 1) - pos(j, 1), pos(i, 2) - pos(j, 2), (i < NU) == (j < NU), L,
+                     row, [=](int c) { return row[L.g(i, c)] - row[L.g(j, c)]; });
 #pragma unroll
   for (int i = 0; i < N; ++i)
     Bs::nuclei(S, [&](int I) {
-      T dx = pos(i, 0) - S.nuc[I][0], dy = pos(i, 1) - S.nuc[I][1], dz = pos(i, 2) - S.nuc[I][2];
-      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-      const int sp = S.species[I];
-      const T* c = S.d_en[sp];
-      PairDerivs<T> pd;
-      pd.init(r, S.b_en, -S.Z[I], Ke, c);
-      // d u_en / d b_en = (Z - s') y^2 = -A1 y^2
-      row[L.o(1)] += -pd.A1 * pd.x * pd.x;
-      const int c0 = 4 + 2 * S.k_ee + sp * Ke;
-      if (r < T(1e-10)) {
-        T xk = T(1);
-        for (int k = 0; k < Ke; ++k) {
-          row[L.o(c0 + k)] += xk * pd.x * pd.x;
-          xk *= pd.x;
-        }
-        return;
-      }
-      const T ir = T(1) / r;
-      const T dlt = row[L.g(i, 0)] * (dx * ir) + row[L.g(i, 1)] * (dy * ir) + row[L.g(i, 2)] * (dz * ir);
-      T du1, dl;
-      pd.d_b(r, S.b_en, du1, dl);
-      row[L.de(1)] += -(T(0.5) * dl + dlt * du1);
-      T xk = T(1);
-      for (int k = 0; k < Ke; ++k) {
-        row[L.o(c0 + k)] += xk * pd.x * pd.x;
-        pd.d_c(k, r, S.b_en, xk, du1, dl);
-        row[L.de(c0 + k)] += -(T(0.5) * dl + dlt * du1);
-        xk *= pd.x;
-      }
+      en_pair_derivs(S, I, pos(i, 0) - S.nuc[I][0], pos(i, 1) - S.nuc[I][1], pos(i, 2) - S.nuc[I][2], L, row,
+                     [=](int c) { return row[L.g(i, c)]; });
     });
 }
 
@@ -257,6 +273,200 @@ __global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __rest
   if (live)
     for (int q = 0; q < A.Qp; ++q) A.Y[(int64_t)q * A.ldy + w] = q == L.e() ? el : (double)row[q];
 }
+
+// ---------------------------------------------------------------------------
+// The wave version: one wavefront per walker, for shapes without a register
+// kernel (benzene: 21 + 21 electrons, 66 AOs).  The walker's positions and
+// inverses sit in LDS (the inverses from the FP64 Gauss-Jordan build, cast
+// to the arithmetic type).  Pass 1 puts electron i on lane i (determinant
+// and Jastrow gradients, E_L terms, HF force, Omega, and the Omega terms of
+// MO coefficients that follow the geometry); pass 2 spreads the electron
+// pairs and electron-nucleus pairs over the lanes (O_k, dE_L/dp_k).  Every
+// lane accumulates into its own LDS row (stride Qp + 1, so the rows' banks
+// differ); the rows are summed in lane order at the end.
+// ---------------------------------------------------------------------------
+template <class T>
+struct WRow {
+  T* base;
+  __device__ __forceinline__ T& operator[](int q) const { return base[q]; }
+};
+
+template <class T>
+__global__ void __launch_bounds__(64) wave_derivs_kernel(const DevSys<double>* __restrict__ Sd,
+                                                         const DevSys<T>* __restrict__ St, DerivArgs A, WaveLayout Y) {
+  extern __shared__ double lds_d[];
+  const DevSys<T>& S = *St;
+  const int64_t w = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int N = Y.N, Mn = A.M, rs = A.Qp + 1;
+  DerivLayout Ly{A.P, A.M, N, A.Qp};
+  T* L = (T*)lds_d;
+  T* G = L + Y.chi();  // d ln psi / d r_i, [N][3]
+  T* rows = G + 3 * N; // [64][rs]
+  double* pd = lds_d + ((size_t)(Y.chi() + 3 * N + 64 * rs) * sizeof(T) + 7) / 8;  // FP64 positions [N][3]
+  double* a = pd + 3 * N;
+  double* inv = a + Y.nmax * Y.nmax;
+  for (int q = lane; q < 3 * N; q += 64) {
+    const double v = A.pos_aos ? A.pos_aos[w * 3 * N + q]
+                               : (A.pos64 ? A.pos64[(int64_t)q * A.ld + w] : (double)A.pos32[(int64_t)q * A.ld + w]);
+    pd[q] = v;
+    L[Y.pos() + q] = (T)v;
+  }
+  for (int q = lane; q < 64 * rs; q += 64) rows[q] = T(0);
+  __syncthreads();
+  for (int spin = 0; spin < 2; ++spin) {
+    const int n = spin ? Y.nd : Y.nu;
+    double la, sg;
+    wave_sector_inverse(*Sd, spin, pd, n, a, inv, la, sg);
+    for (int q = lane; q < n * n; q += 64) L[Y.ainv(spin) + q] = (T)inv[q];
+    __syncthreads();
+  }
+  WRow<T> row{rows + lane * rs};
+  double kin = 0.0, pot = 0.0;
+  for (int i = lane; i < N; i += 64) {
+    const int spin = S.elec_spin[i], k = S.elec_slot[i];
+    const int n = spin ? Y.nd : Y.nu;
+    const T* ai = L + Y.ainv(spin);
+    const T x = L[3 * i], y = L[3 * i + 1], z = L[3 * i + 2];
+    auto wgt = [&](int ao) {
+      T v = 0;
+      for (int m = 0; m < n; ++m) v += S.moT[spin][ao][m] * ai[m * n + k];
+      return v;
+    };
+    auto on_shell = [&](int c, T sx, T sy, T sz) {
+      row[Ly.om(c, 0)] -= sx;
+      row[Ly.om(c, 1)] -= sy;
+      row[Ly.om(c, 2)] -= sz;
+    };
+    T gdx, gdy, gdz, ld;
+    det_terms_w<T, decltype(wgt), decltype(on_shell), RtBasis>(S, x, y, z, wgt, gdx, gdy, gdz, ld, on_shell);
+    T gux = 0, guy = 0, guz = 0, lu = 0, pairsq = 0;
+    double pe = 0.0;
+    for (int j = 0; j < N; ++j) {
+      if (j == i) continue;
+      T dx = x - L[3 * j], dy = y - L[3 * j + 1], dz = z - L[3 * j + 2];
+      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+      T ir = M<T>::rcp(r);
+      T u1, lp;
+      uee_d(S, r, ir, S.elec_spin[j] == spin, u1, lp);
+      if (r >= T(1e-10)) {
+        T f = u1 * ir;
+        gux += f * dx;
+        guy += f * dy;
+        guz += f * dz;
+        lu += lp;
+        pairsq += u1 * u1;
+      }
+      pe += 0.5 * (double)ir;
+    }
+    for (int I = 0; I < Mn; ++I) {
+      T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+      T ir = M<T>::rcp(r);
+      pe -= (double)(S.Z[I] * ir);
+      if (r > T(1e-10)) {  // methane.rs:1253
+        const T f3 = S.Z[I] * ir * ir * ir;
+        row[Ly.f(I, 0)] += f3 * dx;
+        row[Ly.f(I, 1)] += f3 * dy;
+        row[Ly.f(I, 2)] += f3 * dz;
+      }
+      if (S.en_on) {
+        T u1, lp;
+        uen_d(S, r, ir, I, u1, lp);
+        if (r >= T(1e-10)) {
+          T f = u1 * ir;
+          gux += f * dx;
+          guy += f * dy;
+          guz += f * dz;
+          lu += lp;
+          row[Ly.om(I, 0)] -= f * dx;
+          row[Ly.om(I, 1)] -= f * dy;
+          row[Ly.om(I, 2)] -= f * dz;
+        }
+      }
+    }
+    // d ln D / d C[m][ao] = sum_k ainv[m][k] chi_ao(r_k), times dC/dR
+    int last = -1;
+    T chi[4] = {T(0), T(0), T(0), T(0)};
+    for (int p = A.mdp0[spin]; p < A.mdp0[spin + 1]; ++p) {
+      const MoDerivPair P = A.mdp[p];
+      if (P.sh != last) {
+        shell_values(S, P.sh, x, y, z, chi, true);
+        last = P.sh;
+      }
+      const T cq = P.q == 0 ? chi[0] : (P.q == 1 ? chi[1] : (P.q == 2 ? chi[2] : chi[3]));
+      const T v = ai[P.m * n + k] * cq;
+      for (int e = P.e0; e < P.e1; ++e) row[Ly.om(A.mde[e].I, A.mde[e].c)] += (T)A.mde[e].d * v;
+    }
+    const T gs = S.grad_scale;
+    T gjx = gs * gux, gjy = gs * guy, gjz = gs * guz;
+    T lj = (S.flags & F_J2_LAP_QUIRK) ? (lu + pairsq) : (lu + gux * gux + guy * guy + guz * guz);
+    const T lap = ld + lj + T(2) * (gdx * gjx + gdy * gjy + gdz * gjz);
+    G[3 * i] = gdx + gux;
+    G[3 * i + 1] = gdy + guy;
+    G[3 * i + 2] = gdz + guz;
+    kin += -0.5 * (double)lap;
+    pot += pe;
+  }
+  __syncthreads();
+  if (S.jas_kind == JAS_PADE) {
+    const int npair = N * (N - 1) / 2;
+    for (int p = lane; p < npair; p += 64) {
+      int i = 0, r = p;  // p -> (i, j > i), rows of the upper triangle in order
+      while (r >= N - 1 - i) {
+        r -= N - 1 - i;
+        ++i;
+      }
+      const int j = i + 1 + r;
+      ee_pair_derivs(S, L[3 * i] - L[3 * j], L[3 * i + 1] - L[3 * j + 1], L[3 * i + 2] - L[3 * j + 2],
+                     S.elec_spin[i] == S.elec_spin[j], Ly, row,
+                     [&](int c) { return G[3 * i + c] - G[3 * j + c]; });
+    }
+    for (int p = lane; p < N * Mn; p += 64) {
+      const int i = p / Mn, I = p % Mn;
+      en_pair_derivs(S, I, L[3 * i] - S.nuc[I][0], L[3 * i + 1] - S.nuc[I][1], L[3 * i + 2] - S.nuc[I][2], Ly, row,
+                     [&](int c) { return G[3 * i + c]; });
+    }
+  }
+  const double el = wsum(kin) + wsum(pot) + S.vnn;
+  __syncthreads();
+  for (int q = lane; q < A.Qp; q += 64) {
+    double v;
+    if (q == Ly.e()) {
+      v = el;
+    } else if (q == Ly.one()) {
+      v = 1.0;
+    } else {
+      v = 0.0;
+      for (int l = 0; l < 64; ++l) v += (double)rows[l * rs + q];
+    }
+    A.Y[(int64_t)q * A.ldy + w] = v;
+  }
+}
+
+template <class T>
+hipError_t wave_derivs(const void* Sd, const void* St, DerivArgs A, const WaveShape& sh, hipStream_t st) {
+  WaveLayout Y;
+  Y.N = sh.N;
+  Y.nu = sh.nu;
+  Y.nd = sh.nd;
+  Y.nmax = sh.nu > sh.nd ? sh.nu : sh.nd;
+  Y.A = sh.A;
+  if (Y.N > 64 || Y.nmax > 32) return hipErrorInvalidValue;
+  const size_t b = ((size_t)(Y.chi() + 3 * Y.N + 64 * (A.Qp + 1)) * sizeof(T) + 7) / 8 * 8 +
+                   sizeof(double) * (3 * (size_t)Y.N + 2 * (size_t)Y.nmax * Y.nmax);
+  if (b + 512 > 160 * 1024) return hipErrorInvalidValue;
+  // the kernel also holds a static 512 B pivot column, so the dynamic
+  // limit is set to what this shape needs rather than the CU's 160 KiB
+  hipError_t e = hipFuncSetAttribute((const void*)wave_derivs_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)b);
+  if (e != hipSuccess) return e;
+  wave_derivs_kernel<T><<<(unsigned)A.W, 64, b, st>>>((const DevSys<double>*)Sd, (const DevSys<T>*)St, A, Y);
+  return hipGetLastError();
+}
+
+template hipError_t wave_derivs<double>(const void*, const void*, DerivArgs, const WaveShape&, hipStream_t);
+template hipError_t wave_derivs<float>(const void*, const void*, DerivArgs, const WaveShape&, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // G = Y Y^T over a walker chunk per wave with v_mfma_f64_16x16x4_f64.

@@ -6,6 +6,18 @@
 
 namespace qmc {
 
+// Dependence of MO coefficients on the nuclear positions (MOs that follow
+// the geometry, such as benzene's C-H sigma bonds): pair p names the
+// coefficient C[spin][m][ao], where AO ao is value q of shell sh, and its
+// entries [e0, e1) give dC/dR_{I,c}.
+struct MoDerivPair {
+  int spin, m, sh, q, e0, e1;
+};
+struct MoDerivEntry {
+  int I, c;
+  double d;
+};
+
 struct DerivArgs {
   int64_t W;
   int P, M, Qp;                 // Jastrow parameters, nuclei, padded feature count
@@ -15,7 +27,12 @@ struct DerivArgs {
   int64_t ld;                   // SoA walker stride
   double* Y;                    // [Qp][ldy] feature matrix
   int64_t ldy;
+  const MoDerivPair* mdp = nullptr;  // coefficient derivatives (wave kernel only):
+  const MoDerivEntry* mde = nullptr; // pairs of spin s are [mdp0[s], mdp0[s+1])
+  int mdp0[3] = {0, 0, 0};
 };
+
+struct WaveShape;
 
 template <class T>
 struct DevSys;
@@ -30,6 +47,10 @@ struct DerivOps {
 // derivative kernels for a system: its compiled shell structure when one
 // matches (unless generic_only), else the generic kernels of its shape
 const DerivOps* find_derivs(const DevSys<double>& D, bool generic_only);
+// The derivative matrix Y from one wavefront per walker (any shape the wave
+// engine runs; positions from the same sources as the register kernels).
+template <class T>
+hipError_t wave_derivs(const void* Sd, const void* St, DerivArgs A, const WaveShape& sh, hipStream_t st);
 // G[Qp][Qp] (upper triangle valid) = Y Y^T over W walkers; partial holds
 // ceil(W/chunk) * Qp * Qp doubles.
 hipError_t launch_gram(const double* Y, int64_t ldy, int64_t W, int Qp, double* partial, int chunk, double* G,

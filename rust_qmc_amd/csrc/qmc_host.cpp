@@ -92,6 +92,7 @@ struct HostSys {
   std::vector<int32_t> init_nuc;
   double init_sigma = 1.0;
   uint32_t flags = 0;
+  int32_t mo_rule = QMC_MO_FIXED;  // MO coefficients that follow the geometry
 
   int n_elec() const { return (int)spins.size(); }
   int n_nuc() const { return (int)Z.size(); }
@@ -132,9 +133,16 @@ struct qmc_system {
     return wave() && tile.ok && prec == QMC_MIXED && engine != QMC_ENGINE_WAVE;
   }
   WaveShape shape() const { return WaveShape{h.n_elec(), h.n_up(), h.n_dn(), h.n_ao()}; }
+  // dC/dR of MO coefficients that follow the geometry (device tables for
+  // the wave derivative kernel; empty for fixed coefficients)
+  MoDerivPair* mdp = nullptr;
+  MoDerivEntry* mde = nullptr;
+  int mdp0[3] = {0, 0, 0};
   ~qmc_system() {
     if (d64) (void)hipFree(d64);
     if (d32) (void)hipFree(d32);
+    if (mdp) (void)hipFree(mdp);
+    if (mde) (void)hipFree(mde);
   }
 };
 
@@ -175,6 +183,59 @@ bool same_prims(const HostSys& h, const qmc_ao& a, const qmc_ao& b, bool coefs) 
 }
 
 bool is_p(const qmc_ao& a, int m) { return a.kind == QMC_AO_GTO && a.l == 1 && a.m == m; }
+
+// benzene.rs:345-381: MOs 9-14 are the C-H sigma bonds; the C 2p part of
+// MO 9+n points from carbon n to hydrogen n (inner 0.20, outer 0.55 along
+// the bond), so BenzeneGTO::set_nuclei (benzene.rs:786-809) rebuilds them.
+// AO layout: carbon n owns AOs 9n..9n+8 (1s, 2s in, 2p in x/y/z, 2s out,
+// 2p out x/y/z), hydrogen n owns 54+2n (inner) and 55+2n (outer).
+static bool benzene_layout(const HostSys& h) {
+  return h.n_nuc() == 12 && h.n_ao() == 66 && h.n_mo == 21 && h.n_up() == 21 && h.n_dn() == 21;
+}
+
+static void benzene_ch_dir(const HostSys& h, int n, double dir[3], double& len) {
+  for (int c = 0; c < 3; ++c) dir[c] = h.nuc[3 * (6 + n) + c] - h.nuc[3 * n + c];
+  len = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  for (int c = 0; c < 3; ++c) dir[c] /= len;
+}
+
+static bool benzene_ch_ok(const HostSys& h) {
+  if (!benzene_layout(h) || (int)h.mo_up.size() != 21 * 66 || (int)h.mo_dn.size() != 21 * 66) return false;
+  for (int n = 0; n < 6; ++n)
+    for (int c = 0; c < 3; ++c)
+      if (!is_p(h.aos[9 * n + 2 + c], c) || !is_p(h.aos[9 * n + 6 + c], c) || h.aos[9 * n + 2 + c].center != n ||
+          h.aos[9 * n + 6 + c].center != n)
+        return false;
+  return true;
+}
+
+int apply_mo_rule(HostSys& h) {
+  if (h.mo_rule == QMC_MO_FIXED) return QMC_OK;
+  if (h.mo_rule != QMC_MO_BENZENE_CH) return fail(QMC_ERR_INVALID, "unknown MO rule");
+  if (!benzene_ch_ok(h)) return fail(QMC_ERR_INVALID, "the benzene C-H MO rule needs the BenzeneGTO AO/MO layout");
+  for (int n = 0; n < 6; ++n) {
+    double dir[3], len;
+    benzene_ch_dir(h, n, dir, len);
+    if (!(len > 0.0) || !std::isfinite(len)) return fail(QMC_ERR_INVALID, "a C-H bond of zero length");
+  }
+  const int NA = 66;
+  for (std::vector<double>* mo : {&h.mo_up, &h.mo_dn})
+    for (int n = 0; n < 6; ++n) {
+      double* row = mo->data() + (size_t)(9 + n) * NA;
+      double dir[3], len;
+      benzene_ch_dir(h, n, dir, len);
+      std::fill(row, row + NA, 0.0);
+      row[9 * n + 1] = 0.15;
+      row[9 * n + 5] = 0.30;
+      for (int c = 0; c < 3; ++c) {
+        row[9 * n + 2 + c] = 0.20 * dir[c];
+        row[9 * n + 6 + c] = 0.55 * dir[c];
+      }
+      row[54 + 2 * n] = 0.25;
+      row[54 + 2 * n + 1] = 0.15;
+    }
+  return QMC_OK;
+}
 
 template <class T>
 int fill_devsys(const HostSys& h, DevSys<T>& D) {
@@ -301,6 +362,50 @@ int fill_devsys(const HostSys& h, DevSys<T>& D) {
   return QMC_OK;
 }
 
+// dC[spin][m][ao] / dR_{I,c} of the MO rows a rule owns, as the pair and
+// entry tables of the wave derivative kernel.  Benzene's C-H sigma MO n
+// (benzene.rs:350-381) has C 2p coefficients 0.20 u and 0.55 u with
+// u = (R_H - R_C) / |R_H - R_C|, so d u_c / d R_H,c' = (delta_cc' - u_c u_c')
+// / |R_H - R_C| and d u / d R_C is its negative.
+void mo_deriv_tables(const HostSys& h, const DevSys<double>& D, std::vector<MoDerivPair>& pairs,
+                     std::vector<MoDerivEntry>& ents, int p0[3]) {
+  pairs.clear();
+  ents.clear();
+  p0[0] = p0[1] = p0[2] = 0;
+  if (h.mo_rule != QMC_MO_BENZENE_CH) return;
+  auto locate = [&](int ao, int& sh, int& q) {
+    for (sh = 0; sh < D.n_shell; ++sh) {
+      const int k = D.sh_kind[sh], nao = k == SH_SP ? 4 : (k == SH_P ? 3 : 1);
+      if (ao >= D.sh_ao0[sh] && ao < D.sh_ao0[sh] + nao) {
+        q = ao - D.sh_ao0[sh];
+        return;
+      }
+    }
+  };
+  for (int spin = 0; spin < 2; ++spin) {
+    p0[spin] = (int)pairs.size();
+    for (int n = 0; n < 6; ++n) {
+      double u[3], len;
+      benzene_ch_dir(h, n, u, len);
+      for (int part = 0; part < 2; ++part) {
+        const double coef = part ? 0.55 : 0.20;
+        for (int c = 0; c < 3; ++c) {
+          MoDerivPair P{spin, 9 + n, 0, 0, (int)ents.size(), 0};
+          locate(9 * n + (part ? 6 : 2) + c, P.sh, P.q);
+          for (int c2 = 0; c2 < 3; ++c2) {
+            const double d = coef * ((c == c2 ? 1.0 : 0.0) - u[c] * u[c2]) / len;
+            ents.push_back({6 + n, c2, d});
+            ents.push_back({n, c2, -d});
+          }
+          P.e1 = (int)ents.size();
+          pairs.push_back(P);
+        }
+      }
+    }
+  }
+  p0[2] = (int)pairs.size();
+}
+
 int upload(qmc_system* s) {
   if (!s->dirty) return QMC_OK;
   int rc = validate(s->h);
@@ -313,6 +418,21 @@ int upload(qmc_system* s) {
   if (!s->d32) HIP_TRY(hipMalloc(&s->d32, sizeof(DevSys<float>)));
   HIP_TRY(hipMemcpy(s->d64, d64.get(), sizeof(DevSys<double>), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(s->d32, d32.get(), sizeof(DevSys<float>), hipMemcpyHostToDevice));
+  {
+    std::vector<MoDerivPair> pairs;
+    std::vector<MoDerivEntry> ents;
+    mo_deriv_tables(s->h, *d64, pairs, ents, s->mdp0);
+    if (s->mdp) HIP_TRY(hipFree(s->mdp));
+    if (s->mde) HIP_TRY(hipFree(s->mde));
+    s->mdp = nullptr;
+    s->mde = nullptr;
+    if (!pairs.empty()) {
+      HIP_TRY(hipMalloc(&s->mdp, pairs.size() * sizeof(MoDerivPair)));
+      HIP_TRY(hipMalloc(&s->mde, ents.size() * sizeof(MoDerivEntry)));
+      HIP_TRY(hipMemcpy(s->mdp, pairs.data(), pairs.size() * sizeof(MoDerivPair), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(s->mde, ents.data(), ents.size() * sizeof(MoDerivEntry), hipMemcpyHostToDevice));
+    }
+  }
   s->ops_spec = find_engine(*d64, false);
   if (s->ops_spec && !s->ops_spec->match) s->ops_spec = nullptr;
   s->ops_gen = find_engine(*d64, true);
@@ -402,6 +522,29 @@ int need_engine(const qmc_system* s) {
   if (!s->ops)
     return fail(QMC_ERR_UNSUPPORTED, "no register engine compiled for " + std::to_string(s->h.n_up()) + " up / " +
                                          std::to_string(s->h.n_dn()) + " down electrons");
+  return QMC_OK;
+}
+
+// The derivative matrix Y of A.W walkers in the arithmetic type prec: the
+// wave derivative kernel when the wave or tile engine runs the system (or
+// its MO coefficients follow the geometry), else the register kernels.
+int launch_derivs(const qmc_system* s, DerivArgs A, int32_t prec, hipStream_t st) {
+  if (s->wave() || !s->dops || s->h.mo_rule != QMC_MO_FIXED) {
+    if (int rc = need_engine(s)) return rc;
+    if (s->h.n_elec() > 64) return fail(QMC_ERR_UNSUPPORTED, "the wave derivative kernel takes up to 64 electrons");
+    A.mdp = s->mdp;
+    A.mde = s->mde;
+    for (int k = 0; k < 3; ++k) A.mdp0[k] = s->mdp0[k];
+    if (prec == QMC_FP64)
+      HIP_TRY(wave_derivs<double>(s->d64, s->d64, A, s->shape(), st));
+    else
+      HIP_TRY(wave_derivs<float>(s->d64, s->d32, A, s->shape(), st));
+    return QMC_OK;
+  }
+  if (prec == QMC_FP64)
+    HIP_TRY(s->dops->derivs64(s->d64, s->d64, A, st));
+  else
+    HIP_TRY(s->dops->derivs32(s->d64, s->d32, A, st));
   return QMC_OK;
 }
 
@@ -631,6 +774,8 @@ int qmc_system_create(const qmc_system_desc* d, int32_t precision, qmc_system** 
     h.init_nuc.assign(d->n_elec, 0);
   h.init_sigma = d->init_sigma > 0 ? d->init_sigma : 1.0;
   h.flags = d->flags;
+  h.mo_rule = d->mo_rule;
+  if (int rc = apply_mo_rule(h)) return rc;
   return make_system(std::move(h), precision, out);
 }
 
@@ -787,7 +932,6 @@ int qmc_benzene_gto_create(double b_ee, double b_en, double cc, double ch, int32
   const int NA = 66;
   std::vector<double> mo(21 * NA, 0.0);
   auto C = [&](int m, int n, int k) -> double& { return mo[m * NA + 9 * n + k]; };
-  auto Hh = [&](int m, int n, int k) -> double& { return mo[m * NA + 54 + 2 * n + k]; };
   int m = 0;
   for (int n = 0; n < 6; ++n, ++m) {
     C(m, n, 0) = 0.9943;
@@ -809,20 +953,7 @@ int qmc_benzene_gto_create(double b_ee, double b_en, double cc, double ch, int32
       C(m, n, pass == 0 ? 2 : 3) = 0.50 * w;
       C(m, n, pass == 0 ? 6 : 7) = 0.55 * w;
     }
-  for (int n = 0; n < 6; ++n, ++m) {
-    double dir[3];
-    for (int c = 0; c < 3; ++c) dir[c] = h.nuc[3 * (6 + n) + c] - h.nuc[3 * n + c];
-    const double dn = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-    for (double& v : dir) v /= dn;
-    C(m, n, 1) = 0.15;
-    C(m, n, 5) = 0.30;
-    for (int c = 0; c < 3; ++c) {
-      C(m, n, 2 + c) = 0.20 * dir[c];
-      C(m, n, 6 + c) = 0.55 * dir[c];
-    }
-    Hh(m, n, 0) = 0.25;
-    Hh(m, n, 1) = 0.15;
-  }
+  m += 6;  // MOs 9-14: the C-H sigma bonds, from the geometry (benzene_ch_mos)
   for (int n = 0; n < 6; ++n) {
     C(m, n, 4) = 0.30 * s6;
     C(m, n, 8) = 0.70 * s6;
@@ -855,6 +986,8 @@ int qmc_benzene_gto_create(double b_ee, double b_en, double cc, double ch, int32
   h.n_mo = 21;
   h.mo_up = mo;
   h.mo_dn = mo;
+  h.mo_rule = QMC_MO_BENZENE_CH;
+  if (int rc = apply_mo_rule(h)) return rc;
   jastrow3(h, b_ee, b_en);
   h.flags = QMC_FLAG_HF_NN_REFSIGN;  // benzene.rs:846-856
   return make_system(std::move(h), precision, out);
@@ -981,7 +1114,10 @@ int qmc_set_nuclei(qmc_system* s, const double* R, int32_t n) {
   // methane.rs:965-978 asserts the count and rebuilds basis centres; AO
   // centres here are nucleus indices, so only the coordinates change.
   if (n != s->h.n_nuc()) return fail(QMC_ERR_INVALID, "nucleus count mismatch");
-  s->h.nuc.assign(R, R + 3 * n);
+  HostSys h = s->h;
+  h.nuc.assign(R, R + 3 * n);
+  if (int rc = apply_mo_rule(h)) return rc;  // benzene.rs:797: the C-H sigma MOs follow the bonds
+  s->h = std::move(h);
   s->dirty = true;
   return upload(s);
 }
@@ -1043,7 +1179,6 @@ int qmc_derivatives(const qmc_system* sys, int64_t W, const double* pos, double*
                     double* F, double* Om, void* stream) {
   if (!sys || !pos || W < 0) return fail(QMC_ERR_INVALID, "bad arguments");
   if (W == 0) return QMC_OK;
-  if (!sys->dops) return need_engine(sys) ? need_engine(sys) : fail(QMC_ERR_UNSUPPORTED, "no derivative kernel");
   hipStream_t st = (hipStream_t)stream;
   const int N = sys->h.n_elec();
   const Cols C(sys->h);
@@ -1054,10 +1189,7 @@ int qmc_derivatives(const qmc_system* sys, int64_t W, const double* pos, double*
   HIP_TRY(hipMalloc(&Y, sizeof(double) * (size_t)C.Qp * W));
   std::unique_ptr<double, void (*)(double*)> yg(Y, [](double* p) { (void)hipFree(p); });
   DerivArgs A{W, C.P, C.M, C.Qp, (const double*)P.dev, nullptr, nullptr, 0, Y, W};
-  if (sys->precision == QMC_FP64)
-    HIP_TRY(sys->dops->derivs64(sys->d64, sys->d64, A, st));
-  else
-    HIP_TRY(sys->dops->derivs32(sys->d64, sys->d32, A, st));
+  if ((rc = launch_derivs(sys, A, sys->precision, st))) return rc;
   std::vector<double> h((size_t)C.Qp * W);
   HIP_TRY(hipMemcpyAsync(h.data(), Y, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -1414,7 +1546,6 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
 int qmc_ensemble_measure(qmc_ensemble* e, int32_t reset) {
   if (!e) return fail(QMC_ERR_INVALID, "null ensemble");
   const qmc_system* s = e->sys;
-  if (!s->dops) return fail(QMC_ERR_UNSUPPORTED, "no derivative kernel for this shape");
   int rc;
   if (e->sys_version != s->version && (rc = ens_refresh(e, 1))) return rc;
   const Cols C(s->h);
@@ -1424,13 +1555,11 @@ int qmc_ensemble_measure(qmc_ensemble* e, int32_t reset) {
   if ((rc = ensure_cap(&e->gpart, &e->gpart_cap, (size_t)nblk * C.Qp * C.Qp))) return rc;
   if ((rc = ensure_cap(&e->G, &e->G_cap, (size_t)C.Qp * C.Qp))) return rc;
   DerivArgs A{e->W, C.P, C.M, C.Qp, nullptr, nullptr, nullptr, e->ld, e->Y, e->ld};
-  if (e->prec == QMC_FP64) {
+  if (e->prec == QMC_FP64)
     A.pos64 = (const double*)e->pos;
-    HIP_TRY(s->dops->derivs64(s->d64, s->d64, A, e->st));
-  } else {
+  else
     A.pos32 = (const float*)e->pos;
-    HIP_TRY(s->dops->derivs32(s->d64, s->d32, A, e->st));
-  }
+  if ((rc = launch_derivs(s, A, e->prec, e->st))) return rc;
   HIP_TRY(launch_gram(e->Y, e->ld, e->W, C.Qp, e->gpart, chunk, e->G, e->st));
   std::vector<double> g((size_t)C.Qp * C.Qp);
   HIP_TRY(hipMemcpyAsync(g.data(), e->G, g.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
@@ -1677,7 +1806,9 @@ int qmc_system_to_json(const qmc_system* s, char* buf, int64_t len, int64_t* nee
   ints(h.init_nuc.data(), h.init_nuc.size());
   o += ", \"init_sigma\": ";
   num(h.init_sigma);
-  o += ", \"flags\": " + std::to_string(h.flags) + "}";
+  o += ", \"flags\": " + std::to_string(h.flags);
+  if (h.mo_rule == QMC_MO_BENZENE_CH) o += ", \"mo_rule\": \"benzene_ch\"";
+  o += "}";
   if (needed) *needed = (int64_t)o.size() + 1;
   if (len == 0) return QMC_OK;  // size query
   if ((int64_t)o.size() + 1 > len) return fail(QMC_ERR_INVALID, "buffer too small");

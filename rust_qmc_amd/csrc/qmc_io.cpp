@@ -478,6 +478,12 @@ int generic(const JVal& root, int32_t precision, qmc_system** out, Conv& cv) {
   d.species = species.empty() ? nullptr : species.data();
   d.init_nuc = init_nuc.empty() ? nullptr : init_nuc.data();
   d.flags = flags;
+  d.mo_rule = QMC_MO_FIXED;
+  if (const JVal* mr = root.get("mo_rule")) {
+    const std::string v = mr->k == JVal::STR ? mr->str : "";
+    d.mo_rule = v == "fixed" ? QMC_MO_FIXED : v == "benzene_ch" ? QMC_MO_BENZENE_CH : -1;
+    if (d.mo_rule < 0) return qmc_io_fail(QMC_ERR_INVALID, "mo_rule: expected \"fixed\" or \"benzene_ch\"");
+  }
   return qmc_system_create(&d, precision, out);
 }
 

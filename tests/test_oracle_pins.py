@@ -467,6 +467,53 @@ def test_wfn_nuclear_gradient_fd(oracle, name):
             assert Om[I, k] == pytest.approx(fd, rel=1e-5, abs=1e-6), (name, I, k)
 
 
+def _bent_benzene(oracle, seed=3):
+    """Benzene with every nucleus displaced (in and out of the ring plane)."""
+    b = oracle.System.benzene()
+    R0, _ = b.nuclei()
+    R = R0 + np.random.default_rng(seed).normal(size=R0.shape) * 0.15
+    return b, R0, R
+
+
+def test_benzene_ch_sigma_mos_follow_the_bonds(oracle):
+    """BenzeneGTO::set_nuclei rebuilds the MO coefficients (benzene.rs:797):
+    after a distortion the C 2p part of the C-H sigma MO n (MO 9+n) points
+    from carbon n to hydrogen n with lengths 0.20 (inner) and 0.55 (outer),
+    every other coefficient is unchanged, and the original geometry gives
+    back the original coefficients bit for bit."""
+    b, R0, R = _bent_benzene(oracle)
+    m0 = b.mo(0)
+    b.set_nuclei(R)
+    m1 = b.mo(0)
+    assert np.array_equal(b.mo(1), m1)
+    for n in range(6):
+        d = R[6 + n] - R[n]
+        d /= np.linalg.norm(d)
+        pin, pout = m1[9 + n, 9 * n + 2:9 * n + 5], m1[9 + n, 9 * n + 6:9 * n + 9]
+        assert np.linalg.norm(pin) == pytest.approx(0.20, rel=1e-14)
+        assert np.linalg.norm(pout) == pytest.approx(0.55, rel=1e-14)
+        assert np.allclose(pin / 0.20, d, atol=1e-14) and np.allclose(pout / 0.55, d, atol=1e-14)
+    changed = np.argwhere(m1 != m0)
+    assert set(changed[:, 0]) <= set(range(9, 15))
+    b.set_nuclei(R0)
+    assert np.array_equal(b.mo(0), m0)
+
+
+def test_benzene_nuclear_gradient_translation_sum_rule(oracle):
+    """Every benzene MO is unchanged when all nuclei move together, so
+    translating electrons and nuclei leaves psi unchanged:
+    sum_I Omega_I + sum_i grad_i ln psi = 0 (the reference has no Omega
+    for BenzeneGTO; the oracle defines it by central differences through
+    set_nuclei, which this rule checks on a distorted geometry)."""
+    b, _, R = _bent_benzene(oracle, 5)
+    b.set_nuclei(R)
+    r = b.initialize(2, 1)
+    Om = b.wfn_nuclear_gradient(r)
+    g = b.derivative(r) / b.evaluate(r)
+    tot = Om.sum(axis=0) + g.sum(axis=0)
+    assert np.allclose(tot, 0.0, atol=2e-6 * (np.abs(Om).sum() + 1.0))
+
+
 def test_hf_force_against_potential_gradient(oracle):
     """The electron-nucleus part of hellmann_feynman_force is -dV_en/dR_I.
     The nucleus-nucleus part carries the reference's sign: it subtracts
