@@ -228,8 +228,11 @@ void orc_mcmc_time_steps_mt(const orc_system* s, int64_t W, int64_t warm, int64_
 void orc_single_electron_steps(const orc_system* s, int64_t W, double* pos, double* logpsi,
                                uint64_t seed, uint32_t step0, int nsteps, double step,
                                double* eloc, int64_t* accepted);
+/* flags: ORC_DD_NO_GREEN drops the Green's function ratio; ORC_DD_LOG takes
+ * the acceptance ratio from ln|psi| without the reference's 1e-30 guard */
+enum { ORC_DD_NO_GREEN = 1, ORC_DD_LOG = 2 };
 void orc_dd_steps(const orc_system* s, int64_t W, double* pos, uint64_t seed, uint32_t step0, int nsteps,
-                  double tau, double max_drift, int no_green, double* eloc, int64_t* accepted);
+                  double tau, double max_drift, int flags, double* eloc, int64_t* accepted);
 void orc_all_electron_steps(const orc_system* s, int64_t W, double* pos, double* logpsi,
                             double* eloc, uint64_t seed, uint32_t step0, int nsteps, double step,
                             int64_t* accepted);

@@ -362,13 +362,15 @@ class System:
                                         _ptr(el), C.byref(acc))
         return pos, lp, el, acc.value
 
-    def dd_steps(self, pos, seed, step0, nsteps, tau, max_drift, no_green=False):
-        """DriftDiffusionVMC sweeps (vmc.rs:382-438) with the GPU's counters."""
+    def dd_steps(self, pos, seed, step0, nsteps, tau, max_drift, no_green=False, log_domain=False):
+        """DriftDiffusionVMC sweeps (vmc.rs:382-438) with the GPU's counters;
+        log_domain: acceptance from ln|psi| without the 1e-30 guard."""
         pos = _f64(pos).copy()
         W = pos.shape[0]
         el = np.zeros((nsteps, W))
         acc = C.c_int64()
-        lib().orc_dd_steps(self._h, W, _ptr(pos), seed, step0, nsteps, tau, max_drift, int(no_green), _ptr(el),
+        flags = (1 if no_green else 0) | (2 if log_domain else 0)
+        lib().orc_dd_steps(self._h, W, _ptr(pos), seed, step0, nsteps, tau, max_drift, flags, _ptr(el),
                            C.byref(acc))
         return pos, el, acc.value
 

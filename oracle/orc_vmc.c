@@ -240,9 +240,9 @@ void orc_single_electron_steps(const orc_system* s, int64_t W, double* pos, doub
  * (w, step, 2e, DRIFT), U from (w, step, 2e+1, DRIFT) word 0.  E_L after
  * each sweep into eloc [nsteps][W].  no_green != 0 drops the Green's
  * function factor (a deliberately wrong sampler for the tests). */
-static void dd_drift(const orc_system* s, const double (*r)[3], double psi, double (*F)[3]) {
+static void dd_drift(const orc_system* s, const double (*r)[3], double psi, int logd, double (*F)[3]) {
     int n = s->n_elec;
-    if (fabs(psi) < 1e-30) {
+    if (logd ? psi == 0.0 : fabs(psi) < 1e-30) {
         for (int e = 0; e < n; ++e) F[e][0] = F[e][1] = F[e][2] = 0.0;
         return;
     }
@@ -263,7 +263,8 @@ static void dd_cap(const double F[3], double tau, double max_drift, double d[3])
 }
 
 void orc_dd_steps(const orc_system* s, int64_t W, double* pos, uint64_t seed, uint32_t step0, int nsteps,
-                  double tau, double max_drift, int no_green, double* eloc, int64_t* accepted) {
+                  double tau, double max_drift, int flags, double* eloc, int64_t* accepted) {
+    const int no_green = flags & ORC_DD_NO_GREEN, logd = flags & ORC_DD_LOG;
     int n = s->n_elec;
     int64_t acc = 0;
     int order[ORC_MAXE], q = 0;
@@ -272,11 +273,15 @@ void orc_dd_steps(const orc_system* s, int64_t W, double* pos, uint64_t seed, ui
     for (int e = 0; e < n; ++e)
         if (s->spins[e] != 1) order[q++] = e;
     const double s2t = sqrt(2.0 * tau);
+    /* ORC_DD_LOG: the ratio from ln|psi| and no 1e-30 guard (vmc.rs:363);
+     * the drift 2 grad psi / psi stays a ratio of normal doubles for any
+     * psi above ~1e-300, benzene's e^-650 included */
     for (int64_t w = 0; w < W; ++w) {
         double(*r)[3] = (double(*)[3])(pos + w * n * 3);
-        double psi = orc_evaluate(s, (const double(*)[3])r);
+        double psi = orc_evaluate(s, (const double(*)[3])r), la = 0.0, sg = 0.0;
+        if (logd) orc_log_psi(s, (const double(*)[3])r, &la, &sg);
         double F[ORC_MAXE][3], Fn[ORC_MAXE][3], t[ORC_MAXE][3];
-        dd_drift(s, (const double(*)[3])r, psi, F);
+        dd_drift(s, (const double(*)[3])r, psi, logd, F);
         for (int k = 0; k < nsteps; ++k) {
             uint32_t st = step0 + (uint32_t)k;
             for (int qq = 0; qq < n; ++qq) {
@@ -287,8 +292,14 @@ void orc_dd_steps(const orc_system* s, int64_t W, double* pos, uint64_t seed, ui
                 memcpy(t, r, sizeof(double) * 3 * n);
                 for (int c = 0; c < 3; ++c) t[e][c] = r[e][c] + d[c] + s2t * z[c];
                 const double psin = orc_evaluate(s, (const double(*)[3])t);
-                if (fabs(psin) < 1e-30) continue;
-                dd_drift(s, (const double(*)[3])t, psin, Fn);
+                double lan = 0.0, sgn = 0.0;
+                if (logd) {
+                    orc_log_psi(s, (const double(*)[3])t, &lan, &sgn);
+                    if (sgn == 0.0) continue;
+                } else if (fabs(psin) < 1e-30) {
+                    continue;
+                }
+                dd_drift(s, (const double(*)[3])t, psin, logd, Fn);
                 double dn[3], gf = 0.0, gb = 0.0;
                 dd_cap(Fn[e], tau, max_drift, dn);
                 for (int c = 0; c < 3; ++c) {
@@ -298,18 +309,26 @@ void orc_dd_steps(const orc_system* s, int64_t W, double* pos, uint64_t seed, ui
                 }
                 gf /= 4.0 * tau;
                 gb /= 4.0 * tau;
-                const double ratio = psin / psi;
-                double p = ratio * ratio * (no_green ? 1.0 : exp(gb - gf));
+                double p;
+                if (logd) {
+                    p = exp(2.0 * (lan - la) + (no_green ? 0.0 : gb - gf));
+                } else {
+                    const double ratio = psin / psi;
+                    p = ratio * ratio * (no_green ? 1.0 : exp(gb - gf));
+                }
                 if (p > 1.0) p = 1.0;
                 if (u < p) {
                     memcpy(r, t, sizeof(double) * 3 * n);
                     psi = psin;
+                    la = lan;
+                    sg = sgn;
                     memcpy(F, Fn, sizeof(double) * 3 * n);
                     ++acc;
                 }
             }
             if (eloc) eloc[(int64_t)k * W + w] = orc_local_energy(s, (const double(*)[3])r);
         }
+        (void)sg;
     }
     if (accepted) *accepted = acc;
 }

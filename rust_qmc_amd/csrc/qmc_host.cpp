@@ -1466,9 +1466,10 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
   if (e->sys_version != s->version && (rc = ens_refresh(e, 1))) return rc;  // parameters or nuclei moved
   if (sm && e->inverse_stale && (rc = ens_refresh(e, 0))) return rc;
   const bool wave = s->wave();
-  if (wave && p->move != QMC_MOVE_SINGLE)
-    return fail(QMC_ERR_UNSUPPORTED, "the wave and tile engines run Metropolis single-electron moves only");
-  const int nblk = p->move == QMC_MOVE_SINGLE ? (int)sweep_blocks(e) : (int)((e->W + 63) / 64);
+  if (wave && p->move == QMC_MOVE_ALL)
+    return fail(QMC_ERR_UNSUPPORTED, "the wave and tile engines run single-electron moves only");
+  // drift-diffusion on the wave engine (also for tile-engine systems): one block per walker
+  const int nblk = p->move == QMC_MOVE_SINGLE ? (int)sweep_blocks(e) : (int)(wave ? e->W : (e->W + 63) / 64);
   const int refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
   int chunk = p->steps_per_launch > 0 ? p->steps_per_launch : 16;
   if ((rc = ensure_cap(&e->partial, &e->partial_cap, (size_t)chunk * nblk * 3))) return rc;
@@ -1497,7 +1498,11 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
     if (p->move == QMC_MOVE_SINGLE) {
       if ((rc = enqueue_single_sweeps(e, A, trace, e->st))) return rc;
     } else if (p->move == QMC_MOVE_DRIFT) {
-      if (e->prec == QMC_FP64)
+      if (wave && e->prec == QMC_FP64)
+        HIP_TRY(wave_dd<double>(s->d64, e->view<double>(), s->shape(), A, e->partial, trace, e->st));
+      else if (wave)
+        HIP_TRY(wave_dd<float>(s->d32, e->view<float>(), s->shape(), A, e->partial, trace, e->st));
+      else if (e->prec == QMC_FP64)
         HIP_TRY(s->ops->dd64(s->d64, e->view<double>(), A, e->partial, trace, e->st));
       else
         HIP_TRY(s->ops->dd32(s->d32, e->view<float>(), A, e->partial, trace, e->st));

@@ -269,6 +269,204 @@ __global__ void __launch_bounds__(64) wave_sweep_kernel(const DevSys<T>* __restr
   }
 }
 
+// Persistent drift-diffusion sweeps (DriftDiffusionVMC::sweep,
+// vmc.rs:382-438), one wavefront per walker: Langevin proposal
+// r' = r + cap(tau F(r)) + sqrt(2 tau) z for each electron in turn, with
+// F = 2 grad ln|psi| of the moved electron, acceptance R^2 e^{2 du}
+// e^{G_b - G_f} (vmc.rs:323-346), Sherman-Morrison on acceptance.  The
+// moves and random streams are those of the register engine's
+// sector_moves_dd.  Lanes own shells for the determinant part: the AO
+// weights w_ao = sum_m C[m][ao] Ainv[m][k] give grad ln|D| at r, and at r'
+// R = sum w chi(r') and grad D'/D = sum w grad chi(r'); lanes own partner
+// electrons and nuclei for the Jastrow part.  Blocks past a branching
+// population's live count write zero partial sums.
+template <class T>
+__global__ void __launch_bounds__(64) wave_dd_kernel(const DevSys<T>* __restrict__ Sp, EnsView<T> E, WaveLayout Y,
+                                                     SweepArgs A, double* __restrict__ partial,
+                                                     double* __restrict__ el_trace) {
+  extern __shared__ double lds_d[];
+  const DevSys<T>& S = *Sp;
+  const int64_t w = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t W = E.n_live ? *E.n_live : E.W;
+  if (w >= W) {
+    if (lane == 0)
+      for (int s = 0; s < A.n_steps; ++s)
+        for (int q = 0; q < 3; ++q) partial[((int64_t)s * gridDim.x + blockIdx.x) * 3 + q] = 0.0;
+    return;
+  }
+  T* L = (T*)lds_d;
+  load_walker(E, Y, w, L);
+  T* pos = L + Y.pos();
+  T* chi = L + Y.chi();
+  T* phi = L + Y.phi();
+  T* cv = L + Y.cvec();
+  T* wv = L + Y.wvec();
+  T* wao = L + Y.wao();
+  double logdet = E.logdet[w], sgn = E.sign[w], ju = E.ju[w], el = 0.0;
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  const uint32_t wid = (uint32_t)(A.walker0 + w);
+  const T tau = (T)A.step_size, md = (T)A.max_drift;
+  const T s2t = M<T>::sq(T(2) * tau), i4t = T(0.25) / tau;
+  const T gs = S.grad_scale;
+  // the Jastrow value and gradient terms of electron e at (x, y, z) over
+  // the partners this lane owns
+  auto jastrow_part = [&](int e, int spin, T x, T y, T z, T& u, T& gx, T& gy, T& gz) {
+    u = gx = gy = gz = T(0);
+    for (int j = lane; j < Y.N; j += 64) {
+      if (j == e) continue;
+      const T dx = x - pos[3 * j], dy = y - pos[3 * j + 1], dz = z - pos[3 * j + 2];
+      const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+      const T ir = M<T>::rcp(r);
+      const bool same = S.elec_spin[j] == spin;
+      T u1, lp;
+      uee_d(S, r, ir, same, u1, lp);
+      u += uee_val(S, r, same);
+      const T f = r >= T(1e-10) ? u1 * ir : T(0);
+      gx += f * dx;
+      gy += f * dy;
+      gz += f * dz;
+    }
+    if (S.en_on)
+      for (int I = lane; I < S.n_nuc; I += 64) {
+        const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+        const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+        const T ir = M<T>::rcp(r);
+        T u1, lp;
+        uen_d(S, r, ir, I, u1, lp);
+        u += uen_val(S, r, I);
+        const T f = r >= T(1e-10) ? u1 * ir : T(0);
+        gx += f * dx;
+        gy += f * dy;
+        gz += f * dz;
+      }
+  };
+  for (int s = 0; s < A.n_steps; ++s) {
+    const uint32_t step = A.step0 + (uint32_t)s;
+    int acc = 0;
+    for (int spin = 0; spin < 2; ++spin) {
+      const int n = spin ? Y.nd : Y.nu;
+      T* ai = L + Y.ainv(spin);
+      for (int k = 0; k < n; ++k) {
+        const int e = S.sector_elec[spin][k];
+        const T ox = pos[3 * e], oy = pos[3 * e + 1], oz = pos[3 * e + 2];
+        // quantum force at r: AO weights and grad ln|D|, Jastrow gradient
+        T gx = 0, gy = 0, gz = 0;
+        for (int sh = lane; sh < S.n_shell; sh += 64) {
+          T v[4], g[4][3];
+          const int nq = shell_values_grad(S, sh, ox, oy, oz, v, g);
+          const int a0 = S.sh_ao0[sh];
+          for (int q = 0; q < nq; ++q) {
+            T wq = 0;
+            for (int m = 0; m < n; ++m) wq += S.moT[spin][a0 + q][m] * ai[m * n + k];
+            wao[a0 + q] = wq;
+            gx += wq * g[q][0];
+            gy += wq * g[q][1];
+            gz += wq * g[q][2];
+          }
+        }
+        T uo, jx, jy, jz;
+        jastrow_part(e, spin, ox, oy, oz, uo, jx, jy, jz);
+        gx = wsum(gx);
+        gy = wsum(gy);
+        gz = wsum(gz);
+        uo = wsum(uo);
+        jx = wsum(jx);
+        jy = wsum(jy);
+        jz = wsum(jz);
+        T dold[3];
+        cap_drift(T(2) * (gx + gs * jx), T(2) * (gy + gs * jy), T(2) * (gz + gs * jz), tau, md, dold);
+        T z[3];
+        T U;
+        Rng<T>::move(philox(wid, step, (uint32_t)e, RNG_DRIFT, k0, k1), z, U);
+        const T nx = ox + dold[0] + s2t * z[0], ny = oy + dold[1] + s2t * z[1], nz = oz + dold[2] + s2t * z[2];
+        __syncthreads();  // AO weights complete
+        // at r': R = sum w chi, grad D'/D = sum w grad chi; chi(r') kept for the update
+        T R = 0, hx = 0, hy = 0, hz = 0;
+        for (int sh = lane; sh < S.n_shell; sh += 64) {
+          T v[4], g[4][3];
+          const int nq = shell_values_grad(S, sh, nx, ny, nz, v, g);
+          const int a0 = S.sh_ao0[sh];
+          for (int q = 0; q < nq; ++q) {
+            const T wq = wao[a0 + q];
+            chi[a0 + q] = v[q];
+            R += wq * v[q];
+            hx += wq * g[q][0];
+            hy += wq * g[q][1];
+            hz += wq * g[q][2];
+          }
+        }
+        T un, vx, vy, vz;
+        jastrow_part(e, spin, nx, ny, nz, un, vx, vy, vz);
+        R = wsum(R);
+        hx = wsum(hx);
+        hy = wsum(hy);
+        hz = wsum(hz);
+        un = wsum(un);
+        vx = wsum(vx);
+        vy = wsum(vy);
+        vz = wsum(vz);
+        const T iR = M<T>::rcp(R);
+        T dnew[3];
+        cap_drift(T(2) * (hx * iR + gs * vx), T(2) * (hy * iR + gs * vy), T(2) * (hz * iR + gs * vz), tau, md,
+                  dnew);
+        const T f0 = nx - ox - dold[0], f1 = ny - oy - dold[1], f2 = nz - oz - dold[2];
+        const T b0 = ox - nx - dnew[0], b1 = oy - ny - dnew[1], b2 = oz - nz - dnew[2];
+        const T gf = -(f0 * f0 + f1 * f1 + f2 * f2) * i4t;
+        const T gb = -(b0 * b0 + b1 * b1 + b2 * b2) * i4t;
+        const T du = un - uo;
+        const T p = R * R * M<T>::ex(T(2) * du + gb - gf);
+        __syncthreads();  // chi(r') complete, AO weight reads done
+        if (U < p) {      // wave-uniform
+          for (int m = lane; m < n; m += 64) {
+            T v = 0;
+            for (int ao = 0; ao < Y.A; ++ao) v += S.moT[spin][ao][m] * chi[ao];
+            phi[m] = v;
+          }
+          __syncthreads();
+          for (int j = lane; j < n; j += 64) {
+            T v = 0;
+            for (int m = 0; m < n; ++m) v += phi[m] * ai[m * n + j];
+            wv[j] = v * iR;
+            cv[j] = ai[j * n + k];
+          }
+          __syncthreads();
+          for (int idx = lane; idx < n * n; idx += 64) {
+            const int m = idx / n, j = idx % n;
+            ai[idx] = (j == k) ? cv[m] * iR : ai[idx] - cv[m] * wv[j];
+          }
+          if (lane == 0) {
+            pos[3 * e] = nx;
+            pos[3 * e + 1] = ny;
+            pos[3 * e + 2] = nz;
+          }
+          logdet += (double)M<T>::lg(R < T(0) ? -R : R);
+          if (R < T(0)) sgn = -sgn;
+          ju += (double)du;
+          ++acc;
+          __syncthreads();
+        }
+      }
+    }
+    el = wave_local_energy(S, Y, L, nullptr, nullptr);
+    if (lane == 0) {
+      if (el_trace) el_trace[(int64_t)s * A.trace_ld + w] = el;
+      double* pp = partial + ((int64_t)s * gridDim.x + blockIdx.x) * 3;
+      pp[0] = el;
+      pp[1] = el * el;
+      pp[2] = (double)acc;
+    }
+    __syncthreads();
+  }
+  store_walker(E, Y, w, L);
+  if (lane == 0) {
+    E.logdet[w] = logdet;
+    E.sign[w] = sgn;
+    E.ju[w] = ju;
+    E.eloc[w] = el;
+  }
+}
+
 // per-configuration evaluation from AoS input
 template <class T>
 __global__ void __launch_bounds__(64) wave_eval_kernel(const DevSys<double>* __restrict__ Sd,
@@ -360,6 +558,17 @@ hipError_t wave_sweep(const void* St, EnsView<T> E, WaveShape sh, SweepArgs A, d
 }
 
 template <class T>
+hipError_t wave_dd(const void* St, EnsView<T> E, WaveShape sh, SweepArgs A, double* partial, double* trace,
+                   hipStream_t st) {
+  WaveLayout Y = layout(sh.N, sh.nu, sh.nd, sh.A);
+  const size_t b = lds_bytes<T>(Y, false);
+  hipError_t e = set_lds(wave_dd_kernel<T>, b);
+  if (e != hipSuccess) return e;
+  wave_dd_kernel<T><<<(unsigned)E.W, 64, b, st>>>((const DevSys<T>*)St, E, Y, A, partial, trace);
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t wave_eval(const void* Sd, const void* St, int64_t W, const double* pos, WaveShape sh, EvalOut O,
                      int* sing, hipStream_t st) {
   WaveLayout Y = layout(sh.N, sh.nu, sh.nd, sh.A);
@@ -377,6 +586,9 @@ template hipError_t wave_sweep<double>(const void*, EnsView<double>, WaveShape, 
                                        hipStream_t);
 template hipError_t wave_sweep<float>(const void*, EnsView<float>, WaveShape, SweepArgs, double*, double*,
                                       hipStream_t);
+template hipError_t wave_dd<double>(const void*, EnsView<double>, WaveShape, SweepArgs, double*, double*,
+                                    hipStream_t);
+template hipError_t wave_dd<float>(const void*, EnsView<float>, WaveShape, SweepArgs, double*, double*, hipStream_t);
 template hipError_t wave_eval<double>(const void*, const void*, int64_t, const double*, WaveShape, EvalOut, int*,
                                       hipStream_t);
 template hipError_t wave_eval<float>(const void*, const void*, int64_t, const double*, WaveShape, EvalOut, int*,

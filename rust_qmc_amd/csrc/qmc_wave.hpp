@@ -16,6 +16,10 @@ hipError_t wave_build(const void* Sd, const void* St, EnsView<T> E, WaveShape sh
 template <class T>
 hipError_t wave_sweep(const void* St, EnsView<T> E, WaveShape sh, SweepArgs A, double* partial, double* trace,
                       hipStream_t st);
+// drift-diffusion sweeps; blocks past *E.n_live (when set) write zero partials
+template <class T>
+hipError_t wave_dd(const void* St, EnsView<T> E, WaveShape sh, SweepArgs A, double* partial, double* trace,
+                   hipStream_t st);
 template <class T>
 hipError_t wave_eval(const void* Sd, const void* St, int64_t W, const double* pos, WaveShape sh, EvalOut O,
                      int* sing, hipStream_t st);

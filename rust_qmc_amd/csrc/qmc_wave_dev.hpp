@@ -25,8 +25,66 @@ struct WaveLayout {
   __device__ __host__ int phi() const { return chi() + A; }
   __device__ __host__ int cvec() const { return phi() + nmax; }
   __device__ __host__ int wvec() const { return cvec() + nmax; }
-  __device__ __host__ int total() const { return wvec() + nmax; }
+  __device__ __host__ int wao() const { return wvec() + nmax; }  // AO weights of a drift-diffusion move
+  __device__ __host__ int total() const { return wao() + A; }
 };
+
+// Values and gradients of the AOs of shell sh at (x, y, z), shell-local
+// (v[q], g[q][0..2] for q below the shell's AO count); returns that count.
+// The same functions as shell_values and det_terms_w: contracted Gaussians
+// with s / p / sp angular parts, and radial STO sums whose gradient at the
+// centre is taken as zero.
+template <class T>
+__device__ __forceinline__ int shell_values_grad(const DevSys<T>& S, int sh, T x, T y, T z, T v[4], T g[4][3]) {
+  const int c = S.sh_center[sh], kind = S.sh_kind[sh], p0 = S.sh_prim0[sh], np = S.sh_nprim[sh];
+  const T dx = x - S.nuc[c][0], dy = y - S.nuc[c][1], dz = z - S.nuc[c][2];
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  const T d[3] = {dx, dy, dz};
+  if (kind <= SH_P1) {
+    // R = sum c e, Ra = sum c a e for the s and p coefficient sets
+    T rs = 0, rsa = 0, rp = 0, rpa = 0;
+    for (int k = 0; k < np; ++k) {
+      const T a = S.prim_exp[p0 + k];
+      const T e = M<T>::ex(-a * r2);
+      const T cs = S.coef_s[p0 + k] * e, cp = S.coef_p[p0 + k] * e;
+      rs += cs;
+      rsa += cs * a;
+      rp += cp;
+      rpa += cp * a;
+    }
+    int q = 0;
+    if (kind == SH_S || kind == SH_SP) {  // grad = -2 Ra d
+      v[0] = rs;
+      for (int j = 0; j < 3; ++j) g[0][j] = T(-2) * rsa * d[j];
+      q = 1;
+    }
+    if (kind == SH_S) return 1;
+    // p_m = d_m R: grad_j = delta_jm R - 2 d_m Ra d_j
+    const int m0 = kind == SH_P1 ? S.sh_m[sh] : 0, m1 = kind == SH_P1 ? m0 + 1 : 3;
+    for (int m = m0; m < m1; ++m, ++q) {
+      v[q] = d[m] * rp;
+      for (int j = 0; j < 3; ++j) g[q][j] = (j == m ? rp : T(0)) - T(2) * d[m] * rpa * d[j];
+    }
+    return q;
+  }
+  // radial STO: f(s) = sum c s^p e^{-zeta s}; grad = f'(s) d / s
+  const T s = M<T>::sq(r2);
+  T f = 0, d1 = 0;
+  for (int k = 0; k < np; ++k) {
+    const T zt = S.prim_exp[p0 + k];
+    const int p = S.prim_pow[p0 + k];
+    T sp = T(1);
+    for (int q = 0; q < p; ++q) sp *= s;
+    const T spm1 = p >= 1 ? sp / s : T(0);
+    const T e = S.coef_s[p0 + k] * M<T>::ex(-zt * s);
+    f += e * sp;
+    d1 += e * (-zt * sp + T(p) * spm1);
+  }
+  v[0] = f;
+  const T gs = s == T(0) ? T(0) : d1 / s;
+  for (int j = 0; j < 3; ++j) g[0][j] = gs * d[j];
+  return 1;
+}
 
 template <class T>
 __device__ __forceinline__ T* ainv_at(T* L, const WaveLayout& Y, int spin, int n, int m, int k) {

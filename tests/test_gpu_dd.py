@@ -93,8 +93,57 @@ def test_ddvmc_run_matches_metropolis_vmc(Q):
     assert abs(r.energy - rv.energy) < 3 * math.hypot(r.error, rv.error) + 2e-4
 
 
-def test_dd_not_available_on_the_wave_engine(Q):
+def test_dd_wave_benzene_trajectory_fp64(Q, oracle):
+    """Benzene's drift-diffusion moves on the wave engine follow the
+    oracle's chain in the log domain (the reference's 1e-30 guard on the
+    linear psi, vmc.rs:363, rejects every benzene move: psi ~ e^-650)."""
+    g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05)
+    o = oracle.System.benzene(1.5, 2.0, 2.64, 2.05)
+    W, steps, seed, tau = 3, 2, 7, 0.02
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    es, e2, acc, tr = ens.sweeps(steps, tau, Q.QMC_MOVE_DRIFT, trace=True, refresh_interval=1, max_drift=1.0)
+    pos_o, el_o, acc_o = o.dd_steps(p0, seed, 0, steps, tau, 1.0, log_domain=True)
+    assert acc_o > 0 and acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=1e-9)
+    assert _close(tr, el_o, 1e-9)
+    assert o.dd_steps(p0, seed, 0, 1, tau, 1.0)[2] == 0  # the reference's guard
+
+
+@pytest.mark.parametrize("name", ["h2", "li_seeded", "ch4_sto3g"])
+def test_dd_wave_engine_matches_register_engine(Q, oracle, name):
+    out = []
+    for eng in ("auto", "wave"):
+        g, _ = _pair(Q, oracle, name)
+        g.set_engine(eng)
+        ens = Q.Ensemble(g, 48, 5)
+        es, e2, acc = ens.sweeps(4, 0.05, Q.QMC_MOVE_DRIFT, refresh_interval=2, max_drift=1.0)
+        out.append((es, acc, ens.positions()))
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.allclose(out[0][2], out[1][2], rtol=0, atol=1e-10)
+    assert np.allclose(out[0][0], out[1][0], rtol=1e-10, atol=0)
+
+
+def test_dd_benzene_mixed_and_ddvmc_run(Q):
+    """Mixed-precision benzene (a tile-engine system) runs its drift-
+    diffusion moves on the FP32 wave kernel; DriftDiffusionVMC::run on
+    benzene gives a finite energy with the time step inside the clamp, and
+    the mixed chain's energy agrees with the FP64 chain's."""
+    res = {}
+    for prec in ("fp64", "mixed"):
+        s = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision=prec)
+        dd = Q.DriftDiffusionVMC(s, Q.DDVMCParams(n_walkers=256, n_steps=40, n_burnin=20, time_step=0.02,
+                                                  adaptation_interval=20), seed=9)
+        r = dd.run()
+        assert np.isfinite(r.energy) and 0.001 <= r.final_time_step <= 0.05
+        assert 0.3 < r.acceptance_rate <= 1.0
+        res[prec] = r
+    a, b = res["fp64"], res["mixed"]
+    assert abs(a.energy - b.energy) < 5 * math.hypot(a.error, b.error) + 0.05
+
+
+def test_all_electron_moves_not_available_on_the_wave_engine(Q):
     s = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05)
     ens = Q.Ensemble(s, 2, 0)
     with pytest.raises(Q.QmcError):
-        ens.sweeps(1, 0.01, Q.QMC_MOVE_DRIFT)
+        ens.sweeps(1, 0.1, Q.QMC_MOVE_ALL)

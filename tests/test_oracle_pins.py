@@ -646,3 +646,31 @@ def test_drift_diffusion_zero_variance_hydrogenic(oracle):
     pos, el, acc = s.dd_steps(pos, 0, 0, 20, 0.05, 1.0)
     assert acc > 0
     assert np.allclose(el, -4.5, atol=1e-12)
+
+
+def test_drift_diffusion_log_domain_is_the_same_chain(oracle):
+    """The log-domain acceptance exp(2 (ln|psi'| - ln|psi|)) is the
+    reference's (psi'/psi)^2: on H2 (psi far from underflow) both modes
+    make the same decisions and positions; the zero-variance ion stays at
+    E_L = -4.5 in the log domain too."""
+    h2 = oracle.System.h2mo()
+    p = h2.initialize_batch(1, 40)
+    a = h2.dd_steps(p, 3, 0, 6, 0.05, 1.0)
+    b = h2.dd_steps(p, 3, 0, 6, 0.05, 1.0, log_domain=True)
+    assert a[2] == b[2] and a[2] > 0
+    assert np.allclose(a[0], b[0], rtol=0, atol=1e-12)
+    assert np.allclose(a[1], b[1], rtol=1e-12, atol=0)
+    s = oracle.System.sto_system([([1.0], [0], [3.0])], [1], F=1.0, Z=3)
+    pos, el, acc = s.dd_steps(s.initialize_batch(0, 8), 0, 0, 10, 0.05, 1.0, log_domain=True)
+    assert acc > 0 and np.allclose(el, -4.5, atol=1e-12)
+
+
+def test_drift_diffusion_benzene_guard(oracle):
+    """Benzene's psi (~e^-650) is below the reference's 1e-30 guard
+    (vmc.rs:363), so the reference-mode chain never moves; the log-domain
+    chain accepts most Langevin moves at tau = 0.02."""
+    bz = oracle.System.benzene(1.5, 2.0, 2.64, 2.05)
+    p = bz.initialize_batch(2, 1)
+    assert bz.dd_steps(p, 7, 0, 1, 0.02, 1.0)[2] == 0
+    pos, el, acc = bz.dd_steps(p, 7, 0, 1, 0.02, 1.0, log_domain=True)
+    assert acc > 0.7 * 42 and np.isfinite(el).all()
