@@ -243,6 +243,7 @@ typedef struct {
     double time_step, max_drift, e_ref_damping;
     int64_t n_vmc_equilibrate, n_vmc_force_samples, capacity;
     int32_t matched_branching; /* 1: weights and E_ref feedback over 2 tau */
+    int32_t log_domain;        /* 1: drift-diffusion acceptance from ln|psi| (ORC_DD_LOG) */
 } orc_isdmc_params;
 typedef struct {
     double energy, error, acceptance_rate, avg_population;

@@ -51,7 +51,7 @@ class McmcResults(C.Structure):
 class IsdmcParams(C.Structure):
     _fields_ = [("n_walkers", _i64), ("n_steps", _i64), ("n_burnin", _i64), ("time_step", _d),
                 ("max_drift", _d), ("e_ref_damping", _d), ("n_vmc_equilibrate", _i64),
-                ("n_vmc_force_samples", _i64), ("capacity", _i64), ("matched_branching", _i)]
+                ("n_vmc_force_samples", _i64), ("capacity", _i64), ("matched_branching", _i), ("log_domain", _i)]
 
 
 class IsdmcResults(C.Structure):
@@ -375,13 +375,15 @@ class System:
         return pos, el, acc.value
 
     def isdmc_run(self, seed, n_walkers, n_steps, n_burnin, time_step, max_drift=1.0, e_ref_damping=0.5,
-                  n_vmc_equilibrate=500, n_vmc_force_samples=2000, capacity=0, matched_branching=False):
+                  n_vmc_equilibrate=500, n_vmc_force_samples=2000, capacity=0, matched_branching=False,
+                  log_domain=False):
         """ImportanceSampledDMC::run (is_dmc.rs:376-546) with the GPU's
-        counters; capacity 0 means 4 x n_walkers rounded up to 256."""
+        counters; capacity 0 means 4 x n_walkers rounded up to 256;
+        log_domain: drift-diffusion acceptance from ln|psi|."""
         if capacity <= 0:
             capacity = (4 * n_walkers + 255) // 256 * 256
         p = IsdmcParams(n_walkers, n_steps, n_burnin, time_step, max_drift, e_ref_damping, n_vmc_equilibrate,
-                        n_vmc_force_samples, capacity, int(matched_branching))
+                        n_vmc_force_samples, capacity, int(matched_branching), int(log_domain))
         r = IsdmcResults()
         M = self.n_nuc
         fm, fv, fx = np.zeros((M, 3)), np.zeros((M, 3)), np.zeros((M, 3))

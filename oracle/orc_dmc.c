@@ -149,7 +149,8 @@ void orc_isdmc_run(const orc_system* s, const orc_isdmc_params* p, uint64_t seed
     double acc_p = 0.0, moves_p = 0.0, pop_sum = 0.0, e_sum = 0.0;
     for (int64_t t = 0; t < T; ++t) {
         int64_t acc = 0;
-        orc_dd_steps(s, pop, pos, seed, (uint32_t)t, 1, p->time_step, p->max_drift, 0, eln, &acc);
+        orc_dd_steps(s, pop, pos, seed, (uint32_t)t, 1, p->time_step, p->max_drift, p->log_domain ? ORC_DD_LOG : 0,
+                     eln, &acc);
         const int prod = t >= p->n_burnin;
         if (prod) {
             acc_p += (double)acc;

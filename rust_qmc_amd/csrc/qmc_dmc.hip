@@ -254,13 +254,13 @@ __global__ void __launch_bounds__(DMC_BLOCK) dmc_copy_kernel(const DevSys<double
 // acc_out receives the step's accepted moves.
 __global__ void __launch_bounds__(1024) dmc_finalize_kernel(DmcArgs a, DmcBufs b, int cur, int64_t ts,
                                                             int with_force, const double* __restrict__ sweep_part,
-                                                            double* __restrict__ acc_out) {
+                                                            int sweep_bs, double* __restrict__ acc_out) {
   const int nxt = 1 - cur;
   const int Q = 1 + 3 * a.n_nuc;
   const int nq = with_force ? Q : 1;
-  // rows written for the source population: its 256-walker chunks and 64-walker sweep blocks
+  // rows written for the source population: its 256-walker chunks and sweep blocks
   const int64_t W = b.ctl->n_live[cur];
-  const int nb = (int)((W + DMC_BLOCK - 1) / DMC_BLOCK), sweep_nblk = (int)((W + 63) / 64);
+  const int nb = (int)((W + DMC_BLOCK - 1) / DMC_BLOCK), sweep_nblk = (int)((W + sweep_bs - 1) / sweep_bs);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   __shared__ double tot[2 + 3 * MAXN];
   for (int j = wv; j <= nq; j += 16) {
@@ -338,18 +338,19 @@ __global__ void __launch_bounds__(DMC_BLOCK) hf_partials_kernel(const DevSys<dou
 template <class T>
 hipError_t dmc_branch_step(const void* Sd, const DmcArgs& a, const DmcBufs& b, int cur, EnsView<T> src,
                            const double* eprev_src, EnsView<T> dst, double* eprev_dst, uint32_t t, int64_t ts,
-                           int with_force, const double* sweep_part, int sweep_nblk, double* acc_out,
+                           int with_force, const double* sweep_part, int sweep_nblk, int sweep_bs, double* acc_out,
                            hipStream_t st) {
   if (a.cap <= 0 || a.cap % DMC_BLOCK || a.cap / DMC_BLOCK > DMC_MAX_BLOCKS || a.n_nuc > MAXN || src.ld < a.cap ||
       dst.ld < a.cap)
     return hipErrorInvalidValue;
   const int nb = (int)(a.cap / DMC_BLOCK), nxt = 1 - cur;
-  if (sweep_nblk * 64 < a.cap) return hipErrorInvalidValue;  // the sweep's partial rows cover the capacity
+  if (sweep_bs < 1 || (int64_t)sweep_nblk * sweep_bs < a.cap)  // the sweep's partial rows cover the capacity
+    return hipErrorInvalidValue;
   dmc_branch_kernel<<<nb, DMC_BLOCK, 0, st>>>(a, b, cur, src.eloc, eprev_src, t);
   dmc_scan_kernel<<<1, 1024, 0, st>>>(a, b, nxt, nb);
   dmc_copy_kernel<T><<<nb < 1024 ? nb : 1024, DMC_BLOCK, 0, st>>>((const DevSys<double>*)Sd, a, b, cur, src, dst,
                                                                   eprev_dst, with_force);
-  dmc_finalize_kernel<<<1, 1024, 0, st>>>(a, b, cur, ts, with_force, sweep_part, acc_out);
+  dmc_finalize_kernel<<<1, 1024, 0, st>>>(a, b, cur, ts, with_force, sweep_part, sweep_bs, acc_out);
   return hipGetLastError();
 }
 
@@ -371,10 +372,10 @@ hipError_t hf_partials(const void* Sd, EnsView<T> E, int64_t W, int n3, int n_nu
 
 template hipError_t dmc_branch_step<double>(const void*, const DmcArgs&, const DmcBufs&, int, EnsView<double>,
                                             const double*, EnsView<double>, double*, uint32_t, int64_t, int,
-                                            const double*, int, double*, hipStream_t);
+                                            const double*, int, int, double*, hipStream_t);
 template hipError_t dmc_branch_step<float>(const void*, const DmcArgs&, const DmcBufs&, int, EnsView<float>,
                                            const double*, EnsView<float>, double*, uint32_t, int64_t, int,
-                                           const double*, int, double*, hipStream_t);
+                                           const double*, int, int, double*, hipStream_t);
 template hipError_t hf_partials<double>(const void*, EnsView<double>, int64_t, int, int, double*, int*, hipStream_t);
 template hipError_t hf_partials<float>(const void*, EnsView<float>, int64_t, int, int, double*, int*, hipStream_t);
 

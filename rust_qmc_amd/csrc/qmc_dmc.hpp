@@ -45,12 +45,13 @@ struct DmcBufs {
 // in eprev_src) into buffer nxt and update E_ref; step t is the global step
 // (Philox counter), trace slot ts its row in the traces (-1: none), and
 // with_force adds the new population's HF force to ctl->force_sum.  The
-// sweep's per-block partials [sweep_nblk][3] (accepted moves in column 2)
-// are summed into acc_out[ts].
+// sweep's per-block partials [sweep_nblk][3] (accepted moves in column 2,
+// one row per sweep_bs walkers: 64 for the register engine, 1 for the wave
+// engine) are summed into acc_out[ts].
 template <class T>
 hipError_t dmc_branch_step(const void* Sd, const DmcArgs& a, const DmcBufs& b, int cur, EnsView<T> src,
                            const double* eprev_src, EnsView<T> dst, double* eprev_dst, uint32_t t, int64_t ts,
-                           int with_force, const double* sweep_part, int sweep_nblk, double* acc_out,
+                           int with_force, const double* sweep_part, int sweep_nblk, int sweep_bs, double* acc_out,
                            hipStream_t st);
 // a fresh population of n0 walkers in buffer b0 (positions, state and E_L
 // already built): live count, pre-sweep E_L = E_L, E_ref = mean E_L

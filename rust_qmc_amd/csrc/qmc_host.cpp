@@ -1466,9 +1466,8 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
   if (e->sys_version != s->version && (rc = ens_refresh(e, 1))) return rc;  // parameters or nuclei moved
   if (sm && e->inverse_stale && (rc = ens_refresh(e, 0))) return rc;
   const bool wave = s->wave();
-  if (wave && p->move == QMC_MOVE_ALL)
-    return fail(QMC_ERR_UNSUPPORTED, "the wave and tile engines run single-electron moves only");
-  // drift-diffusion on the wave engine (also for tile-engine systems): one block per walker
+  // drift-diffusion and all-electron moves on the wave engine (also for
+  // tile-engine systems): one block per walker
   const int nblk = p->move == QMC_MOVE_SINGLE ? (int)sweep_blocks(e) : (int)(wave ? e->W : (e->W + 63) / 64);
   const int refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
   int chunk = p->steps_per_launch > 0 ? p->steps_per_launch : 16;
@@ -1507,7 +1506,11 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
       else
         HIP_TRY(s->ops->dd32(s->d32, e->view<float>(), A, e->partial, trace, e->st));
     } else {
-      if (e->prec == QMC_FP64)
+      if (wave && e->prec == QMC_FP64)
+        HIP_TRY(wave_ae<double>(s->d64, s->d64, e->view<double>(), s->shape(), A, e->partial, trace, e->st));
+      else if (wave)
+        HIP_TRY(wave_ae<float>(s->d64, s->d32, e->view<float>(), s->shape(), A, e->partial, trace, e->st));
+      else if (e->prec == QMC_FP64)
         HIP_TRY(s->ops->ae64(s->d64, s->d64, e->view<double>(), A, e->partial, trace, e->st));
       else
         HIP_TRY(s->ops->ae32(s->d64, s->d32, e->view<float>(), A, e->partial, trace, e->st));
@@ -2377,7 +2380,9 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
   const int N = s->h.n_elec(), M = s->h.n_nuc(), nu = s->h.n_up(), nd = s->h.n_dn();
   const int nainv = nu * nu + nd * nd;
   const int64_t T_all = p->n_burnin + p->n_steps;
-  const int nb = (int)(cap / DMC_BLOCK), nblk64 = (int)(cap / 64);
+  // sweep partial rows: one per 64 walkers (register engine) or per walker (wave engine)
+  const bool wave = s->wave();
+  const int nb = (int)(cap / DMC_BLOCK), sweep_bs = wave ? 1 : 64, sweep_rows = (int)(cap / sweep_bs);
   const bool f64 = sizeof(T) == 8;
   hipStream_t st = nullptr;
   DevArena A;
@@ -2396,7 +2401,7 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
   if ((rc = A.get(&B.ctl, sizeof(DmcCtl))) || (rc = A.get(&B.copies, sizeof(int) * cap)) ||
       (rc = A.get(&B.loff, sizeof(int) * cap)) || (rc = A.get(&B.bsum, sizeof(int64_t) * nb)) ||
       (rc = A.get(&B.part, sizeof(double) * nb * (1 + 3 * M))) || (rc = A.get(&B.e_trace, sizeof(double) * T_all)) ||
-      (rc = A.get(&B.pop_trace, sizeof(double) * T_all)) || (rc = A.get(&partial, sizeof(double) * 3 * nblk64)) ||
+      (rc = A.get(&B.pop_trace, sizeof(double) * T_all)) || (rc = A.get(&partial, sizeof(double) * 3 * sweep_rows)) ||
       (rc = A.get(&acc_dev, sizeof(double) * T_all)))
     return rc;
   HIP_TRY(hipMemsetAsync(B.ctl, 0, sizeof(DmcCtl), st));
@@ -2412,10 +2417,16 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
     E.n_live = nullptr;
     if constexpr (sizeof(T) == 8) {
       HIP_TRY(launch_init64(s->d64, E, seed, w0, st));
-      HIP_TRY(s->ops->build64(s->d64, s->d64, E, 1, nullptr, st));
+      if (wave)
+        HIP_TRY(wave_build<double>(s->d64, s->d64, E, s->shape(), 1, nullptr, st));
+      else
+        HIP_TRY(s->ops->build64(s->d64, s->d64, E, 1, nullptr, st));
     } else {
       HIP_TRY(launch_init32(s->d64, E, seed, w0, st));
-      HIP_TRY(s->ops->build32(s->d64, s->d32, E, 1, nullptr, st));
+      if (wave)
+        HIP_TRY(wave_build<float>(s->d64, s->d32, E, s->shape(), 1, nullptr, st));
+      else
+        HIP_TRY(s->ops->build32(s->d64, s->d32, E, 1, nullptr, st));
     }
     HIP_TRY(dmc_reset(B.ctl, b, p->n_walkers, eloc[b], eprev[b], st));
     return QMC_OK;
@@ -2440,17 +2451,26 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
     for (int64_t u = t; u < t_end; ++u) {
       const SweepArgs sa{1, (uint32_t)u, seed, 0, p->time_step, 0, p->max_drift};
       if constexpr (sizeof(T) == 8) {
-        HIP_TRY(s->ops->dd64(s->d64, V[cur], sa, partial, nullptr, st));
+        if (wave)
+          HIP_TRY(wave_dd<double>(s->d64, V[cur], s->shape(), sa, partial, nullptr, st));
+        else
+          HIP_TRY(s->ops->dd64(s->d64, V[cur], sa, partial, nullptr, st));
       } else {
         if (R > 0 && since >= R) {
-          HIP_TRY(s->ops->build32(s->d64, s->d32, V[cur], 0, nullptr, st));
+          if (wave)
+            HIP_TRY(wave_build<float>(s->d64, s->d32, V[cur], s->shape(), 0, nullptr, st));
+          else
+            HIP_TRY(s->ops->build32(s->d64, s->d32, V[cur], 0, nullptr, st));
           since = 0;
         }
-        HIP_TRY(s->ops->dd32(s->d32, V[cur], sa, partial, nullptr, st));
+        if (wave)
+          HIP_TRY(wave_dd<float>(s->d32, V[cur], s->shape(), sa, partial, nullptr, st));
+        else
+          HIP_TRY(s->ops->dd32(s->d32, V[cur], sa, partial, nullptr, st));
       }
       const int nxt = 1 - cur;
       HIP_TRY(dmc_branch_step<T>(s->d64, a, B, cur, V[cur], eprev[cur], V[nxt], eprev[nxt], (uint32_t)u, u,
-                                 u >= p->n_burnin ? 1 : 0, partial, nblk64, acc_dev, st));
+                                 u >= p->n_burnin ? 1 : 0, partial, sweep_rows, sweep_bs, acc_dev, st));
       cur = nxt;
       cur_after[u] = cur;
       ++since;
@@ -2532,8 +2552,6 @@ int qmc_isdmc_run(const qmc_system* sys, const qmc_isdmc_params* p, uint64_t see
     return fail(QMC_ERR_INVALID, "bad arguments");
   int rc = need_engine(sys);
   if (rc) return rc;
-  if (sys->wave() || !sys->ops)
-    return fail(QMC_ERR_UNSUPPORTED, "IS-DMC needs drift-diffusion moves, which run on the register engine");
   const int M = sys->h.n_nuc();
   if (M > MAXN) return fail(QMC_ERR_UNSUPPORTED, "too many nuclei");
   int64_t cap = p->capacity > 0 ? p->capacity : 4 * p->n_walkers;

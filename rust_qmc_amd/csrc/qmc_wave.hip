@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(64) wave_build_kernel(const DevSys<double>* __
                                                         WaveLayout Y, int want_eloc, int* singular_count) {
   extern __shared__ double lds_d[];
   const int64_t w = blockIdx.x;
+  if (E.n_live && w >= *E.n_live) return;  // past a branching population's live walkers
   T* L = (T*)lds_d;
   double* a = lds_d + (Y.total() * sizeof(T) + 7) / 8;
   double* inv = a + Y.nmax * Y.nmax;
@@ -467,6 +468,82 @@ __global__ void __launch_bounds__(64) wave_dd_kernel(const DevSys<T>* __restrict
   }
 }
 
+// All-electron Metropolis (MCMCSimulation::metropolis_step, vmc.rs:95-119),
+// one wavefront per walker: every electron displaced by sigma z with the
+// register engine's random streams, ln|psi'| from the FP64 Gauss-Jordan
+// build of both sectors and the full Jastrow sum, acceptance
+// U < exp(2 (ln|psi'| - ln|psi|)), E_L recomputed on acceptance and cached
+// otherwise.  Only the positions persist: the proposal's inverses serve
+// its E_L, and the host refreshes the inverses before single-electron
+// sweeps.
+template <class T>
+__global__ void __launch_bounds__(64) wave_ae_kernel(const DevSys<double>* __restrict__ Sd,
+                                                     const DevSys<T>* __restrict__ St, EnsView<T> E, WaveLayout Y,
+                                                     SweepArgs A, double* __restrict__ partial,
+                                                     double* __restrict__ el_trace) {
+  extern __shared__ double lds_d[];
+  const DevSys<T>& S = *St;
+  const int64_t w = blockIdx.x;
+  const int lane = threadIdx.x;
+  T* L = (T*)lds_d;
+  double* a = lds_d + (Y.total() * sizeof(T) + 7) / 8;
+  double* inv = a + Y.nmax * Y.nmax;
+  T* qn = (T*)(inv + Y.nmax * Y.nmax);  // the proposal, [N][3]
+  T* pos = L + Y.pos();
+  for (int q = lane; q < 3 * Y.N; q += 64) pos[q] = E.pos[(int64_t)q * E.ld + w];
+  __syncthreads();
+  double ldet = E.logdet[w], ju = E.ju[w], sgn = E.sign[w], el = E.eloc[w];
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  const uint32_t wid = (uint32_t)(A.walker0 + w);
+  const T sigma = (T)A.step_size;
+  for (int s = 0; s < A.n_steps; ++s) {
+    const uint32_t step = A.step0 + (uint32_t)s;
+    for (int e = lane; e < Y.N; e += 64) {
+      T z[3];
+      Rng<T>::normals3(philox(wid, step, (uint32_t)e, RNG_ALL, k0, k1), z);
+      for (int c = 0; c < 3; ++c) qn[3 * e + c] = pos[3 * e + c] + sigma * z[c];
+    }
+    __syncthreads();
+    double lnew = 0.0, snew = 1.0;
+    for (int spin = 0; spin < 2; ++spin) {
+      const int n = spin ? Y.nd : Y.nu;
+      double la, sg;
+      wave_sector_inverse(*Sd, spin, qn, n, a, inv, la, sg);
+      lnew += la;
+      snew *= sg;
+      for (int q = lane; q < n * n; q += 64) L[Y.ainv(spin) + q] = (T)inv[q];
+      __syncthreads();
+    }
+    const double jun = wave_jastrow_u(*Sd, qn, Y.N);
+    const double U = u01d(philox(wid, step, (uint32_t)Y.N, RNG_ALL, k0, k1).x);
+    int acc = 0;
+    if (U < exp(2.0 * ((lnew + jun) - (ldet + ju)))) {  // wave-uniform
+      for (int q = lane; q < 3 * Y.N; q += 64) pos[q] = qn[q];
+      __syncthreads();
+      ldet = lnew;
+      ju = jun;
+      sgn = snew;
+      el = wave_local_energy(S, Y, L, nullptr, nullptr);
+      acc = 1;
+    }
+    if (lane == 0) {
+      if (el_trace) el_trace[(int64_t)s * A.trace_ld + w] = el;
+      double* pp = partial + ((int64_t)s * gridDim.x + blockIdx.x) * 3;
+      pp[0] = el;
+      pp[1] = el * el;
+      pp[2] = (double)acc;
+    }
+    __syncthreads();
+  }
+  for (int q = lane; q < 3 * Y.N; q += 64) E.pos[(int64_t)q * E.ld + w] = pos[q];
+  if (lane == 0) {
+    E.logdet[w] = ldet;
+    E.sign[w] = sgn;
+    E.ju[w] = ju;
+    E.eloc[w] = el;
+  }
+}
+
 // per-configuration evaluation from AoS input
 template <class T>
 __global__ void __launch_bounds__(64) wave_eval_kernel(const DevSys<double>* __restrict__ Sd,
@@ -569,6 +646,18 @@ hipError_t wave_dd(const void* St, EnsView<T> E, WaveShape sh, SweepArgs A, doub
 }
 
 template <class T>
+hipError_t wave_ae(const void* Sd, const void* St, EnsView<T> E, WaveShape sh, SweepArgs A, double* partial,
+                   double* trace, hipStream_t st) {
+  WaveLayout Y = layout(sh.N, sh.nu, sh.nd, sh.A);
+  const size_t b = lds_bytes<T>(Y, true);  // the FP64 scratch's 3N doubles hold the proposal
+  hipError_t e = set_lds(wave_ae_kernel<T>, b);
+  if (e != hipSuccess) return e;
+  wave_ae_kernel<T><<<(unsigned)E.W, 64, b, st>>>((const DevSys<double>*)Sd, (const DevSys<T>*)St, E, Y, A, partial,
+                                                  trace);
+  return hipGetLastError();
+}
+
+template <class T>
 hipError_t wave_eval(const void* Sd, const void* St, int64_t W, const double* pos, WaveShape sh, EvalOut O,
                      int* sing, hipStream_t st) {
   WaveLayout Y = layout(sh.N, sh.nu, sh.nd, sh.A);
@@ -589,6 +678,10 @@ template hipError_t wave_sweep<float>(const void*, EnsView<float>, WaveShape, Sw
 template hipError_t wave_dd<double>(const void*, EnsView<double>, WaveShape, SweepArgs, double*, double*,
                                     hipStream_t);
 template hipError_t wave_dd<float>(const void*, EnsView<float>, WaveShape, SweepArgs, double*, double*, hipStream_t);
+template hipError_t wave_ae<double>(const void*, const void*, EnsView<double>, WaveShape, SweepArgs, double*,
+                                    double*, hipStream_t);
+template hipError_t wave_ae<float>(const void*, const void*, EnsView<float>, WaveShape, SweepArgs, double*, double*,
+                                   hipStream_t);
 template hipError_t wave_eval<double>(const void*, const void*, int64_t, const double*, WaveShape, EvalOut, int*,
                                       hipStream_t);
 template hipError_t wave_eval<float>(const void*, const void*, int64_t, const double*, WaveShape, EvalOut, int*,

@@ -20,6 +20,11 @@ hipError_t wave_sweep(const void* St, EnsView<T> E, WaveShape sh, SweepArgs A, d
 template <class T>
 hipError_t wave_dd(const void* St, EnsView<T> E, WaveShape sh, SweepArgs A, double* partial, double* trace,
                    hipStream_t st);
+// all-electron Metropolis steps (positions, ln|psi| and E_L persist; the
+// inverses must be rebuilt before single-electron sweeps)
+template <class T>
+hipError_t wave_ae(const void* Sd, const void* St, EnsView<T> E, WaveShape sh, SweepArgs A, double* partial,
+                   double* trace, hipStream_t st);
 template <class T>
 hipError_t wave_eval(const void* Sd, const void* St, int64_t W, const double* pos, WaveShape sh, EvalOut O,
                      int* sing, hipStream_t st);

@@ -141,9 +141,3 @@ def test_dd_benzene_mixed_and_ddvmc_run(Q):
     a, b = res["fp64"], res["mixed"]
     assert abs(a.energy - b.energy) < 5 * math.hypot(a.error, b.error) + 0.05
 
-
-def test_all_electron_moves_not_available_on_the_wave_engine(Q):
-    s = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05)
-    ens = Q.Ensemble(s, 2, 0)
-    with pytest.raises(Q.QmcError):
-        ens.sweeps(1, 0.1, Q.QMC_MOVE_ALL)

@@ -148,3 +148,47 @@ def test_isdmc_rejects_bad_arguments(Q):
         kw.update(bad)
         with pytest.raises(Q.QmcError):
             Q.ImportanceSampledDMC(g, Q.ISDMCParams(**kw)).run()
+
+
+def test_isdmc_benzene_wave_engine_parity_fp64(Q, oracle):
+    """Benzene IS-DMC on the wave engine (drift-diffusion sweeps one
+    wavefront per walker, all-electron VMC force phase) against the
+    oracle's log-domain run: the reference's 1e-30 guard on the linear psi
+    (is_dmc.rs:137, 234) would freeze every benzene walker."""
+    g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05)
+    o = oracle.System.benzene(1.5, 2.0, 2.64, 2.05)
+    W, n_steps, n_burnin, tau, n_eq, n_fs = 6, 3, 1, 0.01, 2, 6
+    P = Q.ISDMCParams(n_walkers=W, n_steps=n_steps, n_burnin=n_burnin, time_step=tau, max_drift=1.0,
+                      e_ref_damping=0.5, n_vmc_equilibrate=n_eq, n_vmc_force_samples=n_fs, capacity=256,
+                      refresh_interval=0)
+    dmc = Q.ImportanceSampledDMC(g, P, seed=3)
+    r = dmc.run()
+    ro = o.isdmc_run(3, W, n_steps, n_burnin, tau, 1.0, 0.5, n_eq, n_fs, capacity=256, log_domain=True)
+    assert np.array_equal(dmc.populations, ro["populations"])
+    assert _close(dmc.energies, ro["energies"], 1e-9)
+    assert r.acceptance_rate == pytest.approx(ro["acceptance_rate"], rel=1e-12)
+    scale = np.abs(ro["forces_vmc"]).max()
+    assert np.allclose(r.forces_vmc, ro["forces_vmc"], rtol=1e-8, atol=1e-9 * scale)
+    scale = np.abs(ro["forces_mixed"]).max()
+    assert np.allclose(r.forces_mixed, ro["forces_mixed"], rtol=1e-8, atol=1e-9 * scale)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_isdmc_wave_engine_matches_register_engine(Q, oracle, precision):
+    """CH4 IS-DMC with the sweeps, builds and VMC force phase on the wave
+    engine follows the register-engine run (same decisions, same
+    population)."""
+    out = []
+    for eng in ("auto", "wave"):
+        g, _ = _pair(Q, oracle, "ch4_sto3g", precision=precision)
+        g.set_engine(eng)
+        P = Q.ISDMCParams(n_walkers=64, n_steps=20, n_burnin=5, time_step=0.01, max_drift=1.0, e_ref_damping=0.5,
+                          n_vmc_equilibrate=5, n_vmc_force_samples=100, capacity=256, refresh_interval=8)
+        d = Q.ImportanceSampledDMC(g, P, seed=4)
+        out.append((d.run(), d.populations, d.energies))
+    (ra, pa, ea), (rb, pb, eb) = out
+    assert np.array_equal(pa, pb)
+    tol = 1e-9 if precision == "fp64" else 1e-5
+    assert _close(ea, eb, tol)
+    assert ra.acceptance_rate == pytest.approx(rb.acceptance_rate, rel=1e-12)
+    assert np.allclose(ra.forces_vmc, rb.forces_vmc, rtol=tol, atol=tol * np.abs(rb.forces_vmc).max())

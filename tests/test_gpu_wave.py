@@ -141,3 +141,39 @@ def test_tile_engine_matches_wave_engine(Q):
     assert np.allclose(p1, p2, rtol=0, atol=2e-4)
     assert _close(t1, t2, 2e-4, scale=np.abs(t2) + 10.0)
     assert _close(s1[0], s2[0], 1e-6, scale=np.abs(s2[0]))
+
+
+def test_wave_all_electron_benzene_parity_fp64(Q, oracle):
+    """All-electron Metropolis (vmc.rs:95-119) on the wave engine: every
+    electron moved, ln|psi'| from the FP64 rebuild, E_L on acceptance only."""
+    g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05)
+    o = oracle.System.benzene(1.5, 2.0, 2.64, 2.05)
+    W, steps, seed, step = 4, 4, 2, 0.05
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    es, e2, acc, tr = ens.sweeps(steps, step, Q.QMC_MOVE_ALL, trace=True)
+    ref = o.batch_eval(p0)
+    pos_o, lp_o, el_o, acc_o = o.all_electron_steps(p0, ref["logabs"], ref["eloc"], seed, 0, steps, step)
+    assert acc_o > 0 and acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=1e-9)
+    assert _close(tr[-1], el_o, 1e-9)
+    # the inverses are rebuilt before single-electron sweeps continue
+    ens.sweeps(1, 0.2)
+    la, sg, el = ens.state()
+    ref = o.batch_eval(ens.positions())
+    assert _close(la, ref["logabs"], 1e-10)
+
+
+@pytest.mark.parametrize("name", ["h2", "li_seeded", "ch4_sto3g"])
+def test_wave_all_electron_matches_register_engine(Q, oracle, name):
+    from test_gpu_parity import _pair
+    out = []
+    for eng in ("auto", "wave"):
+        g, _ = _pair(Q, oracle, name)
+        g.set_engine(eng)
+        ens = Q.Ensemble(g, 50, 3)
+        es, e2, acc = ens.sweeps(6, 0.3, Q.QMC_MOVE_ALL)
+        out.append((es, acc, ens.positions()))
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.allclose(out[0][2], out[1][2], rtol=0, atol=1e-12)
+    assert np.allclose(out[0][0], out[1][0], rtol=1e-10, atol=0)
