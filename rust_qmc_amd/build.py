@@ -33,9 +33,15 @@ FLAGS = [
 ]
 
 
+# per-source flags: the Gram kernel keeps its MFMA accumulators in VGPRs
+# (the AGPR form copied all 168 accumulator registers through VGPRs on
+# every loop iteration)
+EXTRA = {"qmc_derivs.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=true"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = ["hipcc", *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = ["hipcc", *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

@@ -478,29 +478,30 @@ template hipError_t wave_derivs<float>(const void*, const void*, DerivArgs, cons
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+constexpr int GRAM_WAVES = 4;  // waves per workgroup, one per SIMD
+
 template <int NB>
-__global__ void __launch_bounds__(64) gram_kernel(const double* __restrict__ Y, int64_t ldy, int64_t W, int chunk,
-                                                  double* __restrict__ partial) {
-  const int lane = threadIdx.x;
-  const int64_t w0 = (int64_t)blockIdx.x * chunk;
-  const int64_t w1 = w0 + chunk < W ? w0 + chunk : W;
+__global__ void __launch_bounds__(64 * GRAM_WAVES) gram_kernel(const double* __restrict__ Y, int64_t ldy, int64_t Wp,
+                                                               int64_t chunk, double* __restrict__ partial) {
   constexpr int NT = NB * (NB + 1) / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the workgroup's walkers [w0, w1), both multiples of 16; at step s wave
+  // v takes walkers w0 + 16 s + 4 v .. + 3, so the four waves read 16
+  // consecutive walkers.  Columns up to Wp (W rounded up to 16) are
+  // readable and zero past W, so no load needs a guard.
+  const int64_t w0 = (int64_t)blockIdx.x * chunk;
+  const int64_t w1 = w0 + chunk < Wp ? w0 + chunk : Wp;
+  const int nsteps = w1 > w0 ? (int)((w1 - w0) / 16) : 0;
   dbl4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
   const int fr = lane & 15, kk = lane >> 4;
-  // the next step's fragments are loaded before this step's MFMAs, so the
-  // loads overlap the matrix work instead of waiting in front of it
-  double fn[NB];
+  const double* yl = Y + (int64_t)fr * ldy + 4 * wave + kk;
+  auto load = [&](double* f, int s) {
 #pragma unroll
-  for (int b = 0; b < NB; ++b) fn[b] = w0 + kk < w1 ? Y[(int64_t)(16 * b + fr) * ldy + w0 + kk] : 0.0;
-  for (int64_t k0 = w0; k0 < w1; k0 += 4) {
-    double f[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) f[b] = fn[b];
-    const int64_t wn = k0 + 4 + kk;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) fn[b] = wn < w1 ? Y[(int64_t)(16 * b + fr) * ldy + wn] : 0.0;
+    for (int b = 0; b < NB; ++b) f[b] = yl[(int64_t)16 * b * ldy + w0 + 16 * s];
+  };
+  auto mfmas = [&](const double* f) {
     int t = 0;
 #pragma unroll
     for (int I = 0; I < NB; ++I)
@@ -509,30 +510,71 @@ __global__ void __launch_bounds__(64) gram_kernel(const double* __restrict__ Y, 
         acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(f[I], f[J], acc[t], 0, 0, 0);
         ++t;
       }
+  };
+  // double buffer: each step's fragments are loaded one step ahead, and
+  // the scheduling barriers keep the loads in front of the MFMAs that do
+  // not need them (a last, unused load re-reads the final step)
+  double fa[NB], fb[NB];
+  if (nsteps > 0) load(fa, 0);
+  int s = 0;
+  for (; s + 2 <= nsteps; s += 2) {
+    load(fb, s + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    mfmas(fa);
+    __builtin_amdgcn_sched_barrier(0);
+    load(fa, s + 2 < nsteps ? s + 2 : nsteps - 1);
+    __builtin_amdgcn_sched_barrier(0);
+    mfmas(fb);
+    __builtin_amdgcn_sched_barrier(0);
   }
-  // partial [block][Qp][Qp] (upper tiles only)
+  if (s < nsteps) mfmas(fa);
+  // the four waves' accumulators summed in LDS in wave order (deterministic)
+  extern __shared__ double red[];  // [NT][4][64]
+  for (int v = 0; v < GRAM_WAVES; ++v) {
+    if (wave == v) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double* q = red + (t * 4 + r) * 64 + lane;
+          *q = v == 0 ? acc[t][r] : *q + acc[t][r];
+        }
+    }
+    __syncthreads();
+  }
+  // partial [block][Qp][Qp] (upper tiles only); slot (t, r, lane) is
+  // row 16 I + 4 r + lane / 16, column 16 J + lane % 16
   const int Qp = 16 * NB;
   double* out = partial + (int64_t)blockIdx.x * Qp * Qp;
-  int t = 0;
-#pragma unroll
-  for (int I = 0; I < NB; ++I)
-#pragma unroll
-    for (int J = I; J < NB; ++J) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) out[(16 * I + 4 * r + kk) * Qp + 16 * J + fr] = acc[t][r];
-      ++t;
+  for (int i = threadIdx.x; i < NT * 4 * 64; i += 64 * GRAM_WAVES) {
+    const int t = i >> 8, r = (i >> 6) & 3, l = i & 63;
+    int I = 0, tt = t;  // tile t -> (I, J >= I) in the order the MFMA loop forms them
+    while (tt >= NB - I) {
+      tt -= NB - I;
+      ++I;
     }
+    const int J = I + tt;
+    out[(16 * I + 4 * r + (l >> 4)) * Qp + 16 * J + (l & 15)] = red[i];
+  }
 }
 
-// deterministic sum of the per-chunk partial Grams (upper triangle)
+// deterministic sum of the per-workgroup partial Grams: four threads per
+// element each sum every fourth partial in order, then the four sums are
+// added in a fixed order
 __global__ void __launch_bounds__(256) gram_reduce_kernel(const double* __restrict__ partial, int nblk, int Qp,
                                                           double* __restrict__ G) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= (int64_t)Qp * Qp) return;
+  __shared__ double sh[4][64];
+  const int part = threadIdx.x >> 6, el = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 64 + el;
+  const int64_t n = (int64_t)Qp * Qp;
   double s = 0.0;
+  if (i < n) {
 #pragma unroll 16
-  for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * Qp * Qp + i];  // loads in flight together, same order
-  G[i] = s;
+    for (int b = part; b < nblk; b += 4) s += partial[(int64_t)b * n + i];  // loads in flight together
+  }
+  sh[part][el] = s;
+  __syncthreads();
+  if (part == 0 && i < n) G[i] = ((sh[0][el] + sh[1][el]) + sh[2][el]) + sh[3][el];
 }
 
 template <int NU, int ND, class Bs, class T>
@@ -585,12 +627,28 @@ const DerivOps* find_derivs(const DevSys<double>& D, bool generic_only) {
   return nullptr;
 }
 
-hipError_t launch_gram(const double* Y, int64_t ldy, int64_t W, int Qp, double* partial, int chunk, double* G,
-                       hipStream_t st) {
-  const int nblk = (int)((W + chunk - 1) / chunk);
+int gram_blocks(int64_t W) {
+  const int64_t n = (W + 255) / 256;
+  return (int)(n < 1 ? 1 : (n > 256 ? 256 : n));
+}
+
+hipError_t launch_gram(const double* Y, int64_t ldy, int64_t W, int Qp, double* partial, double* G, hipStream_t st) {
+  const int64_t Wp = (W + 15) / 16 * 16;
+  if (W < 1 || ldy < Wp) return hipErrorInvalidValue;
+  const int nblk = gram_blocks(W);
+  // walkers per workgroup: a multiple of the 16 one step of its four waves takes
+  const int64_t chunk = ((Wp + nblk - 1) / nblk + 15) / 16 * 16;
+  const size_t lds = sizeof(double) * (size_t)((Qp / 16) * (Qp / 16 + 1) / 2) * 4 * 64;
   switch (Qp / 16) {
-#define QMC_GRAM_CASE(nb) \
-  case nb: gram_kernel<nb><<<nblk, 64, 0, st>>>(Y, ldy, W, chunk, partial); break;
+#define QMC_GRAM_CASE(nb)                                                                                    \
+  case nb:                                                                                                   \
+    if (lds > 64 * 1024) {                                                                                   \
+      hipError_t ea = hipFuncSetAttribute((const void*)gram_kernel<nb>,                                     \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);             \
+      if (ea != hipSuccess) return ea;                                                                       \
+    }                                                                                                        \
+    gram_kernel<nb><<<nblk, 64 * GRAM_WAVES, lds, st>>>(Y, ldy, Wp, chunk, partial);                         \
+    break;
     QMC_GRAM_CASE(1)
     QMC_GRAM_CASE(2)
     QMC_GRAM_CASE(3)
@@ -606,7 +664,7 @@ hipError_t launch_gram(const double* Y, int64_t ldy, int64_t W, int Qp, double* 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int64_t n = (int64_t)Qp * Qp;
-  gram_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(partial, nblk, Qp, G);
+  gram_reduce_kernel<<<(unsigned)((n + 63) / 64), 256, 0, st>>>(partial, nblk, Qp, G);
   return hipGetLastError();
 }
 

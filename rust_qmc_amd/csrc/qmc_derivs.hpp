@@ -52,8 +52,9 @@ const DerivOps* find_derivs(const DevSys<double>& D, bool generic_only);
 template <class T>
 hipError_t wave_derivs(const void* Sd, const void* St, DerivArgs A, const WaveShape& sh, hipStream_t st);
 // G[Qp][Qp] (upper triangle valid) = Y Y^T over W walkers; partial holds
-// ceil(W/chunk) * Qp * Qp doubles.
-hipError_t launch_gram(const double* Y, int64_t ldy, int64_t W, int Qp, double* partial, int chunk, double* G,
-                       hipStream_t st);
+// gram_blocks(W) * Qp * Qp doubles.  Y's columns W .. W rounded up to 16
+// must exist (ldy >= that) and hold zeros.
+int gram_blocks(int64_t W);
+hipError_t launch_gram(const double* Y, int64_t ldy, int64_t W, int Qp, double* partial, double* G, hipStream_t st);
 
 }  // namespace qmc

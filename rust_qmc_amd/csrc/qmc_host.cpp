@@ -1557,8 +1557,7 @@ int qmc_ensemble_measure(qmc_ensemble* e, int32_t reset) {
   int rc;
   if (e->sys_version != s->version && (rc = ens_refresh(e, 1))) return rc;
   const Cols C(s->h);
-  const int chunk = 256;
-  const int nblk = (int)((e->W + chunk - 1) / chunk);
+  const int nblk = gram_blocks(e->W);
   if ((rc = ensure_cap(&e->Y, &e->Y_cap, (size_t)C.Qp * e->ld))) return rc;
   if ((rc = ensure_cap(&e->gpart, &e->gpart_cap, (size_t)nblk * C.Qp * C.Qp))) return rc;
   if ((rc = ensure_cap(&e->G, &e->G_cap, (size_t)C.Qp * C.Qp))) return rc;
@@ -1568,7 +1567,9 @@ int qmc_ensemble_measure(qmc_ensemble* e, int32_t reset) {
   else
     A.pos32 = (const float*)e->pos;
   if ((rc = launch_derivs(s, A, e->prec, e->st))) return rc;
-  HIP_TRY(launch_gram(e->Y, e->ld, e->W, C.Qp, e->gpart, chunk, e->G, e->st));
+  if (e->ld > e->W)  // the Gram reads the padding columns (up to W rounded to 16) as zeros
+    HIP_TRY(hipMemset2DAsync(e->Y + e->W, sizeof(double) * e->ld, 0, sizeof(double) * (e->ld - e->W), C.Qp, e->st));
+  HIP_TRY(launch_gram(e->Y, e->ld, e->W, C.Qp, e->gpart, e->G, e->st));
   std::vector<double> g((size_t)C.Qp * C.Qp);
   HIP_TRY(hipMemcpyAsync(g.data(), e->G, g.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
   HIP_TRY(hipStreamSynchronize(e->st));
