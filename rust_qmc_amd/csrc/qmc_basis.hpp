@@ -46,6 +46,7 @@ struct RtShell {
 };
 
 struct RtBasis {
+  static constexpr int n_nuc_ct = 0;  // nucleus count unknown at compile time
   template <class T, class F>
   static __device__ __forceinline__ void shells(const DevSys<T>& S, F&& f) {
     for (int i = 0; i < S.n_shell; ++i) f(RtShell<T>{S, i});
@@ -83,6 +84,7 @@ struct CtShell {
 
 template <class Bd>
 struct CtBasis {
+  static constexpr int n_nuc_ct = Bd::n_nuc;
   template <class T, class F>
   static __device__ __forceinline__ void shells(const DevSys<T>&, F&& f) {
     static_for<0, Bd::n_shell>([&](auto I) { f(CtShell<Bd, decltype(I)::value>{}); });

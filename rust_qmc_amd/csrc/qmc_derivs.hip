@@ -42,18 +42,68 @@ struct Row {
   __device__ __forceinline__ T& operator[](int q) const { return base[q * 64]; }
 };
 
+// Omega and HF-force sums per nucleus.  With a compiled shell structure
+// (NM > 0 nuclei, indices known at compile time) they stay in registers
+// over all electrons and reach the LDS rows once; otherwise every term goes
+// to the rows.
+template <class T, int NM>
+struct NucAcc {
+  T om[NM > 0 ? NM : 1][3], f[NM > 0 ? NM : 1][3];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int I = 0; I < (NM > 0 ? NM : 1); ++I)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) om[I][c] = f[I][c] = T(0);
+  }
+  template <class RowT>
+  __device__ __forceinline__ void add_om(RowT row, const DerivLayout& L, int I, int c, T v) {
+    if constexpr (NM > 0)
+      om[I][c] += v;
+    else
+      row[L.om(I, c)] += v;
+  }
+  template <class RowT>
+  __device__ __forceinline__ void add_f(RowT row, const DerivLayout& L, int I, int c, T v) {
+    if constexpr (NM > 0)
+      f[I][c] += v;
+    else
+      row[L.f(I, c)] += v;
+  }
+  // the register sums into the rows: all reads first, then all writes
+  template <class RowT>
+  __device__ __forceinline__ void flush(RowT row, const DerivLayout& L) {
+    if constexpr (NM > 0) {
+      T a[NM][6];
+#pragma unroll
+      for (int I = 0; I < NM; ++I)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          a[I][c] = row[L.om(I, c)];
+          a[I][3 + c] = row[L.f(I, c)];
+        }
+#pragma unroll
+      for (int I = 0; I < NM; ++I)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          row[L.om(I, c)] = a[I][c] + om[I][c];
+          row[L.f(I, c)] = a[I][3 + c] + f[I][c];
+        }
+    }
+  }
+};
+
 // Pass 1 for the electron in slot 0 of sector A (combined index q):
 // determinant and Jastrow gradients, E_L terms, Omega, HF force.
-template <int NA, int NB, class T, class Bs>
+template <int NA, int NB, class T, class Bs, int NM>
 __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, int q, const T ra[Cap<NA>::v][3],
                                                 const T ai[Cap<NA>::v][Cap<NA>::v], const T rb[Cap<NB>::v][3],
-                                                const DerivLayout& L, Row<T> row, double& pot) {
+                                                const DerivLayout& L, Row<T> row, NucAcc<T, NM>& acc, double& pot) {
   const T x = ra[0][0], y = ra[0][1], z = ra[0][2];
   T gdx, gdy, gdz, ld;
   auto on_shell = [&](int c, T sx, T sy, T sz) {
-    row[L.om(c, 0)] -= sx;
-    row[L.om(c, 1)] -= sy;
-    row[L.om(c, 2)] -= sz;
+    acc.add_om(row, L, c, 0, -sx);
+    acc.add_om(row, L, c, 1, -sy);
+    acc.add_om(row, L, c, 2, -sz);
   };
   det_terms<NA, T, decltype(on_shell), Bs>(S, spin, x, y, z, ai, gdx, gdy, gdz, ld, on_shell);
   T gux = 0, guy = 0, guz = 0, lu = 0, pairsq = 0;
@@ -85,9 +135,9 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
     pe -= (double)(S.Z[I] * ir);
     if (r > T(1e-10)) {  // methane.rs:1253
       const T f3 = S.Z[I] * ir * ir * ir;
-      row[L.f(I, 0)] += f3 * dx;
-      row[L.f(I, 1)] += f3 * dy;
-      row[L.f(I, 2)] += f3 * dz;
+      acc.add_f(row, L, I, 0, f3 * dx);
+      acc.add_f(row, L, I, 1, f3 * dy);
+      acc.add_f(row, L, I, 2, f3 * dz);
     }
     if (S.en_on) {
       T u1, lp;
@@ -99,9 +149,9 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
         guz += f * dz;
         lu += lp;
         // d u_en / d R_I = -u1 (r_i - R_I)/r   (methane.rs:1071-1090)
-        row[L.om(I, 0)] -= f * dx;
-        row[L.om(I, 1)] -= f * dy;
-        row[L.om(I, 2)] -= f * dz;
+        acc.add_om(row, L, I, 0, -f * dx);
+        acc.add_om(row, L, I, 1, -f * dy);
+        acc.add_om(row, L, I, 2, -f * dz);
       }
     }
   });
@@ -116,58 +166,60 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
   return -0.5 * (double)lap;
 }
 
-// O_k and d E_L / d p_k of one electron pair (Pade Jastrow only); (dx, dy,
-// dz) = r_i - r_j and gd(c) = G_i[c] - G_j[c], the difference of the two
-// electrons' d ln psi / d r (read only when the pair is apart).
-template <class T, class RowT, class GD>
-__device__ __forceinline__ void ee_pair_derivs(const DevSys<T>& S, T dx, T dy, T dz, bool same, const DerivLayout& L,
-                                               RowT row, GD gd) {
+// O_k and d E_L / d p_k terms of one electron pair (Pade Jastrow only),
+// handed to addO(slot, v) and addE(slot, v) by parameter slot: 0 = b_ee,
+// 1 = a (a_para for same spins, a_anti otherwise), 2 + k = series term k.
+// (dx, dy, dz) = r_i - r_j and gd(c) = G_i[c] - G_j[c], the difference of
+// the two electrons' d ln psi / d r (read only when the pair is apart).
+template <class T, class GD, class AO, class AE>
+__device__ __forceinline__ void ee_pair_terms(const DevSys<T>& S, T dx, T dy, T dz, bool same, GD gd, AO addO,
+                                              AE addE) {
   const int K = S.k_ee;
   T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
   const T a = same ? S.a_para : S.a_anti;
   const T* c = same ? S.c_para : S.c_anti;
   PairDerivs<T> pd;
   pd.init(r, S.b_ee, a, K, c);
-  const int ia = same ? 3 : 2, c0 = 4 + (same ? K : 0);
   // O: d u / d b_ee = -(a + s') x^2, d u / d a = x, d u / d c_k = x^(k+2)
-  row[L.o(0)] += -pd.A1 * pd.x * pd.x;
-  row[L.o(ia)] += pd.x;
+  addO(0, -pd.A1 * pd.x * pd.x);
+  addO(1, pd.x);
   if (r < T(1e-10)) return;
   const T ir = T(1) / r;
   // Delta = (G_i - G_j) . rhat_ij
   const T dlt = gd(0) * (dx * ir) + gd(1) * (dy * ir) + gd(2) * (dz * ir);
   T du1, dl;
   pd.d_b(r, S.b_ee, du1, dl);
-  row[L.de(0)] += -(dl + dlt * du1);
+  addE(0, -(dl + dlt * du1));
   pd.d_a(r, S.b_ee, du1, dl);
-  row[L.de(ia)] += -(dl + dlt * du1);
+  addE(1, -(dl + dlt * du1));
   T xk = T(1);
-  for (int k = 0; k < K; ++k) {
-    row[L.o(c0 + k)] += xk * pd.x * pd.x;
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {  // unrolled, so the slot is a constant
+    if (k >= K) break;
+    addO(2 + k, xk * pd.x * pd.x);
     pd.d_c(k, r, S.b_ee, xk, du1, dl);
-    row[L.de(c0 + k)] += -(dl + dlt * du1);
+    addE(2 + k, -(dl + dlt * du1));
     xk *= pd.x;
   }
 }
 
-// The same for electron i and nucleus I; (dx, dy, dz) = r_i - R_I and
-// g(c) = G_i[c].
-template <class T, class RowT, class GI>
-__device__ __forceinline__ void en_pair_derivs(const DevSys<T>& S, int I, T dx, T dy, T dz, const DerivLayout& L,
-                                               RowT row, GI g) {
+// The same for electron i and nucleus I (slot 0 = b_en, 2 + k = the term k
+// of the nucleus' species); (dx, dy, dz) = r_i - R_I and g(c) = G_i[c].
+template <class T, class GI, class AO, class AE>
+__device__ __forceinline__ void en_pair_terms(const DevSys<T>& S, int I, T dx, T dy, T dz, GI g, AO addO, AE addE) {
   const int Ke = S.k_en;
   T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-  const int sp = S.species[I];
-  const T* c = S.d_en[sp];
+  const T* c = S.d_en[S.species[I]];
   PairDerivs<T> pd;
   pd.init(r, S.b_en, -S.Z[I], Ke, c);
   // d u_en / d b_en = (Z - s') y^2 = -A1 y^2
-  row[L.o(1)] += -pd.A1 * pd.x * pd.x;
-  const int c0 = 4 + 2 * S.k_ee + sp * Ke;
+  addO(0, -pd.A1 * pd.x * pd.x);
   if (r < T(1e-10)) {
     T xk = T(1);
-    for (int k = 0; k < Ke; ++k) {
-      row[L.o(c0 + k)] += xk * pd.x * pd.x;
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) {
+      if (k >= Ke) break;
+      addO(2 + k, xk * pd.x * pd.x);
       xk *= pd.x;
     }
     return;
@@ -176,34 +228,91 @@ __device__ __forceinline__ void en_pair_derivs(const DevSys<T>& S, int I, T dx, 
   const T dlt = g(0) * (dx * ir) + g(1) * (dy * ir) + g(2) * (dz * ir);
   T du1, dl;
   pd.d_b(r, S.b_en, du1, dl);
-  row[L.de(1)] += -(T(0.5) * dl + dlt * du1);
+  addE(0, -(T(0.5) * dl + dlt * du1));
   T xk = T(1);
-  for (int k = 0; k < Ke; ++k) {
-    row[L.o(c0 + k)] += xk * pd.x * pd.x;
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    if (k >= Ke) break;
+    addO(2 + k, xk * pd.x * pd.x);
     pd.d_c(k, r, S.b_en, xk, du1, dl);
-    row[L.de(c0 + k)] += -(T(0.5) * dl + dlt * du1);
+    addE(2 + k, -(T(0.5) * dl + dlt * du1));
     xk *= pd.x;
   }
 }
 
-// Pass 2: unique pairs -> O_k and d E_L / d p_k (Pade Jastrow only)
+// feature columns of a pair slot: e-e (class 0 anti, 1 para) and e-n
+// (species sp) Jastrow parameters in the order [b_ee, b_en, a_anti,
+// a_para, c_anti[K], c_para[K], d_en[species][Ke]]
+template <class T>
+__device__ __forceinline__ int ee_slot_col(const DevSys<T>& S, int cls, int slot) {
+  return slot == 0 ? 0 : (slot == 1 ? 2 + cls : 4 + cls * S.k_ee + slot - 2);
+}
+template <class T>
+__device__ __forceinline__ int en_slot_col(const DevSys<T>& S, int sp, int slot) {
+  return slot == 0 ? 1 : 4 + 2 * S.k_ee + sp * S.k_en + slot - 2;
+}
+
+// Pass 2: unique pairs -> O_k and d E_L / d p_k (Pade Jastrow only).  The
+// e-e terms are summed in registers per spin class over all pairs and the
+// e-n terms per nucleus over all electrons; each sum reaches the LDS rows
+// once (all reads of a flush first, then the writes).
 template <int NU, int ND, class T, class Bs>
 __device__ __forceinline__ void pair_pass(const DevSys<T>& S, const T ru[Cap<NU>::v][3],
                                           const T rd[Cap<ND>::v][3], const DerivLayout& L, Row<T> row) {
   constexpr int N = NU + ND;
+  constexpr int NS = 2 + MAXK;
   auto pos = [&](int q, int c) -> T { return q < NU ? ru[q][c] : rd[q - NU][c]; };
+  // flush of the slot sums [first, nslot) (slot 1 skipped unless has_a)
+  // into the columns col(slot)
+  auto flush = [&](const T (&o)[NS], const T (&e)[NS], int first, int nslot, bool has_a, auto col) {
+    T ro[NS], re[NS];
+#pragma unroll
+    for (int sl = 0; sl < NS; ++sl)
+      if (sl >= first && sl < nslot && (sl != 1 || has_a)) {
+        ro[sl] = row[L.o(col(sl))];
+        re[sl] = row[L.de(col(sl))];
+      }
+#pragma unroll
+    for (int sl = 0; sl < NS; ++sl)
+      if (sl >= first && sl < nslot && (sl != 1 || has_a)) {
+        row[L.o(col(sl))] = ro[sl] + o[sl];
+        row[L.de(col(sl))] = re[sl] + e[sl];
+      }
+  };
+  T eo[2][NS], ee_[2][NS];
+#pragma unroll
+  for (int cl = 0; cl < 2; ++cl)
+#pragma unroll
+    for (int sl = 0; sl < NS; ++sl) eo[cl][sl] = ee_[cl][sl] = T(0);
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = i + 1; j < N; ++j)
-      ee_pair_derivs(S, pos(i, 0) - pos(j, 0), pos(i, 1) - pos(j, 1), pos(i, 2) - pos(j, 2), (i < NU) == (j < NU), L,
-                     row, [=](int c) { return row[L.g(i, c)] - row[L.g(j, c)]; });
+    for (int j = i + 1; j < N; ++j) {
+      const int cl = ((i < NU) == (j < NU)) ? 1 : 0;
+      ee_pair_terms(S, pos(i, 0) - pos(j, 0), pos(i, 1) - pos(j, 1), pos(i, 2) - pos(j, 2), cl == 1,
+                    [=](int c) { return row[L.g(i, c)] - row[L.g(j, c)]; },
+                    [&](int sl, T v) { eo[cl][sl] += v; }, [&](int sl, T v) { ee_[cl][sl] += v; });
+    }
+  // b_ee (slot 0) is one column for both classes
+  eo[1][0] += eo[0][0];
+  ee_[1][0] += ee_[0][0];
+  const int nse = 2 + S.k_ee;
 #pragma unroll
-  for (int i = 0; i < N; ++i)
-    Bs::nuclei(S, [&](int I) {
-      en_pair_derivs(S, I, pos(i, 0) - S.nuc[I][0], pos(i, 1) - S.nuc[I][1], pos(i, 2) - S.nuc[I][2], L, row,
-                     [=](int c) { return row[L.g(i, c)]; });
-    });
+  for (int cl = 0; cl < 2; ++cl)
+    flush(eo[cl], ee_[cl], cl == 0 ? 1 : 0, nse, true, [&](int sl) { return ee_slot_col(S, cl, sl); });
+  const int nsn = 2 + S.k_en;
+  Bs::nuclei(S, [&](int I) {
+    T no[NS], ne[NS];
+#pragma unroll
+    for (int sl = 0; sl < NS; ++sl) no[sl] = ne[sl] = T(0);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      en_pair_terms(S, I, pos(i, 0) - S.nuc[I][0], pos(i, 1) - S.nuc[I][1], pos(i, 2) - S.nuc[I][2],
+                    [=](int c) { return row[L.g(i, c)]; }, [&](int sl, T v) { no[sl] += v; },
+                    [&](int sl, T v) { ne[sl] += v; });
+    const int sp = S.species[I];
+    flush(no, ne, 0, nsn, false, [&](int sl) { return en_slot_col(S, sp, sl); });
+  });
 }
 
 template <int NU, int ND, class Bs, class T>
@@ -257,16 +366,19 @@ __global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __rest
     for (int m = 0; m < Cap<ND>::v; ++m) R.ad[k][m] = (T)D.ad[k][m];
   }
   double kin = 0.0, pot = 0.0;
+  NucAcc<T, Bs::n_nuc_ct> acc;
+  acc.zero();
 #pragma unroll 1
   for (int k = 0; k < NU; ++k) {
-    kin += electron_pass<NU, ND, T, Bs>(S, 0, k, R.ru, R.au, R.rd, L, row, pot);
+    kin += electron_pass<NU, ND, T, Bs>(S, 0, k, R.ru, R.au, R.rd, L, row, acc, pot);
     rotate<NU>(R.ru, R.au);
   }
 #pragma unroll 1
   for (int k = 0; k < ND; ++k) {
-    kin += electron_pass<ND, NU, T, Bs>(S, 1, NU + k, R.rd, R.ad, R.ru, L, row, pot);
+    kin += electron_pass<ND, NU, T, Bs>(S, 1, NU + k, R.rd, R.ad, R.ru, L, row, acc, pot);
     rotate<ND>(R.rd, R.ad);
   }
+  acc.flush(row, L);
   const double el = kin + pot + S.vnn;
   row[L.one()] = T(1);
   if (S.jas_kind == JAS_PADE) pair_pass<NU, ND, T, Bs>(S, R.ru, R.rd, L, row);
@@ -418,14 +530,19 @@ __global__ void __launch_bounds__(64) wave_derivs_kernel(const DevSys<double>* _
         ++i;
       }
       const int j = i + 1 + r;
-      ee_pair_derivs(S, L[3 * i] - L[3 * j], L[3 * i + 1] - L[3 * j + 1], L[3 * i + 2] - L[3 * j + 2],
-                     S.elec_spin[i] == S.elec_spin[j], Ly, row,
-                     [&](int c) { return G[3 * i + c] - G[3 * j + c]; });
+      const int cl = S.elec_spin[i] == S.elec_spin[j] ? 1 : 0;
+      ee_pair_terms(S, L[3 * i] - L[3 * j], L[3 * i + 1] - L[3 * j + 1], L[3 * i + 2] - L[3 * j + 2], cl == 1,
+                    [&](int c) { return G[3 * i + c] - G[3 * j + c]; },
+                    [&](int sl, T v) { row[Ly.o(ee_slot_col(S, cl, sl))] += v; },
+                    [&](int sl, T v) { row[Ly.de(ee_slot_col(S, cl, sl))] += v; });
     }
     for (int p = lane; p < N * Mn; p += 64) {
       const int i = p / Mn, I = p % Mn;
-      en_pair_derivs(S, I, L[3 * i] - S.nuc[I][0], L[3 * i + 1] - S.nuc[I][1], L[3 * i + 2] - S.nuc[I][2], Ly, row,
-                     [&](int c) { return G[3 * i + c]; });
+      const int sp = S.species[I];
+      en_pair_terms(S, I, L[3 * i] - S.nuc[I][0], L[3 * i + 1] - S.nuc[I][1], L[3 * i + 2] - S.nuc[I][2],
+                    [&](int c) { return G[3 * i + c]; },
+                    [&](int sl, T v) { row[Ly.o(en_slot_col(S, sp, sl))] += v; },
+                    [&](int sl, T v) { row[Ly.de(en_slot_col(S, sp, sl))] += v; });
     }
   }
   const double el = wsum(kin) + wsum(pot) + S.vnn;
