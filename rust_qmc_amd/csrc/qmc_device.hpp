@@ -342,7 +342,11 @@ struct NoShell {
 template <class T>
 __device__ __forceinline__ void series(int K, const T* c, T x, T& s, T& s1, T& s2) {
   T v = 0, v1 = 0, v2 = 0, xp = T(1);
-  for (int k = 0; k < K; ++k) {
+  // unrolled to the maximum length: the coefficient addresses are constants,
+  // so their scalar loads are issued together and can be hoisted
+#pragma unroll
+  for (int k = 0; k < MAXK; ++k) {
+    if (k >= K) break;
     const T n = T(k + 2);
     v += c[k] * xp * x * x;
     v1 += c[k] * n * xp * x;
@@ -358,7 +362,9 @@ __device__ __forceinline__ void series(int K, const T* c, T x, T& s, T& s1, T& s
 template <class T>
 __device__ __forceinline__ T series3(int K, const T* c, T x) {
   T v = 0, xp = T(1);  // x^(k-1) for k >= 1
-  for (int k = 1; k < K; ++k) {
+#pragma unroll
+  for (int k = 1; k < MAXK; ++k) {
+    if (k >= K) break;
     const T n = T(k + 2);
     v += c[k] * n * (n - T(1)) * (n - T(2)) * xp;
     xp *= x;
