@@ -132,6 +132,36 @@ def run_batch(Q, sweeps, trial):
     return out
 
 
+def run_benzene(Q, sweeps, W=8192):
+    """Benzene (BenzeneGTO, 42 electrons, 66 AOs, C-H 2.05 bohr), 8192
+    walkers: Metropolis single-electron sweeps (mixed: tile engine, MFMA MO
+    contraction; FP64: wave engine), drift-diffusion sweeps (wave engine,
+    tau = 0.01) and SR-grad samples (wave derivative kernel: E_L, the 4
+    Jastrow O_k and dE_L/dp_k, HF force and Omega over 12 nuclei with the
+    C-H MOs following the bonds) -- kernel time from HIP events."""
+    out = {"walkers": W}
+    for prec in ("mixed", "fp64"):
+        s = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision=prec)
+        ens = Q.Ensemble(s, W, 3)
+        ens.sweeps(4, 0.2, refresh_interval=64)
+        r = {"engine": s.engine_name()}
+        for name, move, step in (("metropolis", Q.QMC_MOVE_SINGLE, 0.2), ("drift_diffusion", Q.QMC_MOVE_DRIFT, 0.01)):
+            ens.set_timing(True)
+            es, e2, acc = ens.sweeps(sweeps, step, move=move, refresh_interval=64, steps_per_launch=sweeps)
+            kms, nl = ens.timing()
+            ens.set_timing(False)
+            r[name] = {"walker_moves_per_s": W * sweeps / (kms / 1e3), "ms_per_sweep": kms / sweeps,
+                       "acceptance": float(acc.sum() / (W * 42 * sweeps)), "energy": float(es.mean() / W)}
+        ens.measure(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(4):
+            ens.measure()
+        wall = time.perf_counter() - t0
+        r["sr_grad_samples_per_s"] = 4 * W / wall
+        out[prec] = r
+    return out
+
+
 def run_isdmc(Q, precision, W, n_steps):
     """Importance-sampled DMC on CH4 STO-3G: one drift-diffusion sweep plus
     the branching, compaction and E_ref feedback per step, all on the
@@ -212,6 +242,7 @@ def main():
     ap.add_argument("--batch-sweeps", type=int, default=32,
                     help="timed sweeps of the benzene + CH4 + H2 batch (0: skip it)")
     ap.add_argument("--dmc-steps", type=int, default=100, help="IS-DMC steps per precision (0: skip)")
+    ap.add_argument("--benzene-sweeps", type=int, default=8, help="timed benzene sweeps per leg (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 8 device-to-device copies of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -234,6 +265,7 @@ def main():
     sr = {m: run_sr(Q, m, W, args.sr_calls, 16) for m in ("fp64", "mixed")}
     batch = run_batch(Q, args.batch_sweeps, 4) if args.batch_sweeps > 0 else None
     dmc = {m: run_isdmc(Q, m, W, args.dmc_steps) for m in ("mixed", "fp64")} if args.dmc_steps > 0 else None
+    benzene = run_benzene(Q, args.benzene_sweeps) if args.benzene_sweeps > 0 else None
     mx, fp = res["mixed"], res["fp64"]
     # roofline of the dominant kernel (the mixed-precision sweep)
     avg_launch_s = mx["kernel_ms"] / 1e3 / mx["launches"]
@@ -292,6 +324,7 @@ def main():
                                                            "fp64": sr["fp64"]["ms_per_call"]}},
         "heterogeneous_batch": batch,
         "isdmc": dmc,
+        "benzene": benzene,
         "roofline": {
             "bound": "mfma",
             "engine": "vector ALU: the sweep has no GEMM-shaped work; 157.3 TFLOP/s is MI355X's FP32 peak "
