@@ -16,6 +16,8 @@
 // estimator (geometry_opt.rs:174-226) need.
 #include "qmc_derivs.hpp"
 
+#include <type_traits>
+
 #include "qmc_engine.hpp"
 #include "qmc_wave.hpp"
 #include "qmc_wave_dev.hpp"
@@ -387,6 +389,173 @@ __global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __rest
 }
 
 // ---------------------------------------------------------------------------
+// Two wavefronts per 64 walkers for equal spin sectors (NS up, NS down) with
+// a compiled shell structure: wave s owns sector s -- its inverse, the
+// electron pass of its NS electrons, its same-spin pairs, half of the
+// opposite-spin pairs and the electron-nucleus pairs of its electrons -- so
+// each SIMD holds two waves instead of one.  The sector is wave-uniform, so
+// every parameter load stays a scalar load.  The two waves share one LDS
+// row per walker (element q of lane t at lds[q*64 + t]): each writes the
+// gradients G of its own electrons, and the register sums reach the row in
+// wave order (wave 0's read-modify-write, a barrier, wave 1's), so the
+// result does not depend on timing.
+//
+// Opposite-spin pair (u_a, d_b) has offset (b - a) mod NS seen from wave 0
+// and (a - b) mod NS from wave 1.  With m = NS / 2, wave 0 takes offsets
+// 0..m and wave 1 offsets 1..m, except that for even NS the offset-m class
+// (seen from both waves) is split by k < m: every pair is taken once.
+// ---------------------------------------------------------------------------
+template <int NS>
+__device__ __forceinline__ bool anti_pair_taken(int wave, int k, int d) {
+  constexpr int m = NS / 2;
+  if (d > m) return false;
+  if (wave == 1 && d == 0) return false;
+  if (NS % 2 == 0 && d == m) return k < m;
+  return true;
+}
+
+template <int NS, class Bs, class T>
+__global__ void __launch_bounds__(128) derivs2_kernel(const DevSys<double>* __restrict__ Sd,
+                                                     const DevSys<T>* __restrict__ St, DerivArgs A) {
+  static_assert(Bs::n_nuc_ct > 0, "the nucleus sums must be compile-time register arrays");
+  extern __shared__ double lds_raw[];
+  const DevSys<T>& S = *St;
+  constexpr int N = 2 * NS, NSL = 2 + MAXK;
+  const int lane = threadIdx.x & 63;
+  const int sp = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // this wave's sector
+  const int64_t w = (int64_t)blockIdx.x * 64 + lane;
+  const bool live = w < A.W;
+  const int64_t wl = live ? w : A.W - 1;
+  DerivLayout L{A.P, A.M, N, A.Qp};
+  const int nq = A.Qp + 3 * N;
+  T* lds = (T*)lds_raw;
+  double* xch = lds_raw + ((size_t)nq * 64 * sizeof(T) + 7) / 8;  // [2][64] E_L parts
+  Row<T> row{lds + lane};
+  for (int q = sp; q < nq; q += 2) row[q] = T(0);
+  // positions: this wave's sector (a) and the other one (b), in FP64
+  double pa[Cap<NS>::v][3], pb[Cap<NS>::v][3];
+#pragma unroll
+  for (int k = 0; k < NS; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int ea = Sd->sector_elec[sp][k], eb = Sd->sector_elec[1 - sp][k];
+      auto ld = [&](int e) {
+        return A.pos_aos ? A.pos_aos[(wl * N + e) * 3 + c]
+                         : (A.pos64 ? A.pos64[(e * 3 + c) * A.ld + wl] : (double)A.pos32[(e * 3 + c) * A.ld + wl]);
+      };
+      pa[k][c] = ld(ea);
+      pb[k][c] = ld(eb);
+    }
+  double a64[Cap<NS>::v][Cap<NS>::v], la, sg;
+  sector_inverse<NS, double, Bs>(*Sd, sp, pa, a64, la, sg);
+  T ra[Cap<NS>::v][3], rb[Cap<NS>::v][3], ai[Cap<NS>::v][Cap<NS>::v];
+#pragma unroll
+  for (int k = 0; k < Cap<NS>::v; ++k) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ra[k][c] = (T)pa[k][c];
+      rb[k][c] = (T)pb[k][c];
+    }
+#pragma unroll
+    for (int m = 0; m < Cap<NS>::v; ++m) ai[k][m] = (T)a64[k][m];
+  }
+  __syncthreads();  // the row is zeroed before any read-modify-write
+  NucAcc<T, Bs::n_nuc_ct> acc;
+  acc.zero();
+  double kin = 0.0, pot = 0.0;
+#pragma unroll 1
+  for (int k = 0; k < NS; ++k) {
+    kin += electron_pass<NS, NS, T, Bs>(S, sp, sp * NS + k, ra, ai, rb, L, row, acc, pot);
+    rotate<NS>(ra, ai);
+  }
+  // nucleus sums in wave order; the barriers also publish every G
+  for (int v = 0; v < 2; ++v) {
+    if (sp == v) acc.flush(row, L);
+    __syncthreads();
+  }
+  xch[sp * 64 + lane] = kin + pot;
+  if (S.jas_kind == JAS_PADE) {
+    // flush of slot sums [first, nslot) (slot 1 skipped unless has_a)
+    auto flush = [&](const T (&o)[NSL], const T (&e)[NSL], int first, int nslot, bool has_a, auto col) {
+      T ro[NSL], re[NSL];
+#pragma unroll
+      for (int sl = 0; sl < NSL; ++sl)
+        if (sl >= first && sl < nslot && (sl != 1 || has_a)) {
+          ro[sl] = row[L.o(col(sl))];
+          re[sl] = row[L.de(col(sl))];
+        }
+#pragma unroll
+      for (int sl = 0; sl < NSL; ++sl)
+        if (sl >= first && sl < nslot && (sl != 1 || has_a)) {
+          row[L.o(col(sl))] = ro[sl] + o[sl];
+          row[L.de(col(sl))] = re[sl] + e[sl];
+        }
+    };
+    // electron q of the pair-pass order: q < NS up, NS + k down
+    auto gq = [&](int q, int c) { return row[L.g(q, c)]; };
+    const int qa = sp * NS, qb = (1 - sp) * NS;
+    T eo[2][NSL], ee_[2][NSL];
+#pragma unroll
+    for (int cl = 0; cl < 2; ++cl)
+#pragma unroll
+      for (int sl = 0; sl < NSL; ++sl) eo[cl][sl] = ee_[cl][sl] = T(0);
+    // same-spin pairs of this sector
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+#pragma unroll
+      for (int j = i + 1; j < NS; ++j)
+        ee_pair_terms(S, ra[i][0] - ra[j][0], ra[i][1] - ra[j][1], ra[i][2] - ra[j][2], true,
+                      [&](int c) { return gq(qa + i, c) - gq(qa + j, c); },
+                      [&](int sl, T v) { eo[1][sl] += v; }, [&](int sl, T v) { ee_[1][sl] += v; });
+    // this wave's share of the opposite-spin pairs
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+#pragma unroll
+      for (int d = 0; d < NS; ++d) {
+        const int l = (k + d) % NS;
+        if (anti_pair_taken<NS>(sp, k, d))
+          ee_pair_terms(S, ra[k][0] - rb[l][0], ra[k][1] - rb[l][1], ra[k][2] - rb[l][2], false,
+                        [&](int c) { return gq(qa + k, c) - gq(qb + l, c); },
+                        [&](int sl, T v) { eo[0][sl] += v; }, [&](int sl, T v) { ee_[0][sl] += v; });
+      }
+    eo[1][0] += eo[0][0];  // b_ee (slot 0) is one column for both classes
+    ee_[1][0] += ee_[0][0];
+    const int nse = 2 + S.k_ee;
+    for (int v = 0; v < 2; ++v) {
+      if (sp == v) {
+#pragma unroll
+        for (int cl = 0; cl < 2; ++cl)
+          flush(eo[cl], ee_[cl], cl == 0 ? 1 : 0, nse, true, [&](int sl) { return ee_slot_col(S, cl, sl); });
+      }
+      __syncthreads();
+    }
+    const int nsn = 2 + S.k_en;
+    Bs::nuclei(S, [&](int I) {
+      T no[NSL], ne[NSL];
+#pragma unroll
+      for (int sl = 0; sl < NSL; ++sl) no[sl] = ne[sl] = T(0);
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+        en_pair_terms(S, I, ra[k][0] - S.nuc[I][0], ra[k][1] - S.nuc[I][1], ra[k][2] - S.nuc[I][2],
+                      [&](int c) { return gq(qa + k, c); }, [&](int sl, T v) { no[sl] += v; },
+                      [&](int sl, T v) { ne[sl] += v; });
+      const int spc = S.species[I];
+      for (int v = 0; v < 2; ++v) {
+        if (sp == v) flush(no, ne, 0, nsn, false, [&](int sl) { return en_slot_col(S, spc, sl); });
+        __syncthreads();
+      }
+    });
+  } else {
+    __syncthreads();
+  }
+  if (sp == 0) row[L.one()] = T(1);
+  __syncthreads();
+  const double el = xch[lane] + xch[64 + lane] + S.vnn;
+  if (live)
+    for (int q = sp; q < A.Qp; q += 2) A.Y[(int64_t)q * A.ldy + w] = q == L.e() ? el : (double)row[q];
+}
+
+// ---------------------------------------------------------------------------
 // The wave version: one wavefront per walker, for shapes without a register
 // kernel (benzene: 21 + 21 electrons, 66 AOs).  The walker's positions and
 // inverses sit in LDS (the inverses from the FP64 Gauss-Jordan build, cast
@@ -696,6 +865,18 @@ __global__ void __launch_bounds__(256) gram_reduce_kernel(const double* __restri
 
 template <int NU, int ND, class Bs, class T>
 static hipError_t launch_derivs(const void* Sd, const void* St, DerivArgs A, hipStream_t st) {
+  // mixed precision, equal sectors and a compiled shell structure: two
+  // waves per 64 walkers (the FP64 rows would halve the resident workgroups)
+  if constexpr (NU == ND && Bs::n_nuc_ct > 0 && std::is_same<T, float>::value) {
+    const size_t shm2 = (sizeof(T) * 64 * (A.Qp + 3 * (NU + ND)) + 7) / 8 * 8 + sizeof(double) * 128;
+    if (shm2 > 160 * 1024) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute((const void*)derivs2_kernel<NU, Bs, T>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm2);
+    if (e != hipSuccess) return e;
+    derivs2_kernel<NU, Bs, T><<<(unsigned)((A.W + 63) / 64), 128, shm2, st>>>((const DevSys<double>*)Sd,
+                                                                               (const DevSys<T>*)St, A);
+    return hipGetLastError();
+  }
   const size_t shm = sizeof(T) * 64 * (A.Qp + 3 * (NU + ND));
   if (shm > 160 * 1024) return hipErrorInvalidValue;
   static bool attr_set = false;

@@ -229,3 +229,27 @@ def test_compiled_shell_derivatives_match_generic(Q, oracle, name):
     for k in a:
         if b[k].size:
             assert np.allclose(a[k], b[k], rtol=1e-11, atol=1e-11 * (1 + np.abs(b[k]).max())), k
+
+
+@pytest.mark.parametrize("name", ["h2", "ch4_sto3g", "ch4_631g"])
+def test_two_wave_mixed_derivatives_match_one_wave_kernel(Q, oracle, name):
+    """Mixed precision with a compiled shell structure runs the derivative
+    kernel with one wavefront per spin sector (each wave takes its own
+    same-spin pairs and a share of the opposite-spin pairs); the run-time
+    shell kernels (one lane per walker, every pair) give the same columns
+    to FP32 rounding, so no pair is lost or counted twice."""
+    from test_gpu_parity import _configs
+    g, o = _pair(Q, oracle, name, series=name != "h2", precision="mixed")
+    pos = _configs(o, 150)
+    a = g.derivatives_batch(pos)
+    g.set_engine("generic")
+    b = g.derivatives_batch(pos)
+    for k in a:
+        if b[k].size:
+            s = np.abs(b[k]).max()
+            assert np.allclose(a[k], b[k], rtol=2e-5, atol=2e-5 * (1 + s)), (k, np.abs(a[k] - b[k]).max() / (1 + s))
+    ref = g.derivatives_batch(pos)
+    g64, _ = _pair(Q, oracle, name, series=name != "h2")
+    c = g64.derivatives_batch(pos)
+    assert np.allclose(a["eloc"], c["eloc"], rtol=1e-4, atol=1e-3)
+    assert ref["O"].shape == c["O"].shape
