@@ -865,9 +865,10 @@ __global__ void __launch_bounds__(256) gram_reduce_kernel(const double* __restri
 
 template <int NU, int ND, class Bs, class T>
 static hipError_t launch_derivs(const void* Sd, const void* St, DerivArgs A, hipStream_t st) {
-  // mixed precision, equal sectors and a compiled shell structure: two
-  // waves per 64 walkers (the FP64 rows would halve the resident workgroups)
-  if constexpr (NU == ND && Bs::n_nuc_ct > 0 && std::is_same<T, float>::value) {
+  // equal sectors and a compiled shell structure: two waves per 64 walkers
+  // (in FP64 a few registers spill, and the 64 KiB of rows let two
+  // workgroups share a CU; measured 1.6x faster than one wave all the same)
+  if constexpr (NU == ND && Bs::n_nuc_ct > 0) {
     const size_t shm2 = (sizeof(T) * 64 * (A.Qp + 3 * (NU + ND)) + 7) / 8 * 8 + sizeof(double) * 128;
     if (shm2 > 160 * 1024) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute((const void*)derivs2_kernel<NU, Bs, T>,
