@@ -406,13 +406,29 @@ __global__ void __launch_bounds__(64) derivs_kernel(const DevSys<double>* __rest
 // (seen from both waves) is split by k < m: every pair is taken once.
 // ---------------------------------------------------------------------------
 template <int NS>
-__device__ __forceinline__ bool anti_pair_taken(int wave, int k, int d) {
+__host__ __device__ constexpr bool anti_pair_taken(int wave, int k, int d) {
   constexpr int m = NS / 2;
   if (d > m) return false;
   if (wave == 1 && d == 0) return false;
   if (NS % 2 == 0 && d == m) return k < m;
   return true;
 }
+
+// every opposite-spin pair (u_a, d_b) is taken by exactly one wave
+template <int NS>
+constexpr bool anti_pairs_covered_once() {
+  for (int a = 0; a < NS; ++a)
+    for (int b = 0; b < NS; ++b) {
+      const int n = (anti_pair_taken<NS>(0, a, (b - a + NS) % NS) ? 1 : 0) +
+                    (anti_pair_taken<NS>(1, b, (a - b + NS) % NS) ? 1 : 0);
+      if (n != 1) return false;
+    }
+  return true;
+}
+static_assert(anti_pairs_covered_once<1>() && anti_pairs_covered_once<2>() && anti_pairs_covered_once<3>() &&
+                  anti_pairs_covered_once<4>() && anti_pairs_covered_once<5>() && anti_pairs_covered_once<6>() &&
+                  anti_pairs_covered_once<7>() && anti_pairs_covered_once<8>(),
+              "the two waves must share the opposite-spin pairs exactly");
 
 template <int NS, class Bs, class T>
 __global__ void __launch_bounds__(128) derivs2_kernel(const DevSys<double>* __restrict__ Sd,
