@@ -115,6 +115,7 @@ __global__ void __launch_bounds__(DMC_BLOCK) dmc_branch_kernel(DmcArgs a, DmcBuf
                                                                const double* __restrict__ eprev, uint32_t t) {
   const int64_t W = b.ctl->n_live[cur];
   const int64_t w = (int64_t)blockIdx.x * DMC_BLOCK + threadIdx.x;
+  if (t == DMC_STEP_CTL) t = b.ctl->step;
   int m = 0;
   if (w < W) {
     // is_dmc.rs:284-289; the saturating float->usize cast of the reference
@@ -189,6 +190,7 @@ __global__ void __launch_bounds__(DMC_BLOCK) dmc_copy_kernel(const DevSys<double
                                                              double* __restrict__ eprev_dst, int with_force) {
   const DevSys<double>& S = *Sd;
   const int64_t W = b.ctl->n_live[cur];
+  if (with_force == DMC_FORCE_CTL) with_force = (int64_t)b.ctl->step >= a.n_burnin ? 1 : 0;
   const int64_t nch = (W + DMC_BLOCK - 1) / DMC_BLOCK;
   const int rows = a.n3 + a.nainv, Q = 1 + 3 * a.n_nuc;
   const int64_t n_items = nch * (1 + rows + (with_force ? a.n_nuc : 0));
@@ -255,6 +257,9 @@ __global__ void __launch_bounds__(DMC_BLOCK) dmc_copy_kernel(const DevSys<double
 __global__ void __launch_bounds__(1024) dmc_finalize_kernel(DmcArgs a, DmcBufs b, int cur, int64_t ts,
                                                             int with_force, const double* __restrict__ sweep_part,
                                                             int sweep_bs, double* __restrict__ acc_out) {
+  const uint32_t step = b.ctl->step;  // read by every thread before thread 0 advances it
+  if (ts == DMC_TS_CTL) ts = (int64_t)step;
+  if (with_force == DMC_FORCE_CTL) with_force = (int64_t)step >= a.n_burnin ? 1 : 0;
   const int nxt = 1 - cur;
   const int Q = 1 + 3 * a.n_nuc;
   const int nq = with_force ? Q : 1;
@@ -291,7 +296,12 @@ __global__ void __launch_bounds__(1024) dmc_finalize_kernel(DmcArgs a, DmcBufs b
       c->n_force += N;
     }
     c->e_ref = e_mean - a.damping * log((double)N / a.n_target) / a.tau_b;  // is_dmc.rs:487-489
+    c->step = step + 1;
   }
+}
+
+__global__ void dmc_set_step_kernel(DmcCtl* ctl, uint32_t t) {
+  if (threadIdx.x == 0) ctl->step = t;
 }
 
 __global__ void __launch_bounds__(1024) dmc_reset_kernel(DmcCtl* ctl, int b0, int64_t n0,
@@ -351,6 +361,11 @@ hipError_t dmc_branch_step(const void* Sd, const DmcArgs& a, const DmcBufs& b, i
   dmc_copy_kernel<T><<<nb < 1024 ? nb : 1024, DMC_BLOCK, 0, st>>>((const DevSys<double>*)Sd, a, b, cur, src, dst,
                                                                   eprev_dst, with_force);
   dmc_finalize_kernel<<<1, 1024, 0, st>>>(a, b, cur, ts, with_force, sweep_part, sweep_bs, acc_out);
+  return hipGetLastError();
+}
+
+hipError_t dmc_set_step(DmcCtl* ctl, uint32_t t, hipStream_t st) {
+  dmc_set_step_kernel<<<1, 64, 0, st>>>(ctl, t);
   return hipGetLastError();
 }
 

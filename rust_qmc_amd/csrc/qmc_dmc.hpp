@@ -22,13 +22,23 @@ struct DmcCtl {
   int64_t dropped;           // clones dropped at the capacity
   int64_t n_force;           // walkers summed into force_sum
   double force_sum[MAXN * 3];  // production sum of the electronic HF force
+  uint32_t step;             // the step being run; each step's finalize advances it
+  uint32_t pad;
 };
+
+// Step arguments that say "read the step from ctl->step" (a replayed graph
+// of steps has no per-step arguments): the Philox step, the trace slot (=
+// the step) and the force flag (step >= n_burnin).
+constexpr uint32_t DMC_STEP_CTL = 0xFFFFFFFFu;
+constexpr int64_t DMC_TS_CTL = -2;
+constexpr int DMC_FORCE_CTL = -1;
 
 struct DmcArgs {
   int n3, nainv, n_nuc;      // 3N position rows, inverse rows, nuclei
   int64_t cap;               // walkers per population buffer (multiple of DMC_BLOCK)
   double tau_b, damping, n_target;  // branching time (tau, or 2 tau matched to the diffusion)
   uint64_t seed;
+  int64_t n_burnin;          // steps before the forces are summed (for DMC_FORCE_CTL)
 };
 
 struct DmcBufs {
@@ -56,6 +66,8 @@ hipError_t dmc_branch_step(const void* Sd, const DmcArgs& a, const DmcBufs& b, i
 // a fresh population of n0 walkers in buffer b0 (positions, state and E_L
 // already built): live count, pre-sweep E_L = E_L, E_ref = mean E_L
 hipError_t dmc_reset(DmcCtl* ctl, int b0, int64_t n0, const double* eloc, double* eprev, hipStream_t st);
+// ctl->step = t (a kernel, so that it orders with the steps on st)
+hipError_t dmc_set_step(DmcCtl* ctl, uint32_t t, hipStream_t st);
 // per-block partial sums [nblk][3M] of the electronic HF force over the
 // first W walkers of an ensemble
 template <class T>

@@ -705,8 +705,9 @@ __device__ __forceinline__ void dd_sweep_block(const DevSys<T>& S, EnsView<T> E,
   const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
   const uint32_t wid = (uint32_t)(A.walker0 + wl);
   const T tau = (T)A.step_size, md = (T)A.max_drift;
+  const uint32_t step0 = A.step0_dev ? *A.step0_dev : A.step0;
   for (int s = 0; s < A.n_steps; ++s) {
-    const uint32_t step = A.step0 + (uint32_t)s;
+    const uint32_t step = step0 + (uint32_t)s;
     int acc = 0;
     sector_moves_dd<NU, ND, T, Bs, Js>(S, 0, R.ru, R.au, R.rd, wid, step, k0, k1, tau, md, logdet, sgn, ju, acc);
     sector_moves_dd<ND, NU, T, Bs, Js>(S, 1, R.rd, R.ad, R.ru, wid, step, k0, k1, tau, md, logdet, sgn, ju, acc);

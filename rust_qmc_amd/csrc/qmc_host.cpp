@@ -10,6 +10,7 @@
 #include <limits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -2385,7 +2386,20 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
   const bool wave = s->wave();
   const int nb = (int)(cap / DMC_BLOCK), sweep_bs = wave ? 1 : 64, sweep_rows = (int)(cap / sweep_bs);
   const bool f64 = sizeof(T) == 8;
+  // a stream of its own (graphs cannot capture the null stream) and the
+  // two-step graphs, one per parity of the buffer a pair of steps starts in
   hipStream_t st = nullptr;
+  hipGraphExec_t gx[2] = {nullptr, nullptr};
+  HIP_TRY(hipStreamCreate(&st));
+  struct Release {
+    hipStream_t& st;
+    hipGraphExec_t* gx;
+    ~Release() {
+      for (int i = 0; i < 2; ++i)
+        if (gx[i]) (void)hipGraphExecDestroy(gx[i]);
+      (void)hipStreamDestroy(st);
+    }
+  } release{st, gx};
   DevArena A;
   int rc;
   T* pos[2];
@@ -2410,7 +2424,7 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
   for (int b = 0; b < 2; ++b)
     V[b] = EnsView<T>{cap, cap, pos[b], ainv[b], logdet[b], sign[b], ju[b], eloc[b], &B.ctl->n_live[b]};
   const double tau_b = p->matched_branching ? 2.0 * p->time_step : p->time_step;
-  const DmcArgs a{3 * N, nainv, M, cap, tau_b, p->e_ref_damping, (double)p->n_walkers, seed};
+  const DmcArgs a{3 * N, nainv, M, cap, tau_b, p->e_ref_damping, (double)p->n_walkers, seed, p->n_burnin};
   // a fresh population of n_walkers in buffer b from walker ids w0 + w
   auto init_pop = [&](int b, int64_t w0) -> int {
     EnsView<T> E = V[b];
@@ -2446,35 +2460,79 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
   HIP_TRY(hipEventCreate(&ev1));
   HIP_TRY(hipEventRecord(ev0, st));
   const int64_t chunk = 64;
+  // One step: the drift-diffusion sweep of buffer c, then branching into
+  // 1 - c.  In a graph (from_ctl) the step, trace slot and force flag come
+  // from ctl->step, which each step's finalize advances.
+  auto enqueue_step = [&](int c, int64_t u, bool from_ctl) -> int {
+    SweepArgs sa{1, (uint32_t)u, seed, 0, p->time_step, 0, p->max_drift};
+    if (from_ctl) sa.step0_dev = &B.ctl->step;
+    if constexpr (sizeof(T) == 8) {
+      if (wave)
+        HIP_TRY(wave_dd<double>(s->d64, V[c], s->shape(), sa, partial, nullptr, st));
+      else
+        HIP_TRY(s->ops->dd64(s->d64, V[c], sa, partial, nullptr, st));
+    } else {
+      if (wave)
+        HIP_TRY(wave_dd<float>(s->d32, V[c], s->shape(), sa, partial, nullptr, st));
+      else
+        HIP_TRY(s->ops->dd32(s->d32, V[c], sa, partial, nullptr, st));
+    }
+    const int n = 1 - c;
+    HIP_TRY(dmc_branch_step<T>(s->d64, a, B, c, V[c], eprev[c], V[n], eprev[n],
+                               from_ctl ? DMC_STEP_CTL : (uint32_t)u, from_ctl ? DMC_TS_CTL : u,
+                               from_ctl ? DMC_FORCE_CTL : (u >= p->n_burnin ? 1 : 0), partial, sweep_rows, sweep_bs,
+                               acc_dev, st));
+    return QMC_OK;
+  };
+  // pairs of steps replay a captured graph (QMC_DMC_GRAPH=0 keeps every
+  // step an individual launch sequence; the results are the same)
+  const char* genv = std::getenv("QMC_DMC_GRAPH");
+  const bool use_graph = !(genv && std::strcmp(genv, "0") == 0);
+  auto capture = [&](int c) -> int {
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    int rc1 = enqueue_step(c, 0, true);
+    if (!rc1) rc1 = enqueue_step(1 - c, 0, true);
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    if (rc1) {
+      if (ec == hipSuccess && g) (void)hipGraphDestroy(g);
+      return rc1;
+    }
+    HIP_TRY(ec);
+    const hipError_t ei = hipGraphInstantiate(&gx[c], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_TRY(ei);
+    return QMC_OK;
+  };
   int64_t t = 0;
   while (t < T_all) {
     const int64_t t_end = std::min(T_all, t + chunk);
-    for (int64_t u = t; u < t_end; ++u) {
-      const SweepArgs sa{1, (uint32_t)u, seed, 0, p->time_step, 0, p->max_drift};
-      if constexpr (sizeof(T) == 8) {
-        if (wave)
-          HIP_TRY(wave_dd<double>(s->d64, V[cur], s->shape(), sa, partial, nullptr, st));
-        else
-          HIP_TRY(s->ops->dd64(s->d64, V[cur], sa, partial, nullptr, st));
-      } else {
-        if (R > 0 && since >= R) {
+    HIP_TRY(dmc_set_step(B.ctl, (uint32_t)t, st));  // the device step follows t
+    for (int64_t u = t; u < t_end;) {
+      const bool refresh_due = R > 0 && since >= R;
+      if (use_graph && u + 1 < t_end && !refresh_due && !(R > 0 && since + 1 >= R)) {
+        if (!gx[cur] && (rc = capture(cur))) return rc;
+        HIP_TRY(hipGraphLaunch(gx[cur], st));
+        cur_after[u] = 1 - cur;
+        cur_after[u + 1] = cur;
+        since += 2;
+        u += 2;
+        continue;
+      }
+      if constexpr (sizeof(T) == 4) {
+        if (refresh_due) {
           if (wave)
             HIP_TRY(wave_build<float>(s->d64, s->d32, V[cur], s->shape(), 0, nullptr, st));
           else
             HIP_TRY(s->ops->build32(s->d64, s->d32, V[cur], 0, nullptr, st));
           since = 0;
         }
-        if (wave)
-          HIP_TRY(wave_dd<float>(s->d32, V[cur], s->shape(), sa, partial, nullptr, st));
-        else
-          HIP_TRY(s->ops->dd32(s->d32, V[cur], sa, partial, nullptr, st));
       }
-      const int nxt = 1 - cur;
-      HIP_TRY(dmc_branch_step<T>(s->d64, a, B, cur, V[cur], eprev[cur], V[nxt], eprev[nxt], (uint32_t)u, u,
-                                 u >= p->n_burnin ? 1 : 0, partial, sweep_rows, sweep_bs, acc_dev, st));
-      cur = nxt;
+      if ((rc = enqueue_step(cur, u, false))) return rc;
+      cur = 1 - cur;
       cur_after[u] = cur;
       ++since;
+      ++u;
     }
     HIP_TRY(hipMemcpyAsync(out->pop_trace.data() + t, B.pop_trace + t, sizeof(double) * (t_end - t),
                            hipMemcpyDeviceToHost, st));

@@ -31,6 +31,9 @@ struct SweepArgs {
   double step_size;  // Gaussian width (Metropolis) or time step tau (drift-diffusion)
   int64_t trace_ld;  // walker stride of the optional E_L trace
   double max_drift;  // drift-diffusion: cap of |tau F| per electron
+  // drift-diffusion: when set, step0 is read from here at launch (a
+  // replayed graph of DMC steps, whose step lives on the device)
+  const uint32_t* step0_dev = nullptr;
 };
 
 // Batched evaluation outputs; any pointer may be null.

@@ -342,8 +342,9 @@ __global__ void __launch_bounds__(64) wave_dd_kernel(const DevSys<T>* __restrict
         gz += f * dz;
       }
   };
+  const uint32_t step0 = A.step0_dev ? *A.step0_dev : A.step0;
   for (int s = 0; s < A.n_steps; ++s) {
-    const uint32_t step = A.step0 + (uint32_t)s;
+    const uint32_t step = step0 + (uint32_t)s;
     int acc = 0;
     for (int spin = 0; spin < 2; ++spin) {
       const int n = spin ? Y.nd : Y.nu;

@@ -192,3 +192,27 @@ def test_isdmc_wave_engine_matches_register_engine(Q, oracle, precision):
     assert _close(ea, eb, tol)
     assert ra.acceptance_rate == pytest.approx(rb.acceptance_rate, rel=1e-12)
     assert np.allclose(ra.forces_vmc, rb.forces_vmc, rtol=tol, atol=tol * np.abs(rb.forces_vmc).max())
+
+
+@pytest.mark.parametrize("name,precision", [("ch4_sto3g", "fp64"), ("ch4_sto3g", "mixed"), ("h2", "fp64")])
+def test_isdmc_graph_replay_matches_individual_launches(Q, oracle, name, precision, monkeypatch):
+    """Pairs of IS-DMC steps replay a captured graph whose kernels read the
+    step, trace slot and force flag from the device; the same run with
+    every step launched on its own (QMC_DMC_GRAPH=0) gives identical
+    populations, energies and forces.  171 steps in chunks of 64 leave odd
+    tails, the burn-in ends inside a pair, and the mixed run refreshes its
+    inverses every 8 steps between replays."""
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("QMC_DMC_GRAPH", flag)
+        g, _ = _pair(Q, oracle, name, precision=precision)
+        P = Q.ISDMCParams(n_walkers=96, n_steps=150, n_burnin=21, time_step=0.01, max_drift=1.0, e_ref_damping=0.5,
+                          n_vmc_equilibrate=5, n_vmc_force_samples=100, capacity=512, refresh_interval=8)
+        d = Q.ImportanceSampledDMC(g, P, seed=6)
+        r = d.run()
+        out.append((r, d.populations.copy(), d.energies.copy()))
+    (ra, pa, ea), (rb, pb, eb) = out
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(ea, eb)
+    np.testing.assert_array_equal(ra.forces_mixed, rb.forces_mixed)
+    assert ra.energy == rb.energy and ra.acceptance_rate == rb.acceptance_rate
