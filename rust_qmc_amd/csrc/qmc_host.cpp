@@ -134,6 +134,8 @@ struct qmc_system {
     return wave() && tile.ok && prec == QMC_MIXED && engine != QMC_ENGINE_WAVE;
   }
   WaveShape shape() const { return WaveShape{h.n_elec(), h.n_up(), h.n_dn(), h.n_ao()}; }
+  // the Jastrow has no Pade series terms (the kernels compiled without them apply)
+  bool no_series() const { return h.k_ee == 0 && h.k_en == 0; }
   // dC/dR of MO coefficients that follow the geometry (device tables for
   // the wave derivative kernel; empty for fixed coefficients)
   MoDerivPair* mdp = nullptr;
@@ -1503,9 +1505,11 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
       else if (wave)
         HIP_TRY(wave_dd<float>(s->d32, e->view<float>(), s->shape(), A, e->partial, trace, e->st));
       else if (e->prec == QMC_FP64)
-        HIP_TRY(s->ops->dd64(s->d64, e->view<double>(), A, e->partial, trace, e->st));
+        HIP_TRY((s->no_series() ? s->ops->dd64_ns : s->ops->dd64)(s->d64, e->view<double>(), A, e->partial, trace,
+                                                                    e->st));
       else
-        HIP_TRY(s->ops->dd32(s->d32, e->view<float>(), A, e->partial, trace, e->st));
+        HIP_TRY((s->no_series() ? s->ops->dd32_ns : s->ops->dd32)(s->d32, e->view<float>(), A, e->partial, trace,
+                                                                    e->st));
     } else {
       if (wave && e->prec == QMC_FP64)
         HIP_TRY(wave_ae<double>(s->d64, s->d64, e->view<double>(), s->shape(), A, e->partial, trace, e->st));
@@ -2470,12 +2474,12 @@ int isdmc_steps(const qmc_system* s, const qmc_isdmc_params* p, uint64_t seed, i
       if (wave)
         HIP_TRY(wave_dd<double>(s->d64, V[c], s->shape(), sa, partial, nullptr, st));
       else
-        HIP_TRY(s->ops->dd64(s->d64, V[c], sa, partial, nullptr, st));
+        HIP_TRY((s->no_series() ? s->ops->dd64_ns : s->ops->dd64)(s->d64, V[c], sa, partial, nullptr, st));
     } else {
       if (wave)
         HIP_TRY(wave_dd<float>(s->d32, V[c], s->shape(), sa, partial, nullptr, st));
       else
-        HIP_TRY(s->ops->dd32(s->d32, V[c], sa, partial, nullptr, st));
+        HIP_TRY((s->no_series() ? s->ops->dd32_ns : s->ops->dd32)(s->d32, V[c], sa, partial, nullptr, st));
     }
     const int n = 1 - c;
     HIP_TRY(dmc_branch_step<T>(s->d64, a, B, c, V[c], eprev[c], V[n], eprev[n],

@@ -254,9 +254,10 @@ struct Launch {
     sweep_kernel<NU, ND, Bs, Js, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<T>*)St, E, A, partial, trace);
     return hipGetLastError();
   }
+  template <class Js>
   static hipError_t dd(const void* St, EnsView<T> E, SweepArgs A, double* partial, double* trace,
                        hipStream_t st) {
-    dd_sweep_kernel<NU, ND, Bs, JsRt, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<T>*)St, E, A, partial, trace);
+    dd_sweep_kernel<NU, ND, Bs, Js, T><<<nblocks(E.W), 64, 0, st>>>((const DevSys<T>*)St, E, A, partial, trace);
     return hipGetLastError();
   }
   static hipError_t all_electron(const void* Sd, const void* St, EnsView<T> E, SweepArgs A, double* partial,
@@ -292,8 +293,10 @@ static EngineOps make_ops(const char* name, bool (*match)(const DevSys<double>&)
   // specialised kernel agrees, and no cause has been found in the source.
   o.ae64 = &Launch<NU, ND, RtBasis, double>::all_electron;
   o.ae32 = &Launch<NU, ND, RtBasis, float>::all_electron;
-  o.dd64 = &Launch<NU, ND, Bs, double>::dd;
-  o.dd32 = &Launch<NU, ND, Bs, float>::dd;
+  o.dd64 = &Launch<NU, ND, Bs, double>::template dd<JsRt>;
+  o.dd32 = &Launch<NU, ND, Bs, float>::template dd<JsRt>;
+  o.dd64_ns = &Launch<NU, ND, Bs, double>::template dd<JsFast>;
+  o.dd32_ns = &Launch<NU, ND, Bs, float>::template dd<JsFast>;
   o.eval64 = &Launch<NU, ND, Bs, double>::eval;
   o.eval32 = &Launch<NU, ND, Bs, float>::eval;
   return o;

@@ -61,6 +61,9 @@ struct EngineOps {
   hipError_t (*sweep32_ns)(const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*dd64)(const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*dd32)(const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
+  // the drift-diffusion sweeps compiled for a Jastrow without Pade series terms
+  hipError_t (*dd64_ns)(const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
+  hipError_t (*dd32_ns)(const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*ae64)(const void*, const void*, EnsView<double>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*ae32)(const void*, const void*, EnsView<float>, SweepArgs, double*, double*, hipStream_t);
   hipError_t (*eval64)(const void*, const void*, int64_t, const double*, EvalOut, int*, hipStream_t);
