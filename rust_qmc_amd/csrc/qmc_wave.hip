@@ -234,10 +234,7 @@ __global__ void __launch_bounds__(64) wave_sweep_kernel(const DevSys<T>* __restr
             cv[j] = ai[j * n + k];
           }
           __syncthreads();
-          for (int idx = lane; idx < n * n; idx += 64) {
-            const int m = idx / n, j = idx % n;
-            ai[idx] = (j == k) ? cv[m] * iR : ai[idx] - cv[m] * wv[j];
-          }
+          for_each_elem(n, [&](int idx, int m, int j) { ai[idx] = (j == k) ? cv[m] * iR : ai[idx] - cv[m] * wv[j]; });
           if (lane == 0) {
             pos[3 * e] = nx;
             pos[3 * e + 1] = ny;
@@ -433,10 +430,7 @@ __global__ void __launch_bounds__(64) wave_dd_kernel(const DevSys<T>* __restrict
             cv[j] = ai[j * n + k];
           }
           __syncthreads();
-          for (int idx = lane; idx < n * n; idx += 64) {
-            const int m = idx / n, j = idx % n;
-            ai[idx] = (j == k) ? cv[m] * iR : ai[idx] - cv[m] * wv[j];
-          }
+          for_each_elem(n, [&](int idx, int m, int j) { ai[idx] = (j == k) ? cv[m] * iR : ai[idx] - cv[m] * wv[j]; });
           if (lane == 0) {
             pos[3 * e] = nx;
             pos[3 * e + 1] = ny;

@@ -41,6 +41,27 @@ __device__ __forceinline__ T wsum(T v) {
   return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
+// Visits the elements idx = lane, lane + 64, ... of an n x n row-major
+// matrix as f(idx, row, col) with one division per call: each stride of 64
+// elements advances the column by 64 mod n and the row by 64 / n, with the
+// carry (n <= 64).
+template <class F>
+__device__ __forceinline__ void for_each_elem(int n, F f) {
+  if (n <= 0) return;
+  const int lane = threadIdx.x & 63;
+  const int dr = 64 / n, dc = 64 % n;
+  int r = lane / n, c = lane % n;
+  for (int idx = lane; idx < n * n; idx += 64) {
+    f(idx, r, c);
+    r += dr;
+    c += dc;
+    if (c >= n) {
+      c -= n;
+      ++r;
+    }
+  }
+}
+
 // LDS carve-out of one walker (element counts in T unless noted)
 struct WaveLayout {
   int N, nu, nd, nmax, A;
@@ -191,13 +212,12 @@ __device__ bool wave_sector_inverse(const DevSys<double>& S, int spin, const T* 
     }
     for (int i = lane; i < n; i += 64) fcol[i] = a[i * n + c];
     __syncthreads();
-    for (int idx = lane; idx < n * n; idx += 64) {
-      const int i = idx / n, j = idx % n;
-      if (i == c) continue;
+    for_each_elem(n, [&](int idx, int i, int j) {
+      if (i == c) return;
       const double f = fcol[i];
-      a[i * n + j] -= f * a[c * n + j];
-      inv[i * n + j] -= f * inv[c * n + j];
-    }
+      a[idx] -= f * a[c * n + j];
+      inv[idx] -= f * inv[c * n + j];
+    });
     __syncthreads();
   }
   if (singular) {
