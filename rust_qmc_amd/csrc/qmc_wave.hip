@@ -180,6 +180,19 @@ __global__ void __launch_bounds__(64) wave_sweep_kernel(const DevSys<T>* __restr
   const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
   const uint32_t wid = (uint32_t)(A.walker0 + w);
   const T sigma = (T)A.step_size;
+  // MO values in chunks: lane m + n c (c < 64 / n) sums MO m over the AOs
+  // [lo, hi) of chunk c, so no lane runs the whole AO sum alone
+  T* mp = L + Y.mpart();
+  int mc_C[2], mc_c[2], mc_m[2], mc_lo[2], mc_hi[2];
+  for (int sp = 0; sp < 2; ++sp) {
+    const int n = sp ? Y.nd : Y.nu;
+    const int C = n > 0 ? 64 / n : 0, c = n > 0 ? lane / n : 64;
+    mc_C[sp] = C;
+    mc_c[sp] = c;
+    mc_m[sp] = n > 0 ? lane - c * n : 0;
+    mc_lo[sp] = c < C ? c * Y.A / C : 0;
+    mc_hi[sp] = c < C ? (c + 1) * Y.A / C : 0;
+  }
   for (int s = 0; s < A.n_steps; ++s) {
     const uint32_t step = A.step0 + (uint32_t)s;
     int acc = 0;
@@ -196,11 +209,18 @@ __global__ void __launch_bounds__(64) wave_sweep_kernel(const DevSys<T>* __restr
         // AO values of the proposal: one shell per lane
         for (int sh = lane; sh < S.n_shell; sh += 64) shell_values(S, sh, nx, ny, nz, chi);
         __syncthreads();
-        // MO values and the ratio R = sum_m phi_m ainv[m][k]
+        {
+          const int mi = mc_m[spin];
+          T v = 0;
+          for (int ao = mc_lo[spin]; ao < mc_hi[spin]; ++ao) v += S.moT[spin][ao][mi] * chi[ao];
+          if (mc_c[spin] < mc_C[spin]) mp[mc_c[spin] * n + mi] = v;
+        }
+        __syncthreads();
+        // MO values (chunk sums in chunk order) and R = sum_m phi_m ainv[m][k]
         T part = 0;
         for (int m = lane; m < n; m += 64) {
           T v = 0;
-          for (int ao = 0; ao < Y.A; ++ao) v += S.moT[spin][ao][m] * chi[ao];
+          for (int c = 0; c < mc_C[spin]; ++c) v += mp[c * n + m];
           phi[m] = v;
           part += v * ai[m * n + k];
         }

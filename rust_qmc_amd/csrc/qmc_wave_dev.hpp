@@ -72,7 +72,8 @@ struct WaveLayout {
   __device__ __host__ int cvec() const { return phi() + nmax; }
   __device__ __host__ int wvec() const { return cvec() + nmax; }
   __device__ __host__ int wao() const { return wvec() + nmax; }  // AO weights of a drift-diffusion move
-  __device__ __host__ int total() const { return wao() + A; }
+  __device__ __host__ int mpart() const { return wao() + A; }    // 64 partial MO sums of a move
+  __device__ __host__ int total() const { return mpart() + 64; }
 };
 
 // Values and gradients of the AOs of shell sh at (x, y, z), shell-local
