@@ -370,20 +370,21 @@ int fill_devsys(const HostSys& h, DevSys<T>& D) {
 // (benzene.rs:350-381) has C 2p coefficients 0.20 u and 0.55 u with
 // u = (R_H - R_C) / |R_H - R_C|, so d u_c / d R_H,c' = (delta_cc' - u_c u_c')
 // / |R_H - R_C| and d u / d R_C is its negative.
-void mo_deriv_tables(const HostSys& h, const DevSys<double>& D, std::vector<MoDerivPair>& pairs,
-                     std::vector<MoDerivEntry>& ents, int p0[3]) {
+int mo_deriv_tables(const HostSys& h, const DevSys<double>& D, std::vector<MoDerivPair>& pairs,
+                    std::vector<MoDerivEntry>& ents, int p0[3]) {
   pairs.clear();
   ents.clear();
   p0[0] = p0[1] = p0[2] = 0;
-  if (h.mo_rule != QMC_MO_BENZENE_CH) return;
+  if (h.mo_rule != QMC_MO_BENZENE_CH) return QMC_OK;
   auto locate = [&](int ao, int& sh, int& q) {
     for (sh = 0; sh < D.n_shell; ++sh) {
       const int k = D.sh_kind[sh], nao = k == SH_SP ? 4 : (k == SH_P ? 3 : 1);
       if (ao >= D.sh_ao0[sh] && ao < D.sh_ao0[sh] + nao) {
         q = ao - D.sh_ao0[sh];
-        return;
+        return true;
       }
     }
+    return false;
   };
   for (int spin = 0; spin < 2; ++spin) {
     p0[spin] = (int)pairs.size();
@@ -394,7 +395,8 @@ void mo_deriv_tables(const HostSys& h, const DevSys<double>& D, std::vector<MoDe
         const double coef = part ? 0.55 : 0.20;
         for (int c = 0; c < 3; ++c) {
           MoDerivPair P{spin, 9 + n, 0, 0, (int)ents.size(), 0};
-          locate(9 * n + (part ? 6 : 2) + c, P.sh, P.q);
+          if (!locate(9 * n + (part ? 6 : 2) + c, P.sh, P.q))
+            return fail(QMC_ERR_INVALID, "the benzene C-H MO rule found no shell for a 2p AO");
           for (int c2 = 0; c2 < 3; ++c2) {
             const double d = coef * ((c == c2 ? 1.0 : 0.0) - u[c] * u[c2]) / len;
             ents.push_back({6 + n, c2, d});
@@ -407,6 +409,7 @@ void mo_deriv_tables(const HostSys& h, const DevSys<double>& D, std::vector<MoDe
     }
   }
   p0[2] = (int)pairs.size();
+  return QMC_OK;
 }
 
 int upload(qmc_system* s) {
@@ -424,7 +427,7 @@ int upload(qmc_system* s) {
   {
     std::vector<MoDerivPair> pairs;
     std::vector<MoDerivEntry> ents;
-    mo_deriv_tables(s->h, *d64, pairs, ents, s->mdp0);
+    if ((rc = mo_deriv_tables(s->h, *d64, pairs, ents, s->mdp0))) return rc;
     if (s->mdp) HIP_TRY(hipFree(s->mdp));
     if (s->mde) HIP_TRY(hipFree(s->mde));
     s->mdp = nullptr;
