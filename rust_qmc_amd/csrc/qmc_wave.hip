@@ -369,16 +369,21 @@ __global__ void __launch_bounds__(64) wave_dd_kernel(const DevSys<T>* __restrict
       for (int k = 0; k < n; ++k) {
         const int e = S.sector_elec[spin][k];
         const T ox = pos[3 * e], oy = pos[3 * e + 1], oz = pos[3 * e + 2];
-        // quantum force at r: AO weights and grad ln|D|, Jastrow gradient
+        // AO weights of the moved electron, one AO per lane
+        for (int a = lane; a < Y.A; a += 64) {
+          T wq = 0;
+          for (int m = 0; m < n; ++m) wq += S.moT[spin][a][m] * ai[m * n + k];
+          wao[a] = wq;
+        }
+        __syncthreads();
+        // quantum force at r: grad ln|D| from the shells, Jastrow gradient
         T gx = 0, gy = 0, gz = 0;
         for (int sh = lane; sh < S.n_shell; sh += 64) {
           T v[4], g[4][3];
           const int nq = shell_values_grad(S, sh, ox, oy, oz, v, g);
           const int a0 = S.sh_ao0[sh];
           for (int q = 0; q < nq; ++q) {
-            T wq = 0;
-            for (int m = 0; m < n; ++m) wq += S.moT[spin][a0 + q][m] * ai[m * n + k];
-            wao[a0 + q] = wq;
+            const T wq = wao[a0 + q];
             gx += wq * g[q][0];
             gy += wq * g[q][1];
             gz += wq * g[q][2];
