@@ -112,8 +112,8 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
   double pe = 0.0;
   auto ee = [&](T px, T py, T pz, bool same) {
     T dx = x - px, dy = y - py, dz = z - pz;
-    T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-    T ir = M<T>::rcp(r);
+    T r, ir;
+    M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
     T u1, lp;
     uee_d(S, r, ir, same, u1, lp);
     if (r >= T(1e-10)) {
@@ -132,8 +132,8 @@ __device__ __forceinline__ double electron_pass(const DevSys<T>& S, int spin, in
   for (int j = 0; j < NB; ++j) ee(rb[j][0], rb[j][1], rb[j][2], false);
   Bs::nuclei(S, [&](int I) {
     T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-    T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-    T ir = M<T>::rcp(r);
+    T r, ir;
+    M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
     pe -= (double)(S.Z[I] * ir);
     if (r > T(1e-10)) {  // methane.rs:1253
       const T f3 = S.Z[I] * ir * ir * ir;
@@ -177,7 +177,7 @@ template <class T, class GD, class AO, class AE>
 __device__ __forceinline__ void ee_pair_terms(const DevSys<T>& S, T dx, T dy, T dz, bool same, GD gd, AO addO,
                                               AE addE) {
   const int K = S.k_ee;
-  T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+  T r = M<T>::dist(dx * dx + dy * dy + dz * dz);
   const T a = same ? S.a_para : S.a_anti;
   const T* c = same ? S.c_para : S.c_anti;
   PairDerivs<T> pd;
@@ -210,7 +210,7 @@ __device__ __forceinline__ void ee_pair_terms(const DevSys<T>& S, T dx, T dy, T 
 template <class T, class GI, class AO, class AE>
 __device__ __forceinline__ void en_pair_terms(const DevSys<T>& S, int I, T dx, T dy, T dz, GI g, AO addO, AE addE) {
   const int Ke = S.k_en;
-  T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
+  T r = M<T>::dist(dx * dx + dy * dy + dz * dz);
   const T* c = S.d_en[S.species[I]];
   PairDerivs<T> pd;
   pd.init(r, S.b_en, -S.Z[I], Ke, c);
@@ -642,8 +642,8 @@ __global__ void __launch_bounds__(64) wave_derivs_kernel(const DevSys<double>* _
     for (int j = 0; j < N; ++j) {
       if (j == i) continue;
       T dx = x - L[3 * j], dy = y - L[3 * j + 1], dz = z - L[3 * j + 2];
-      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-      T ir = M<T>::rcp(r);
+      T r, ir;
+      M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
       T u1, lp;
       uee_d(S, r, ir, S.elec_spin[j] == spin, u1, lp);
       if (r >= T(1e-10)) {
@@ -658,8 +658,8 @@ __global__ void __launch_bounds__(64) wave_derivs_kernel(const DevSys<double>* _
     }
     for (int I = 0; I < Mn; ++I) {
       T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-      T ir = M<T>::rcp(r);
+      T r, ir;
+      M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
       pe -= (double)(S.Z[I] * ir);
       if (r > T(1e-10)) {  // methane.rs:1253
         const T f3 = S.Z[I] * ir * ir * ir;

@@ -16,13 +16,42 @@ namespace qmc {
 // ---------------------------------------------------------------------------
 template <class T>
 struct M;
+// dist(d2) = sqrt(d2) and dist_inv(d2, r, ir) = (sqrt(d2), 1/sqrt(d2)) are
+// the distance forms of a squared distance d2 >= 0.
 template <>
 struct M<double> {
   static __device__ __forceinline__ double ex(double x) { return exp(x); }
   static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
-  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
   static __device__ __forceinline__ double lg(double x) { return log(x); }
-  static __device__ __forceinline__ double rsq(double x) { return 1.0 / sqrt(x); }
+  // 1/x and 1/sqrt(x): the hardware estimates (v_rcp_f64, v_rsq_f64)
+  // refined by two Newton-Raphson steps each.  Each step squares the
+  // relative error e of the estimate (e -> ~e^2), so two steps reach FP64
+  // rounding from any e < 2^-14: within 1-2 ulp of the correctly rounded
+  // quotient and root at about a third of the instructions of the IEEE
+  // division and square-root sequences.
+  static __device__ __forceinline__ double rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(e, y, y);
+    e = fma(-x, y, 1.0);
+    return fma(e, y, y);
+  }
+  // x >= 0, clamped to 1e-300 so that x = 0 gives a finite 1e150 and
+  // x * rsq(x) = 0
+  static __device__ __forceinline__ double rsq(double x) {
+    x = fmax(x, 1e-300);
+    const double hx = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-hx * y, y, 0.5);  // 1/2 - x y^2 / 2
+    y = fma(y, e, y);
+    e = fma(-hx * y, y, 0.5);
+    return fma(y, e, y);
+  }
+  static __device__ __forceinline__ double dist(double d2) { return d2 * rsq(d2); }
+  static __device__ __forceinline__ void dist_inv(double d2, double& r, double& ir) {
+    ir = rsq(d2);
+    r = d2 * ir;
+  }
 };
 template <>
 struct M<float> {
@@ -31,6 +60,11 @@ struct M<float> {
   static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
   static __device__ __forceinline__ float lg(float x) { return __logf(x); }
   static __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+  static __device__ __forceinline__ float dist(float d2) { return sq(d2); }
+  static __device__ __forceinline__ void dist_inv(float d2, float& r, float& ir) {
+    r = sq(d2);
+    ir = rcp(r);
+  }
 };
 
 // ---------------------------------------------------------------------------

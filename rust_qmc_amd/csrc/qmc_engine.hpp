@@ -57,8 +57,8 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
   T pee = 0, pen = 0;
   auto ee = [&](T px, T py, T pz, bool same) {
     T dx = x - px, dy = y - py, dz = z - pz;
-    T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-    T ir = M<T>::rcp(r);
+    T r, ir;
+    M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
     T u1, lp;
     uee_d<T, Js>(S, r, ir, same, u1, lp);
     const bool ok = r >= T(1e-10);
@@ -77,8 +77,8 @@ __device__ __forceinline__ void electron_terms(const DevSys<T>& S, int spin, con
   const bool en_on = S.en_on;
   Bs::nuclei(S, [&](int I) {
     T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-    T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-    T ir = M<T>::rcp(r);
+    T r, ir;
+    M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
     pen += S.Z[I] * ir;
     if (en_on) {
       T u1, lp;
@@ -345,7 +345,7 @@ __device__ __forceinline__ T en_sum(const DevSys<T>& S, T x, T y, T z) {
   T u = 0;
   Bs::nuclei(S, [&](int I) {
     const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-    u += uen_val<T, Js>(S, M<T>::sq(dx * dx + dy * dy + dz * dz), I);
+    u += uen_val<T, Js>(S, M<T>::dist(dx * dx + dy * dy + dz * dz), I);
   });
   return u;
 }
@@ -379,7 +379,7 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
     auto pair = [&](T px, T py, T pz, bool same) {
       T ax = nx - px, ay = ny - py, az = nz - pz;
       T bx = ox - px, by = oy - py, bz = oz - pz;
-      du += uee_diff<T, Js>(S, M<T>::sq(ax * ax + ay * ay + az * az), M<T>::sq(bx * bx + by * by + bz * bz), same);
+      du += uee_diff<T, Js>(S, M<T>::dist(ax * ax + ay * ay + az * az), M<T>::dist(bx * bx + by * by + bz * bz), same);
     };
 #pragma unroll
     for (int j = 1; j < NA; ++j) pair(ra[j][0], ra[j][1], ra[j][2], true);
@@ -555,8 +555,8 @@ __device__ __forceinline__ void slot0_jastrow(const DevSys<T>& S, T x, T y, T z,
   u = gx = gy = gz = T(0);
   auto ee = [&](T px, T py, T pz, bool same) {
     const T dx = x - px, dy = y - py, dz = z - pz;
-    const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-    const T ir = M<T>::rcp(r);
+    T r, ir;
+    M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
     T u1, lp;
     uee_d<T, Js>(S, r, ir, same, u1, lp);
     u += uee_val<T, Js>(S, r, same);
@@ -572,8 +572,8 @@ __device__ __forceinline__ void slot0_jastrow(const DevSys<T>& S, T x, T y, T z,
   if (S.en_on)
     Bs::nuclei(S, [&](int I) {
       const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-      const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-      const T ir = M<T>::rcp(r);
+      T r, ir;
+      M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
       T u1, lp;
       uen_d<T, Js>(S, r, ir, I, u1, lp);
       u += uen_val<T, Js>(S, r, I);

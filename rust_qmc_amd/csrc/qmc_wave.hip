@@ -59,8 +59,8 @@ __device__ double wave_local_energy(const DevSys<T>& S, const WaveLayout& Y, T* 
     for (int j = 0; j < Y.N; ++j) {
       if (j == i) continue;
       T dx = x - pos[3 * j], dy = y - pos[3 * j + 1], dz = z - pos[3 * j + 2];
-      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-      T ir = M<T>::rcp(r);
+      T r, ir;
+      M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
       T u1, lp;
       uee_d(S, r, ir, S.elec_spin[j] == spin, u1, lp);
       if (r >= T(1e-10)) {
@@ -75,8 +75,8 @@ __device__ double wave_local_energy(const DevSys<T>& S, const WaveLayout& Y, T* 
     }
     for (int I = 0; I < S.n_nuc; ++I) {
       T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-      T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-      T ir = M<T>::rcp(r);
+      T r, ir;
+      M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
       pe -= (double)(S.Z[I] * ir);
       if (S.en_on) {
         T u1, lp;
@@ -232,15 +232,15 @@ __global__ void __launch_bounds__(64) wave_sweep_kernel(const DevSys<T>* __restr
           const bool same = S.elec_spin[j] == spin;
           T ax = nx - pos[3 * j], ay = ny - pos[3 * j + 1], az = nz - pos[3 * j + 2];
           T bx = ox - pos[3 * j], by = oy - pos[3 * j + 1], bz = oz - pos[3 * j + 2];
-          du += uee_val(S, M<T>::sq(ax * ax + ay * ay + az * az), same) -
-                uee_val(S, M<T>::sq(bx * bx + by * by + bz * bz), same);
+          du += uee_val(S, M<T>::dist(ax * ax + ay * ay + az * az), same) -
+                uee_val(S, M<T>::dist(bx * bx + by * by + bz * bz), same);
         }
         if (S.en_on)
           for (int I = lane; I < S.n_nuc; I += 64) {
             T ax = nx - S.nuc[I][0], ay = ny - S.nuc[I][1], az = nz - S.nuc[I][2];
             T bx = ox - S.nuc[I][0], by = oy - S.nuc[I][1], bz = oz - S.nuc[I][2];
-            du += uen_val(S, M<T>::sq(ax * ax + ay * ay + az * az), I) -
-                  uen_val(S, M<T>::sq(bx * bx + by * by + bz * bz), I);
+            du += uen_val(S, M<T>::dist(ax * ax + ay * ay + az * az), I) -
+                  uen_val(S, M<T>::dist(bx * bx + by * by + bz * bz), I);
           }
         du = wsum(du);
         const T p = R * R * M<T>::ex(T(2) * du);
@@ -334,8 +334,8 @@ __global__ void __launch_bounds__(64) wave_dd_kernel(const DevSys<T>* __restrict
     for (int j = lane; j < Y.N; j += 64) {
       if (j == e) continue;
       const T dx = x - pos[3 * j], dy = y - pos[3 * j + 1], dz = z - pos[3 * j + 2];
-      const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-      const T ir = M<T>::rcp(r);
+      T r, ir;
+      M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
       const bool same = S.elec_spin[j] == spin;
       T u1, lp;
       uee_d(S, r, ir, same, u1, lp);
@@ -348,8 +348,8 @@ __global__ void __launch_bounds__(64) wave_dd_kernel(const DevSys<T>* __restrict
     if (S.en_on)
       for (int I = lane; I < S.n_nuc; I += 64) {
         const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-        const T r = M<T>::sq(dx * dx + dy * dy + dz * dz);
-        const T ir = M<T>::rcp(r);
+        T r, ir;
+        M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
         T u1, lp;
         uen_d(S, r, ir, I, u1, lp);
         u += uen_val(S, r, I);
