@@ -177,7 +177,8 @@ template <class T, class GD, class AO, class AE>
 __device__ __forceinline__ void ee_pair_terms(const DevSys<T>& S, T dx, T dy, T dz, bool same, GD gd, AO addO,
                                               AE addE) {
   const int K = S.k_ee;
-  T r = M<T>::dist(dx * dx + dy * dy + dz * dz);
+  T r, ir;
+  M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
   const T a = same ? S.a_para : S.a_anti;
   const T* c = same ? S.c_para : S.c_anti;
   PairDerivs<T> pd;
@@ -186,20 +187,19 @@ __device__ __forceinline__ void ee_pair_terms(const DevSys<T>& S, T dx, T dy, T 
   addO(0, -pd.A1 * pd.x * pd.x);
   addO(1, pd.x);
   if (r < T(1e-10)) return;
-  const T ir = T(1) / r;
   // Delta = (G_i - G_j) . rhat_ij
   const T dlt = gd(0) * (dx * ir) + gd(1) * (dy * ir) + gd(2) * (dz * ir);
   T du1, dl;
-  pd.d_b(r, S.b_ee, du1, dl);
+  pd.d_b(r, ir, S.b_ee, du1, dl);
   addE(0, -(dl + dlt * du1));
-  pd.d_a(r, S.b_ee, du1, dl);
+  pd.d_a(ir, S.b_ee, du1, dl);
   addE(1, -(dl + dlt * du1));
   T xk = T(1);
 #pragma unroll
   for (int k = 0; k < MAXK; ++k) {  // unrolled, so the slot is a constant
     if (k >= K) break;
     addO(2 + k, xk * pd.x * pd.x);
-    pd.d_c(k, r, S.b_ee, xk, du1, dl);
+    pd.d_c(k, ir, S.b_ee, xk, du1, dl);
     addE(2 + k, -(dl + dlt * du1));
     xk *= pd.x;
   }
@@ -210,7 +210,8 @@ __device__ __forceinline__ void ee_pair_terms(const DevSys<T>& S, T dx, T dy, T 
 template <class T, class GI, class AO, class AE>
 __device__ __forceinline__ void en_pair_terms(const DevSys<T>& S, int I, T dx, T dy, T dz, GI g, AO addO, AE addE) {
   const int Ke = S.k_en;
-  T r = M<T>::dist(dx * dx + dy * dy + dz * dz);
+  T r, ir;
+  M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
   const T* c = S.d_en[S.species[I]];
   PairDerivs<T> pd;
   pd.init(r, S.b_en, -S.Z[I], Ke, c);
@@ -226,17 +227,16 @@ __device__ __forceinline__ void en_pair_terms(const DevSys<T>& S, int I, T dx, T
     }
     return;
   }
-  const T ir = T(1) / r;
   const T dlt = g(0) * (dx * ir) + g(1) * (dy * ir) + g(2) * (dz * ir);
   T du1, dl;
-  pd.d_b(r, S.b_en, du1, dl);
+  pd.d_b(r, ir, S.b_en, du1, dl);
   addE(0, -(T(0.5) * dl + dlt * du1));
   T xk = T(1);
 #pragma unroll
   for (int k = 0; k < MAXK; ++k) {
     if (k >= Ke) break;
     addO(2 + k, xk * pd.x * pd.x);
-    pd.d_c(k, r, S.b_en, xk, du1, dl);
+    pd.d_c(k, ir, S.b_en, xk, du1, dl);
     addE(2 + k, -(T(0.5) * dl + dlt * du1));
     xk *= pd.x;
   }

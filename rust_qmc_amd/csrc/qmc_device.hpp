@@ -415,7 +415,7 @@ struct PairDerivs {
   T x, iD, A1, s2, s3;
   __device__ __forceinline__ void init(T r, T b, T A, int K, const T* c) {
     T D = T(1) + b * r;
-    iD = T(1) / D;
+    iD = M<T>::rcp(D);
     x = r * iD;
     T s, s1;
     s1 = 0;
@@ -427,27 +427,27 @@ struct PairDerivs {
     }
     A1 = A + s1;
   }
-  // d/db of u' and of lap = u'' + 2u'/r
-  __device__ __forceinline__ void d_b(T r, T b, T& du1, T& dlap) const {
+  // d/db of u' and of lap = u'' + 2u'/r (ir = 1/r)
+  __device__ __forceinline__ void d_b(T r, T ir, T b, T& du1, T& dlap) const {
     const T iD2 = iD * iD, iD3 = iD2 * iD, iD4 = iD2 * iD2, x2 = x * x;
     du1 = -x2 * s2 * iD2 - T(2) * r * A1 * iD3;
     const T t1 = -x2 * s3 * iD4 - T(4) * r * s2 * iD4 * iD;
     const T t2 = -T(2) * A1 * iD3 + T(2) * b * x2 * s2 * iD3 + T(6) * b * r * A1 * iD4;
-    dlap = t1 + t2 + T(2) / r * du1;
+    dlap = t1 + t2 + T(2) * ir * du1;
   }
   // d/dA (A enters linearly)
-  __device__ __forceinline__ void d_a(T r, T b, T& du1, T& dlap) const {
+  __device__ __forceinline__ void d_a(T ir, T b, T& du1, T& dlap) const {
     du1 = iD * iD;
-    dlap = -T(2) * b * iD * iD * iD + T(2) * du1 / r;
+    dlap = -T(2) * b * iD * iD * iD + T(2) * du1 * ir;
   }
   // d/dc_k for the term c_k x^(k+2)
-  __device__ __forceinline__ void d_c(int k, T r, T b, T xk, T& du1, T& dlap) const {
+  __device__ __forceinline__ void d_c(int k, T ir, T b, T xk, T& du1, T& dlap) const {
     // xk = x^k
     const T n = T(k + 2);
     const T p1 = n * xk * x, p2 = n * (n - T(1)) * xk;
     const T iD2 = iD * iD;
     du1 = p1 * iD2;
-    dlap = p2 * iD2 * iD2 - T(2) * b * p1 * iD2 * iD + T(2) * du1 / r;
+    dlap = p2 * iD2 * iD2 - T(2) * b * p1 * iD2 * iD + T(2) * du1 * ir;
   }
 };
 
