@@ -245,6 +245,22 @@ int qmc_dEL_dp(const qmc_system* sys, int64_t W, const double* pos, double* dE, 
 int qmc_hf_force(const qmc_system* sys, int64_t W, const double* pos, double* F, void* stream);
 int qmc_nuc_grad_log(const qmc_system* sys, int64_t W, const double* pos, double* Om, void* stream);
 
+/* qmc_derivatives without any allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * pos and every non-NULL output must be device memory (host pointers are
+ * rejected with QMC_ERR_INVALID; nothing is staged), and the call is
+ * asynchronous on `stream`: the outputs are complete when the stream is.
+ * The per-walker feature matrix lives in a fixed 64 MiB device array that
+ * every GPU running the library holds until the process ends, so W is
+ * limited to 8 Mi / Qp walkers, Qp = 2P + 6M + 2 rounded up to 16 (87,381
+ * walkers at Qp = 96: CH4 with 32 Jastrow parameters, benzene with 4);
+ * larger W gives QMC_ERR_INVALID.  Calls that overlap on one GPU (two
+ * streams, two threads) share that array and race; serialise them.
+ * Singular Slater matrices take the identity inverse, as in qmc_derivatives. */
+int qmc_derivatives_noalloc(const qmc_system* sys, int64_t W, const double* pos, double* eloc, double* O,
+                            double* dE, double* F, double* Om, void* stream);
+
 /* (S + eps I) dp = -f by LU in FP64, falling back to dp = -f when the
  * matrix is singular (SROptimizer::compute_sr_update, sr_optimize.rs:219-242).
  * Returns QMC_OK, or QMC_ERR_SINGULAR when the fallback was taken. */

@@ -44,7 +44,7 @@ EXPORTED = [
     "qmc_ensemble_create", "qmc_ensemble_destroy", "qmc_ensemble_get_positions", "qmc_ensemble_set_positions",
     "qmc_ensemble_get_state", "qmc_ensemble_refresh", "qmc_ensemble_sweeps", "qmc_ensemble_set_timing",
     "qmc_ensemble_timing", "qmc_vmc_run", "qmc_derivatives", "qmc_log_derivs", "qmc_dEL_dp", "qmc_hf_force",
-    "qmc_nuc_grad_log", "qmc_sr_solve", "qmc_ensemble_measure", "qmc_ensemble_moments", "qmc_sr_optimize",
+    "qmc_nuc_grad_log", "qmc_derivatives_noalloc", "qmc_sr_solve", "qmc_ensemble_measure", "qmc_ensemble_moments", "qmc_sr_optimize",
     "qmc_sample_forces", "qmc_geometry_optimize",
     "qmc_ddvmc_run", "qmc_isdmc_run", "qmc_isdmc_block_error", "qmc_system_from_json", "qmc_system_to_json",
     "qmc_batch_create", "qmc_batch_destroy", "qmc_batch_fusable", "qmc_batch_set_config", "qmc_batch_get_config",
@@ -187,6 +187,7 @@ _SIGS = {
     "qmc_dEL_dp": (_i32, [_p, _i64, _p, _p, _p]),
     "qmc_hf_force": (_i32, [_p, _i64, _p, _p, _p]),
     "qmc_nuc_grad_log": (_i32, [_p, _i64, _p, _p, _p]),
+    "qmc_derivatives_noalloc": (_i32, [_p, _i64, _p, _p, _p, _p, _p, _p, _p]),
     "qmc_sr_solve": (_i32, [_i32, _p, _p, _d, _p]),
     "qmc_ensemble_measure": (_i32, [_p, _i32]),
     "qmc_ensemble_moments": (_i32, [_p, _p]),

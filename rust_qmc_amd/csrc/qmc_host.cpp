@@ -20,6 +20,7 @@
 #include "qmc_device.hpp"
 #include "qmc_dmc.hpp"
 #include "qmc_kernels.hpp"
+#include "qmc_noalloc.hpp"
 #include "qmc_batch.hpp"
 #include "qmc_tile.hpp"
 #include "qmc_wave.hpp"
@@ -464,9 +465,10 @@ struct Cols {
 // nucleus-nucleus part of the Hellmann-Feynman force: the physical
 // repulsion is +Z_I Z_J (R_I - R_J)/|R_I - R_J|^3; the reference subtracts
 // it (methane.rs:1259-1269), reproduced under QMC_FLAG_HF_NN_REFSIGN.
-std::vector<double> hf_nn(const HostSys& h) {
+// F must hold 3 n_nuc doubles; written in place (no allocation)
+void hf_nn_fill(const HostSys& h, double* F) {
   const int M = h.n_nuc();
-  std::vector<double> F(3 * M, 0.0);
+  for (int q = 0; q < 3 * M; ++q) F[q] = 0.0;
   const double sg = (h.flags & QMC_FLAG_HF_NN_REFSIGN) ? -1.0 : 1.0;
   for (int I = 0; I < M; ++I)
     for (int J = 0; J < M; ++J) {
@@ -480,6 +482,10 @@ std::vector<double> hf_nn(const HostSys& h) {
       if (r > 1e-10)
         for (int c = 0; c < 3; ++c) F[3 * I + c] += sg * h.Z[I] * h.Z[J] * d[c] / (r2 * r);
     }
+}
+std::vector<double> hf_nn(const HostSys& h) {
+  std::vector<double> F(3 * h.n_nuc(), 0.0);
+  hf_nn_fill(h, F.data());
   return F;
 }
 
@@ -1230,6 +1236,33 @@ int qmc_hf_force(const qmc_system* s, int64_t W, const double* pos, double* F, v
 }
 int qmc_nuc_grad_log(const qmc_system* s, int64_t W, const double* pos, double* Om, void* st) {
   return qmc_derivatives(s, W, pos, nullptr, nullptr, nullptr, nullptr, Om, st);
+}
+
+int qmc_derivatives_noalloc(const qmc_system* sys, int64_t W, const double* pos, double* eloc, double* O, double* dE,
+                            double* F, double* Om, void* stream) {
+  if (!sys || !pos || W < 0) return fail(QMC_ERR_INVALID, "bad arguments");
+  if (W == 0) return QMC_OK;
+  for (const void* p : {(const void*)pos, (const void*)eloc, (const void*)O, (const void*)dE, (const void*)F,
+                        (const void*)Om})
+    if (p && !is_device_ptr(p))
+      return fail(QMC_ERR_INVALID, "qmc_derivatives_noalloc takes device pointers only (it stages nothing)");
+  const Cols C(sys->h);
+  const int64_t wmax = NOALLOC_SCRATCH_DOUBLES / C.Qp;
+  if (W > wmax)
+    return fail(QMC_ERR_INVALID, "qmc_derivatives_noalloc: " + std::to_string(W) + " walkers exceed the " +
+                                     std::to_string(wmax) + " its " +
+                                     std::to_string(NOALLOC_SCRATCH_DOUBLES * 8 >> 20) + " MiB scratch holds for " +
+                                     std::to_string(C.Qp) + " feature rows");
+  double* Y = nullptr;
+  HIP_TRY(noalloc_scratch(&Y));
+  hipStream_t st = (hipStream_t)stream;
+  DerivArgs A{W, C.P, C.M, C.Qp, pos, nullptr, nullptr, 0, Y, W};
+  if (int rc = launch_derivs(sys, A, sys->precision, st)) return rc;
+  static_assert(3 * MAXN <= (int)(sizeof(DerivUnpack::fnn) / sizeof(double)), "fnn holds 3 MAXN forces");
+  DerivUnpack U{W, C.P, C.M, Y, eloc, O, dE, F, Om, {}};
+  hf_nn_fill(sys->h, U.fnn);
+  HIP_TRY(launch_deriv_unpack(U, st));
+  return QMC_OK;
 }
 
 int qmc_sr_solve(int32_t n, const double* S, const double* f, double eps, double* dp) {
