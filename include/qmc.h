@@ -342,7 +342,13 @@ typedef struct {
   double step_size;        /* Metropolis step (bohr) */
   int32_t decorrelation;   /* sweeps between measurements (reference: 5, sr_optimize.rs:145) */
   int32_t move;            /* QMC_MOVE_ALL as the reference, or QMC_MOVE_SINGLE */
-  int32_t include_nuclear; /* 1: solve the (P+3M) system with Omega appended to O */
+  int32_t include_nuclear; /* 1: solve the (P+3M) system with Omega appended to O (the
+                            * 47 x 47 S of the CH4 differentiation workload) and apply its
+                            * first P entries: the parameter step with the parameter-
+                            * geometry covariances of S taken into account.  The 3M
+                            * nuclear entries are not applied (nuclei move only through
+                            * qmc_set_nuclei / qmc_geometry_optimize).  0: the reference's
+                            * P x P update (sr_optimize.rs:238). */
   int32_t refresh_interval;
 } qmc_sr_params;
 /* SRResult (sr_optimize.rs:351-366) */

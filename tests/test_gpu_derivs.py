@@ -175,6 +175,33 @@ def test_sr_step_and_zv_forces_match_oracle(Q, oracle):
     assert np.allclose(m["f_var"], fv0, rtol=1e-7, atol=1e-9)
 
 
+def test_sr_step_47_columns_matches_oracle(Q, oracle):
+    """The CH4 differentiation workload's 47-dimensional SR system (32
+    Jastrow parameters + 15 nuclear coordinates, O = [O_J, Omega]): the step
+    solved from the GPU moments equals the oracle's literal
+    compute_sr_update (sr_optimize.rs:176-246) on the same 47-column
+    samples, and so does its parameter block."""
+    g, o = _pair(Q, oracle, "ch4_sto3g", series=True)
+    W = 600
+    ens = Q.Ensemble(g, W, 5)
+    ens.sweeps(20, 0.3)
+    ens.measure(reset=True)
+    m = ens.moments()
+    pos = ens.positions()
+    E = np.array([o.local_energy(r) for r in pos])
+    Oj = np.array([o.log_derivatives(r) for r in pos])
+    Om = np.array([np.asarray(o.wfn_nuclear_gradient(r)).ravel() for r in pos])
+    O = np.hstack([Oj, Om])
+    assert O.shape == (W, 47) and m["S"].shape == (47, 47)
+    dp_o, em, var, sing = oracle.sr_update(E, O, 1e-3)
+    dp, singular = Q.sr_solve(m["S"], m["f"], 1e-3)
+    assert not singular and sing == 0
+    scale = np.abs(dp_o).max()
+    assert np.allclose(dp, dp_o, rtol=1e-6, atol=1e-8 * scale), np.abs(dp - dp_o).max() / scale
+    assert m["energy"] == pytest.approx(em, rel=1e-10)
+    assert m["variance"] == pytest.approx(var, rel=1e-8)
+
+
 def test_sr_optimize_reference_case(Q):
     """sr_optimize.rs:399-427: MethaneGTO from (b_ee, b_en) = (0.5, 1.0);
     the final energy lies in (-60, -30) Ha with a finite positive variance

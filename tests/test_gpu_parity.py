@@ -262,9 +262,11 @@ def test_compiled_shell_kernels_match_generic(Q, oracle, name, precision):
 
 @pytest.mark.parametrize("name", SYSTEMS)
 def test_compiled_shell_kernels_all_electron(Q, oracle, name):
-    """All-electron moves (vmc.rs:95-119) with the specialised kernels agree
-    with the run-time-loop kernels: ensemble state after the build and after
-    each step."""
+    """All-electron moves (vmc.rs:95-119) run the run-time-loop kernel under
+    both engine settings (the compiled-shell all-electron kernels are not
+    built; see test_compiled_shell_all_electron_kernel): the state built by
+    the compiled-shell refresh equals the oracle's ln|psi|, and the steps
+    taken after it equal those taken after the run-time-loop refresh."""
     g, o = _pair(Q, oracle, name)
     W, seed, step = 66, 5, 0.3
     res = []
@@ -282,3 +284,45 @@ def test_compiled_shell_kernels_all_electron(Q, oracle, name):
     for a, b in zip(res[0], res[1]):
         for x, y in zip(a, b):
             assert _close(x, y, 1e-10)
+
+
+@pytest.mark.skip(reason="the all-electron kernel compiled for a shell structure is not built into the library: "
+                         "its CH4 6-31G FP32 instance faulted the GPU (memory aperture violation) and its CH4 "
+                         "STO-3G FP64 instance once gave wrong results; the host build of the same source agrees "
+                         "with the run-time-loop kernel and the oracle, and no cause was found in the source")
+def test_compiled_shell_all_electron_kernel():
+    """Placeholder for the compiled-shell all-electron kernel, which stays out
+    of the library until the cause of its fault is shown."""
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_compiled_shell_sweeps_match_generic_at_full_size(Q, oracle, precision):
+    """The headline kernels (CH4 STO-3G sweeps and FP64 refresh compiled for
+    the shell structure) against the run-time-loop kernels on the benchmark's
+    65,536 walkers: the same state after the build, the same accepted moves
+    and positions after four sweeps (FP64 to 1e-9 bohr; in mixed precision
+    the two FP32 instruction orders may round a few acceptance tests
+    differently, so at most 1e-4 of the walkers may part)."""
+    g, _ = _pair(Q, oracle, "ch4_sto3g", precision=precision)
+    W, seed = 65536, 21
+    out = {}
+    for eng in ("auto", "generic"):
+        g.set_engine(eng)
+        ens = Q.Ensemble(g, W, seed)
+        st = ens.state()
+        es, e2, acc = ens.sweeps(4, 0.3, refresh_interval=2, steps_per_launch=4)[:3]
+        out[eng] = (st, acc, ens.positions(), ens.state())
+    g.set_engine("auto")
+    (st_a, acc_a, pos_a, end_a), (st_g, acc_g, pos_g, end_g) = out["auto"], out["generic"]
+    assert _close(st_a[0], st_g[0], 1e-12), "ln|psi| after the build"
+    assert np.array_equal(st_a[1], st_g[1]), "sign after the build"
+    tol_el = 1e-10 if precision == "fp64" else 2e-4
+    assert _close(st_a[2], st_g[2], tol_el, scale=np.maximum(np.abs(st_g[2]), 1.0)), "E_L after the build"
+    moved = np.any(np.abs(pos_a - pos_g) > (1e-9 if precision == "fp64" else 1e-4), axis=(1, 2))
+    if precision == "fp64":
+        assert np.array_equal(acc_a, acc_g)
+        assert not moved.any(), np.nonzero(moved)[0][:10]
+        assert _close(end_a[0], end_g[0], 1e-10)
+    else:
+        assert moved.sum() <= W * 1e-4, moved.sum()
+        assert np.all(np.abs(acc_a - acc_g) <= W * 1e-4 * 10)
