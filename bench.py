@@ -333,15 +333,15 @@ def run_batch(Q, sweeps, trial, precisions=("mixed", "fp64")):
         # start from fresh ensembles, so both precisions' chains stay paired
         tb = Q.Batch([Q.Ensemble(s, W, 100 + i) for i, (s, W) in enumerate(zip(systems, walkers))])
         tb.sweeps(16, steps, refresh_interval=64)
-        cfg, tuned_ms = tb.tune(trial, steps, refresh_interval=64)
+        cfg, tuned_ms = tb.tune(trial, steps, refresh_interval=-256)
         del tb
         ens = [Q.Ensemble(s, W, i) for i, (s, W) in enumerate(zip(systems, walkers))]
         b = Q.Batch(ens)
         b.set_config(**cfg)
-        b.sweeps(64, steps, refresh_interval=64)
+        b.sweeps(64, steps, refresh_interval=-1)
         r = {"tuned_config": cfg, "engines": [s.engine_name() for s in systems]}
         t0 = time.perf_counter()
-        es, e2, acc = b.sweeps(sweeps, steps, refresh_interval=64)
+        es, e2, acc = b.sweeps(sweeps, steps, refresh_interval=-1)
         wall = time.perf_counter() - t0
         kms, nl = b.timing()
         tot = sum(walkers) * sweeps
@@ -351,8 +351,9 @@ def run_batch(Q, sweeps, trial, precisions=("mixed", "fp64")):
                   "energies": [float(t.mean()) for t in tr[prec]]})
         if prec == "mixed":
             for mode in ("fused", "sequential"):
-                b.set_config(mode=mode, blocks_per_cu=0, steps_per_launch=cfg["steps_per_launch"], order="cost")
-                b.sweeps(16, steps, refresh_interval=64)
+                b.set_config(mode=mode, blocks_per_cu=0, steps_per_launch=cfg["steps_per_launch"], order="cost",
+                             refresh_interval=cfg["refresh_interval"])
+                b.sweeps(16, steps, refresh_interval=-1)
                 r[f"ms_per_sweep_{mode}"] = b.timing()[0] / 16
         out[prec] = r
     if "fp64" in tr:

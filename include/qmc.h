@@ -421,6 +421,7 @@ typedef struct {
   int32_t blocks_per_cu;    /* FUSED: persistent workgroups per CU (0: as many as fit) */
   int32_t steps_per_launch; /* sweeps per launch */
   int32_t order;            /* QMC_BATCH_ORDER_* */
+  int32_t refresh_interval; /* FP64 inverse refresh interval the tuner picked (0: none picked) */
 } qmc_batch_config;
 typedef struct qmc_batch qmc_batch;
 /* ens: n distinct ensembles (kept by the caller, n <= 8) */
@@ -432,7 +433,8 @@ int qmc_batch_set_config(qmc_batch* b, const qmc_batch_config* cfg);
 int qmc_batch_get_config(const qmc_batch* b, qmc_batch_config* cfg);
 /* n_steps single-electron sweeps of every ensemble; step_size[n]; outputs
  * (host, may be NULL) [n][n_steps] per-sweep ensemble sums as
- * qmc_ensemble_sweeps */
+ * qmc_ensemble_sweeps; refresh_interval < 0 takes the configuration's tuned
+ * interval (64 when none was tuned) */
 int qmc_batch_sweeps(qmc_batch* b, int32_t n_steps, const double* step_size, int32_t refresh_interval, double* e_sum,
                      double* e2_sum, double* accepted);
 /* kernel milliseconds (first launch start to last launch end, summed over
@@ -441,7 +443,14 @@ int qmc_batch_timing(const qmc_batch* b, double* kernel_ms, int64_t* launches);
 /* Autotuner: times every candidate configuration (modes, workgroups per
  * CU, sweeps per launch, queue order) on trial_steps sweeps of the
  * ensembles themselves (which advance), keeps the fastest and returns it
- * with its kernel milliseconds per sweep. */
+ * with its kernel milliseconds per sweep.  refresh_interval > 0 is the
+ * FP64 inverse refresh interval of the trials; refresh_interval < 0 also
+ * tunes it: of 16, 32, 64, 128 and 256 sweeps (at most -refresh_interval)
+ * it keeps the longest after which, in every ensemble, the local energies
+ * the sweeps computed from the Sherman-Morrison-updated inverses differ from
+ * those of a fresh FP64 inverse by a mean below 0.01 mHa (two standard
+ * errors included), and stores it in the configuration.  The tuned interval
+ * is the one the trials then use. */
 int qmc_batch_tune(qmc_batch* b, int32_t trial_steps, const double* step_size, int32_t refresh_interval,
                    qmc_batch_config* best, double* best_ms_per_sweep);
 

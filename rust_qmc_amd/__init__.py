@@ -774,14 +774,16 @@ class Batch:
         inv_m = {v: k for k, v in _BATCH_MODES.items()}
         inv_o = {v: k for k, v in _BATCH_ORDERS.items()}
         return {"mode": inv_m[c.mode], "blocks_per_cu": c.blocks_per_cu, "steps_per_launch": c.steps_per_launch,
-                "order": inv_o[c.order]}
+                "order": inv_o[c.order], "refresh_interval": c.refresh_interval}
 
-    def set_config(self, mode="fused", blocks_per_cu=0, steps_per_launch=8, order="cost"):
-        c = _lib.QmcBatchConfig(_BATCH_MODES[mode], blocks_per_cu, steps_per_launch, _BATCH_ORDERS[order])
+    def set_config(self, mode="fused", blocks_per_cu=0, steps_per_launch=8, order="cost", refresh_interval=0):
+        c = _lib.QmcBatchConfig(_BATCH_MODES[mode], blocks_per_cu, steps_per_launch, _BATCH_ORDERS[order],
+                                refresh_interval)
         check(lib().qmc_batch_set_config(self._h, C.byref(c)))
 
     def sweeps(self, n_steps, step_sizes, refresh_interval=64):
-        """Per ensemble and sweep: sums of E_L, E_L^2 and accepted moves, each [n][n_steps]."""
+        """Per ensemble and sweep: sums of E_L, E_L^2 and accepted moves, each
+        [n][n_steps].  refresh_interval < 0 takes the tuned interval."""
         n = len(self.ensembles)
         st = _f64(step_sizes).reshape(n)
         es, e2, acc = (np.zeros((n, n_steps)) for _ in range(3))
@@ -794,7 +796,9 @@ class Batch:
         return ms.value, nl.value
 
     def tune(self, trial_steps, step_sizes, refresh_interval=64):
-        """Autotuner: keeps and returns the fastest configuration and its ms per sweep."""
+        """Autotuner: keeps and returns the fastest configuration and its ms
+        per sweep; refresh_interval < 0 also tunes the FP64 inverse refresh
+        interval (up to -refresh_interval) under the drift guard."""
         st = _f64(step_sizes).reshape(len(self.ensembles))
         c = _lib.QmcBatchConfig()
         ms = C.c_double()

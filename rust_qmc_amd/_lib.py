@@ -109,7 +109,8 @@ class QmcForceParams(C.Structure):
 
 
 class QmcBatchConfig(C.Structure):
-    _fields_ = [("mode", _i32), ("blocks_per_cu", _i32), ("steps_per_launch", _i32), ("order", _i32)]
+    _fields_ = [("mode", _i32), ("blocks_per_cu", _i32), ("steps_per_launch", _i32), ("order", _i32),
+                ("refresh_interval", _i32)]
 
 
 class QmcDdvmcParams(C.Structure):

@@ -19,7 +19,7 @@ struct BatchDev {
   TileArgs TA;
 };
 
-__global__ void __launch_bounds__(256) batch_sweep_kernel(BatchDev B) {
+__global__ void __launch_bounds__(256, 2) batch_sweep_kernel(BatchDev B) {
   extern __shared__ float lds_f[];
   __shared__ int64_t item;
   const int tid = threadIdx.x;

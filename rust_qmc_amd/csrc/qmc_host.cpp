@@ -2269,6 +2269,51 @@ int batch_run(qmc_batch* b, int32_t n_steps, const double* step_size, int32_t re
   return QMC_OK;
 }
 
+// Sherman-Morrison drift of the local energy after n sweeps from a fresh
+// FP64 inverse: the E_L the sweeps computed minus the E_L of a fresh inverse
+// at the same positions; mean and standard error over the walkers of each
+// ensemble, the worst ensemble's |mean| + 2 sem returned in *bound (Ha).
+int batch_drift(qmc_batch* b, const double* step_size, int n, double* bound) {
+  int rc;
+  for (qmc_ensemble* e : b->ens)
+    if ((rc = ens_refresh(e, 0))) return rc;
+  if ((rc = batch_run(b, n, step_size, 0, nullptr, nullptr, nullptr))) return rc;
+  *bound = 0.0;
+  for (qmc_ensemble* e : b->ens) {
+    std::vector<double> a(e->W), f(e->W);
+    HIP_TRY(hipStreamSynchronize(e->st));
+    HIP_TRY(hipMemcpy(a.data(), e->eloc, sizeof(double) * e->W, hipMemcpyDeviceToHost));
+    if ((rc = ens_refresh(e, 1))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->st));
+    HIP_TRY(hipMemcpy(f.data(), e->eloc, sizeof(double) * e->W, hipMemcpyDeviceToHost));
+    double m = 0.0, m2 = 0.0;
+    for (int64_t w = 0; w < e->W; ++w) {
+      const double d = a[w] - f[w];
+      m += d;
+      m2 += d * d;
+    }
+    m /= (double)e->W;
+    const double var = std::max(0.0, m2 / (double)e->W - m * m);
+    *bound = std::max(*bound, std::fabs(m) + 2.0 * std::sqrt(var / (double)e->W));
+  }
+  return QMC_OK;
+}
+
+// the longest refresh interval of 16 .. 256 (at most max_interval) whose
+// drift bound stays under 1e-5 Ha in every ensemble
+int batch_tune_refresh(qmc_batch* b, const double* step_size, int max_interval, int* picked) {
+  const double guard = 1e-5;
+  *picked = 16;
+  for (int n : {16, 32, 64, 128, 256}) {
+    if (n > max_interval) break;
+    double bound = 0.0;
+    if (int rc = batch_drift(b, step_size, n, &bound)) return rc;
+    if (!(bound < guard)) break;
+    *picked = n;
+  }
+  return QMC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2313,7 +2358,8 @@ int qmc_batch_fusable(const qmc_batch* b, int32_t* fusable) {
 int qmc_batch_set_config(qmc_batch* b, const qmc_batch_config* c) {
   if (!b || !c) return fail(QMC_ERR_INVALID, "bad arguments");
   if (c->mode < QMC_BATCH_FUSED || c->mode > QMC_BATCH_SEQUENTIAL || c->blocks_per_cu < 0 ||
-      c->steps_per_launch < 0 || (c->order != QMC_BATCH_ORDER_COST && c->order != QMC_BATCH_ORDER_GIVEN))
+      c->steps_per_launch < 0 || c->refresh_interval < 0 ||
+      (c->order != QMC_BATCH_ORDER_COST && c->order != QMC_BATCH_ORDER_GIVEN))
     return fail(QMC_ERR_INVALID, "bad batch configuration");
   if (c->mode == QMC_BATCH_FUSED && !b->fusable)
     return fail(QMC_ERR_UNSUPPORTED, "these ensembles cannot share the fused kernel");
@@ -2329,10 +2375,11 @@ int qmc_batch_get_config(const qmc_batch* b, qmc_batch_config* c) {
 
 int qmc_batch_sweeps(qmc_batch* b, int32_t n_steps, const double* step_size, int32_t refresh_interval, double* e_sum,
                      double* e2_sum, double* accepted) {
-  if (!b || !step_size || n_steps < 0 || refresh_interval < 0) return fail(QMC_ERR_INVALID, "bad arguments");
+  if (!b || !step_size || n_steps < 0) return fail(QMC_ERR_INVALID, "bad arguments");
   for (size_t i = 0; i < b->ens.size(); ++i)
     if (!(step_size[i] >= 0.0)) return fail(QMC_ERR_INVALID, "bad step size");
   if (n_steps == 0) return QMC_OK;
+  if (refresh_interval < 0) refresh_interval = b->cfg.refresh_interval > 0 ? b->cfg.refresh_interval : 64;
   return batch_run(b, n_steps, step_size, refresh_interval, e_sum, e2_sum, accepted);
 }
 
@@ -2345,20 +2392,24 @@ int qmc_batch_timing(const qmc_batch* b, double* kernel_ms, int64_t* launches) {
 
 int qmc_batch_tune(qmc_batch* b, int32_t trial_steps, const double* step_size, int32_t refresh_interval,
                    qmc_batch_config* best, double* best_ms_per_sweep) {
-  if (!b || !step_size || trial_steps < 1) return fail(QMC_ERR_INVALID, "bad arguments");
+  if (!b || !step_size || trial_steps < 1 || refresh_interval == 0) return fail(QMC_ERR_INVALID, "bad arguments");
+  int rc;
+  int refresh = refresh_interval;
+  if (refresh_interval < 0 && (rc = batch_tune_refresh(b, step_size, -refresh_interval, &refresh))) return rc;
   std::vector<qmc_batch_config> cand;
   for (int spl : {4, 8, 16}) {
     if (b->fusable)
       for (int bpc : {1, 2})
         for (int order : {QMC_BATCH_ORDER_COST, QMC_BATCH_ORDER_GIVEN})
-          cand.push_back(qmc_batch_config{QMC_BATCH_FUSED, bpc, spl, order});
-    cand.push_back(qmc_batch_config{QMC_BATCH_STREAMS, 0, spl, QMC_BATCH_ORDER_GIVEN});
-    cand.push_back(qmc_batch_config{QMC_BATCH_SEQUENTIAL, 0, spl, QMC_BATCH_ORDER_GIVEN});
+          cand.push_back(qmc_batch_config{QMC_BATCH_FUSED, bpc, spl, order, 0});
+    cand.push_back(qmc_batch_config{QMC_BATCH_STREAMS, 0, spl, QMC_BATCH_ORDER_GIVEN, 0});
+    cand.push_back(qmc_batch_config{QMC_BATCH_SEQUENTIAL, 0, spl, QMC_BATCH_ORDER_GIVEN, 0});
   }
+  for (auto& c : cand) c.refresh_interval = refresh_interval < 0 ? refresh : 0;
+  refresh_interval = refresh;
   const qmc_batch_config keep = b->cfg;
   double best_ms = std::numeric_limits<double>::infinity();
   qmc_batch_config pick = keep;
-  int rc;
   for (const auto& c : cand) {
     b->cfg = c;
     if ((rc = batch_run(b, c.steps_per_launch, step_size, refresh_interval, nullptr, nullptr, nullptr))) {

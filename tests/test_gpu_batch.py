@@ -72,6 +72,31 @@ def test_batch_autotuner_picks_a_working_configuration(Q):
     assert np.isfinite(es).all() and (acc > 0).all()
 
 
+def test_batch_autotuner_refresh_interval_within_the_drift_guard(Q):
+    """refresh_interval < 0 tunes the FP64 inverse refresh interval: the
+    picked interval is one of the candidates, the sweeps then run with it,
+    and after that many sweeps the local energies from the updated inverses
+    stay within the guard's 0.01 mHa of a fresh inverse's (checked here on
+    the ensembles themselves, mean over walkers)."""
+    systems = _systems(Q)
+    ens = [Q.Ensemble(s, W, seed) for s, W, seed in zip(systems, WALKERS, SEEDS)]
+    b = Q.Batch(ens)
+    cfg, ms = b.tune(2, STEPS, refresh_interval=-128)
+    R = cfg["refresh_interval"]
+    assert R in (16, 32, 64, 128)
+    for e in ens:
+        e.refresh()
+    b.sweeps(R, STEPS, refresh_interval=0)
+    for e in ens:
+        cached = e.state()[2]
+        e.refresh()
+        fresh = e.state()[2]
+        d = cached - fresh
+        assert abs(d.mean()) < 1e-5 + 4 * d.std() / np.sqrt(d.size), (R, d.mean())
+    es, e2, acc = b.sweeps(2 * R + 3, STEPS, refresh_interval=-1)
+    assert np.isfinite(es).all()
+
+
 def test_batch_with_an_fp64_ensemble_is_not_fused(Q):
     s64 = Q.MethaneGTO(1.5, 2.0, 2.06, "STO-3G", 0, precision="fp64")
     s32 = Q.H2MoleculeMO(precision="mixed")
