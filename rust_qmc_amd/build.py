@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libqmc_hip.so")
 
 SOURCES = [
-    "qmc_kernels.hip", "qmc_derivs.hip", "qmc_wave.hip", "qmc_tile.hip", "qmc_batch.hip", "qmc_dmc.hip", "qmc_noalloc.hip", "qmc_host.cpp", "qmc_io.cpp",
+    "qmc_kernels.hip", "qmc_derivs.hip", "qmc_wave.hip", "qmc_tile.hip", "qmc_batch.hip", "qmc_dmc.hip", "qmc_noalloc.hip", "qmc_refresh.hip", "qmc_host.cpp", "qmc_io.cpp",
 ]
 ARCH = os.environ.get("QMC_OFFLOAD_ARCH", "gfx950")
 FLAGS = [

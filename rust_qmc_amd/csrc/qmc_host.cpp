@@ -21,6 +21,7 @@
 #include "qmc_dmc.hpp"
 #include "qmc_kernels.hpp"
 #include "qmc_noalloc.hpp"
+#include "qmc_refresh.hpp"
 #include "qmc_batch.hpp"
 #include "qmc_tile.hpp"
 #include "qmc_wave.hpp"
@@ -1318,9 +1319,11 @@ struct qmc_ensemble {
   size_t gpart_cap = 0;
   double* G = nullptr;
   size_t G_cap = 0;
+  double* rwork = nullptr;  // Slater rows of the tile refresh
+  size_t rwork_cap = 0;
   ~qmc_ensemble() {
     for (void* p : {pos, ainv, (void*)logdet, (void*)sign, (void*)ju, (void*)eloc, (void*)partial, (void*)sums,
-                    (void*)Y, (void*)gpart, (void*)G})
+                    (void*)Y, (void*)gpart, (void*)G, (void*)rwork})
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   }
@@ -1337,9 +1340,25 @@ struct qmc_ensemble {
 
 namespace {
 
+// The periodic refresh of benzene-size ensembles (no local energy wanted)
+// runs on tiles of 16 walkers with FP64 matrix cores; QMC_TILE_REFRESH=0 in
+// the environment sends it to the wave engine's build instead.
+bool tile_refresh_on() {
+  const char* v = std::getenv("QMC_TILE_REFRESH");
+  return !(v && v[0] == '0');
+}
+
+int ensure_cap(double** p, size_t* cap, size_t n);
+
 int ens_refresh(qmc_ensemble* e, int want_eloc) {
   const qmc_system* s = e->sys;
-  if (s->wave()) {
+  if (s->wave() && !want_eloc && s->tile.ok && tile_refresh_on()) {
+    if (int rc = ensure_cap(&e->rwork, &e->rwork_cap, tile_refresh_work(e->W))) return rc;
+    if (e->prec == QMC_FP64)
+      HIP_TRY(tile_refresh<double>(s->d64, e->view<double>(), s->tile, e->rwork, nullptr, e->st));
+    else
+      HIP_TRY(tile_refresh<float>(s->d64, e->view<float>(), s->tile, e->rwork, nullptr, e->st));
+  } else if (s->wave()) {
     if (e->prec == QMC_FP64)
       HIP_TRY(wave_build<double>(s->d64, s->d64, e->view<double>(), s->shape(), want_eloc, nullptr, e->st));
     else
