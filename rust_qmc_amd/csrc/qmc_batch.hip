@@ -34,8 +34,8 @@ __global__ void __launch_bounds__(256, 2) batch_sweep_kernel(BatchDev B) {
     const BatchSeg& G = B.seg[si];
     const int64_t it = t - B.start[si];
     if (G.kind == SEG_TILE_21_21) {
-      tile_sweep_block<21, 21, 17, JsNone>(*(const DevSys<float>*)G.S, G.E, B.TA, G.A, G.partial, nullptr, it,
-                                           G.nblk, tid, lds_f);
+      tile_sweep_block<21, 21, 17, JsNone, float>(*(const DevSys<float>*)G.S, G.E, B.TA, G.A, G.partial, nullptr,
+                                                  it, G.nblk, tid, lds_f);
     } else {
       const int64_t blk = it * 4 + (tid >> 6);
       if (blk < G.nblk) {

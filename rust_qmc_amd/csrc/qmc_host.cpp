@@ -132,8 +132,11 @@ struct qmc_system {
   bool wave() const {
     return engine == QMC_ENGINE_WAVE || engine == QMC_ENGINE_TILE || (engine == QMC_ENGINE_AUTO && !ops);
   }
+  // single-electron sweeps on the tile engine (FP32 or FP64 state) unless
+  // the wave engine is selected
   bool tile_sweeps(int32_t prec) const {
-    return wave() && tile.ok && prec == QMC_MIXED && engine != QMC_ENGINE_WAVE;
+    (void)prec;
+    return wave() && tile.ok && engine != QMC_ENGINE_WAVE;
   }
   WaveShape shape() const { return WaveShape{h.n_elec(), h.n_up(), h.n_dn(), h.n_ao()}; }
   // the Jastrow has no Pade series terms (the kernels compiled without them apply)
@@ -1396,7 +1399,10 @@ int64_t sweep_blocks(const qmc_ensemble* e) {
 int enqueue_single_sweeps(qmc_ensemble* e, const SweepArgs& A, double* trace, hipStream_t st) {
   const qmc_system* s = e->sys;
   if (s->tile_sweeps(e->prec)) {
-    HIP_TRY(tile_sweep(s->d32, e->view<float>(), s->tile, A, e->partial, trace, st));
+    if (e->prec == QMC_FP64)
+      HIP_TRY(tile_sweep<double>(s->d64, e->view<double>(), s->tile, A, e->partial, trace, st));
+    else
+      HIP_TRY(tile_sweep<float>(s->d32, e->view<float>(), s->tile, A, e->partial, trace, st));
   } else if (s->wave()) {
     if (e->prec == QMC_FP64)
       HIP_TRY(wave_sweep<double>(s->d64, e->view<double>(), s->shape(), A, e->partial, trace, st));

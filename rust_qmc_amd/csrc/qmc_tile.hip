@@ -23,12 +23,21 @@
 
 namespace qmc {
 
-template <int NU, int ND, int KS, class Js>
-__global__ void __launch_bounds__(TILE_T, 2) tile_sweep_kernel(const DevSys<float>* __restrict__ Sp, EnsView<float> E,
-                                                            TileArgs TA, SweepArgs A, double* __restrict__ partial,
-                                                            double* __restrict__ el_trace) {
-  extern __shared__ float lds_f[];
-  tile_sweep_block<NU, ND, KS, Js>(*Sp, E, TA, A, partial, el_trace, blockIdx.x, gridDim.x, threadIdx.x, lds_f);
+// FP32 holds two workgroups per CU (a few registers spill, against twice
+// the latency hiding); FP64, whose inverse columns take twice the
+// registers, one.
+template <class T>
+struct TileOcc {
+  static constexpr int v = sizeof(T) == 4 ? 2 : 1;
+};
+
+template <int NU, int ND, int KS, class Js, class T>
+__global__ void __launch_bounds__(TILE_T, TileOcc<T>::v)
+    tile_sweep_kernel(const DevSys<T>* __restrict__ Sp, EnsView<T> E, TileArgs TA, SweepArgs A,
+                      double* __restrict__ partial, double* __restrict__ el_trace) {
+  extern __shared__ double lds_d[];
+  tile_sweep_block<NU, ND, KS, Js, T>(*Sp, E, TA, A, partial, el_trace, blockIdx.x, gridDim.x, threadIdx.x,
+                                      (T*)lds_d);
 }
 
 // ---------------------------------------------------------------------------
@@ -71,23 +80,29 @@ bool tile_plan(const DevSys<float>& D, TilePlan& P) {
   return true;
 }
 
-template <int NU, int ND, int KS, class Js>
-static hipError_t launch_tile(const void* St, EnsView<float> E, const TilePlan& P, SweepArgs A, double* partial,
+template <int NU, int ND, int KS, class Js, class T>
+static hipError_t launch_tile(const void* St, EnsView<T> E, const TilePlan& P, SweepArgs A, double* partial,
                               double* trace, hipStream_t st) {
-  const TileArgs T = make_tile_args<KS>(P);
-  const size_t bytes = sizeof(float) * (size_t)T.lds_floats;
-  auto k = tile_sweep_kernel<NU, ND, KS, Js>;
+  const TileArgs TA = make_tile_args<KS>(P, (int)sizeof(T));
+  const size_t bytes = sizeof(T) * (size_t)TA.lds_floats;
+  auto k = tile_sweep_kernel<NU, ND, KS, Js, T>;
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
-  k<<<(unsigned)tile_blocks(E.W), TILE_T, bytes, st>>>((const DevSys<float>*)St, E, T, A, partial, trace);
+  k<<<(unsigned)tile_blocks(E.W), TILE_T, bytes, st>>>((const DevSys<T>*)St, E, TA, A, partial, trace);
   return hipGetLastError();
 }
 
-hipError_t tile_sweep(const void* St, EnsView<float> E, const TilePlan& P, SweepArgs A, double* partial,
-                      double* trace, hipStream_t st) {
+template <class T>
+hipError_t tile_sweep(const void* St, EnsView<T> E, const TilePlan& P, SweepArgs A, double* partial, double* trace,
+                      hipStream_t st) {
   if (!P.ok) return hipErrorInvalidValue;
-  if (P.no_series) return launch_tile<21, 21, 17, JsNone>(St, E, P, A, partial, trace, st);
-  return launch_tile<21, 21, 17, JsRt>(St, E, P, A, partial, trace, st);
+  if (P.no_series) return launch_tile<21, 21, 17, JsNone, T>(St, E, P, A, partial, trace, st);
+  return launch_tile<21, 21, 17, JsRt, T>(St, E, P, A, partial, trace, st);
 }
+
+template hipError_t tile_sweep<float>(const void*, EnsView<float>, const TilePlan&, SweepArgs, double*, double*,
+                                      hipStream_t);
+template hipError_t tile_sweep<double>(const void*, EnsView<double>, const TilePlan&, SweepArgs, double*, double*,
+                                       hipStream_t);
 
 }  // namespace qmc

@@ -26,11 +26,12 @@ struct TilePlan {
 // 6 primitives)
 bool tile_plan(const DevSys<float>& D, TilePlan& P);
 
-// single-electron sweeps of a mixed-precision ensemble (FP32 state); the
-// FP64 inverse refresh and the state build are the wave engine's, which
-// keeps the same HBM layout
-hipError_t tile_sweep(const void* St, EnsView<float> E, const TilePlan& P, SweepArgs A, double* partial,
-                      double* trace, hipStream_t st);
+// single-electron sweeps of an ensemble with FP32 state (mixed precision,
+// St the FP32 system) or FP64 state (St the FP64 system); the FP64 inverse
+// refresh and the state build keep the wave engine's HBM layout
+template <class T>
+hipError_t tile_sweep(const void* St, EnsView<T> E, const TilePlan& P, SweepArgs A, double* partial, double* trace,
+                      hipStream_t st);
 
 inline int64_t tile_blocks(int64_t W) { return (W + TILE_W - 1) / TILE_W; }
 

@@ -1,5 +1,6 @@
 // qmc_tile_dev.hpp -- device code of the tile engine (see qmc_tile.hip):
-// a 256-thread workgroup owns a tile of 16 walkers, 16 lanes per walker.
+// a 256-thread workgroup owns a tile of 16 walkers, 16 lanes per walker,
+// in FP32 (mixed precision) or FP64; T is the walker state's type.
 #pragma once
 
 #include "qmc_engine.hpp"
@@ -8,18 +9,42 @@
 namespace qmc {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// One 16x16x4 matrix-core step in T, D += A B: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16].  The two types lay the result out differently: register
+// r of lane l is D[4 (l/16) + r][l%16] for FP32 (v_mfma_f32_16x16x4f32) and
+// D[4 r + l/16][l%16] for FP64 (v_mfma_f64_16x16x4f64).
+template <class T>
+struct TileMfma;
+template <>
+struct TileMfma<float> {
+  typedef f32x4 V;
+  static __device__ __forceinline__ V step(float a, float b, V d) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+template <>
+struct TileMfma<double> {
+  typedef f64x4 V;
+  static __device__ __forceinline__ V step(double a, double b, V d) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * r + (l >> 4); }
+};
 
 constexpr int TILE_T = 256;   // threads per tile
 constexpr int TILE_MO = 32;   // MO columns of Phi (two 16-wide MFMA tiles)
 constexpr int TILE_MAXP = 6;  // primitives per shell
-constexpr int TILE_REC = 32;  // floats per shell record in LDS
+constexpr int TILE_REC = 32;  // elements per shell record in LDS
 
 struct TileArgs {
   int N, A, XS, n_nuc, n_shell;
   int lane_sh[16][TILE_MAXSH];  // shells evaluated by group lane g (-1: none)
   // LDS carve-out in floats
   int o_pos, o_x, o_phi, o_spin, o_sh, o_sum;
-  int lds_floats;  // total carve-out
+  int lds_floats;  // total carve-out, in elements of the state type
 };
 
 // sum over the 16 lanes of a group (butterfly inside the 16-lane row)
@@ -33,45 +58,54 @@ __device__ __forceinline__ V gsum(V v) {
 __device__ __forceinline__ int as_int(float f) { return __float_as_int(f); }
 __device__ __forceinline__ float as_float(int i) { return __int_as_float(i); }
 
+// Integers in a shell record of T: bit patterns in a float record, exact
+// values in a double record.
+__device__ __forceinline__ float rec_of(int i, float) { return as_float(i); }
+__device__ __forceinline__ double rec_of(int i, double) { return (double)i; }
+__device__ __forceinline__ int rec_int(float v) { return as_int(v); }
+__device__ __forceinline__ int rec_int(double v) { return (int)v; }
+
 // One shell record: kind, first AO, primitive count, P1 component (ints),
 // centre, then exponents, s and p coefficients and STO powers of up to
 // TILE_MAXP primitives (the DevSys shell and primitive tables, per shell).
-__device__ __forceinline__ void tile_write_record(const DevSys<float>& S, int sh, float* rec) {
-  rec[0] = as_float(S.sh_kind[sh]);
-  rec[1] = as_float(S.sh_ao0[sh]);
-  rec[2] = as_float(S.sh_nprim[sh]);
-  rec[3] = as_float(S.sh_m[sh]);
+template <class T>
+__device__ __forceinline__ void tile_write_record(const DevSys<T>& S, int sh, T* rec) {
+  rec[0] = rec_of(S.sh_kind[sh], T());
+  rec[1] = rec_of(S.sh_ao0[sh], T());
+  rec[2] = rec_of(S.sh_nprim[sh], T());
+  rec[3] = rec_of(S.sh_m[sh], T());
   const int c = S.sh_center[sh], p0 = S.sh_prim0[sh], np = S.sh_nprim[sh];
   rec[4] = S.nuc[c][0];
   rec[5] = S.nuc[c][1];
   rec[6] = S.nuc[c][2];
-  rec[7] = 0.f;
+  rec[7] = T(0);
   for (int k = 0; k < TILE_MAXP; ++k) {
     const bool v = k < np;
-    rec[8 + k] = v ? S.prim_exp[p0 + k] : 0.f;
-    rec[14 + k] = v ? S.coef_s[p0 + k] : 0.f;
-    rec[20 + k] = v ? S.coef_p[p0 + k] : 0.f;
-    rec[26 + k] = as_float(v ? S.prim_pow[p0 + k] : 0);
+    rec[8 + k] = v ? S.prim_exp[p0 + k] : T(0);
+    rec[14 + k] = v ? S.coef_s[p0 + k] : T(0);
+    rec[20 + k] = v ? S.coef_p[p0 + k] : T(0);
+    rec[26 + k] = rec_of(v ? S.prim_pow[p0 + k] : 0, T());
   }
 }
 
 // AO values of one shell from its LDS record, written to chi[ao]; the same
 // arithmetic as shell_values (methane.rs:498-590, slater.rs:17-41, 92-144)
-__device__ __forceinline__ void tile_shell_values(const float* rec, float x, float y, float z, float* chi) {
-  const int kind = as_int(rec[0]), a0 = as_int(rec[1]), np = as_int(rec[2]);
-  const float dx = x - rec[4], dy = y - rec[5], dz = z - rec[6];
-  const float r2 = dx * dx + dy * dy + dz * dz;
+template <class T>
+__device__ __forceinline__ void tile_shell_values(const T* rec, T x, T y, T z, T* chi) {
+  const int kind = rec_int(rec[0]), a0 = rec_int(rec[1]), np = rec_int(rec[2]);
+  const T dx = x - rec[4], dy = y - rec[5], dz = z - rec[6];
+  const T r2 = dx * dx + dy * dy + dz * dz;
   if (kind <= SH_P1) {
-    float rs = 0.f, rp = 0.f;
+    T rs = T(0), rp = T(0);
     for (int k = 0; k < np; ++k) {
-      const float e = M<float>::ex(-rec[8 + k] * r2);
+      const T e = M<T>::ex(-rec[8 + k] * r2);
       rs += rec[14 + k] * e;
       rp += rec[20 + k] * e;
     }
     if (kind == SH_S) {
       chi[a0] = rs;
     } else if (kind == SH_P1) {
-      const int m = as_int(rec[3]);
+      const int m = rec_int(rec[3]);
       chi[a0] = (m == 0 ? dx : (m == 1 ? dy : dz)) * rp;
     } else {
       int b = a0;
@@ -84,26 +118,26 @@ __device__ __forceinline__ void tile_shell_values(const float* rec, float x, flo
       chi[b + 2] = dz * rp;
     }
   } else {
-    const float sr = M<float>::sq(r2);
-    float v = 0.f;
+    const T sr = M<T>::sq(r2);
+    T v = T(0);
     for (int k = 0; k < np; ++k) {
-      float sp = 1.f;
-      for (int q = 0; q < as_int(rec[26 + k]); ++q) sp *= sr;
-      v += rec[14 + k] * sp * M<float>::ex(-rec[8 + k] * sr);
+      T sp = T(1);
+      for (int q = 0; q < rec_int(rec[26 + k]); ++q) sp *= sr;
+      v += rec[14 + k] * sp * M<T>::ex(-rec[8 + k] * sr);
     }
     chi[a0] = v;
   }
 }
 
 // e-n Jastrow value with the charge already in a register (uen_val)
-template <class Js>
-__device__ __forceinline__ float tile_uen(const DevSys<float>& S, float r, float Z, int I) {
-  const float D = 1.f + S.b_en * r;
-  const float y = r * M<float>::rcp(D);
-  float v = -Z * y;
+template <class Js, class T>
+__device__ __forceinline__ T tile_uen(const DevSys<T>& S, T r, T Z, int I) {
+  const T D = T(1) + S.b_en * r;
+  const T y = r * M<T>::rcp(D);
+  T v = -Z * y;
   const int K = Js::ken(S);
   if (K > 0) {
-    float s, s1, s2;
+    T s, s1, s2;
     series(K, S.d_en[S.species[I]], y, s, s1, s2);
     v += s;
   }
@@ -111,71 +145,71 @@ __device__ __forceinline__ float tile_uen(const DevSys<float>& S, float r, float
 }
 
 // n_steps single-electron sweeps of tile blk (16 walkers) by a 256-thread
-// workgroup; lds_f is the workgroup's LDS carve-out described by TA.
-template <int NU, int ND, int KS, class Js>
-__device__ __forceinline__ void tile_sweep_block(const DevSys<float>& S, EnsView<float> E, const TileArgs& TA,
-                                                 SweepArgs A, double* __restrict__ partial,
-                                                 double* __restrict__ el_trace, int64_t blk, int64_t nblk, int tid,
-                                                 float* lds_f) {
+// workgroup; lds is the workgroup's LDS carve-out described by TA, in T.
+template <int NU, int ND, int KS, class Js, class T>
+__device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> E, const TileArgs& TA, SweepArgs A,
+                                                 double* __restrict__ partial, double* __restrict__ el_trace,
+                                                 int64_t blk, int64_t nblk, int tid, T* lds) {
   constexpr int KH = (KS + 1) / 2;  // MFMA steps per K half
+  typedef TileMfma<T> MF;
   const int q = tid >> 6, l = tid & 63;   // wavefront, lane
   const int wt = tid >> 4, g = tid & 15;  // walker in the tile, lane in its group
   const int64_t w = blk * TILE_W + wt;
   const bool live = w < E.W;
   const int64_t wl = live ? w : E.W - 1;  // dead walkers shadow the last one and write nothing
   const int N = TA.N, M_ = TA.n_nuc;
-  float* pos = lds_f + TA.o_pos;  // [TILE_W][3N]
-  float* X = lds_f + TA.o_x;      // [TILE_W][XS], AO values of the proposals
-  float* Phi = lds_f + TA.o_phi;  // [2 K halves][TILE_W][TILE_MO]
-  int* spin_of = (int*)(lds_f + TA.o_spin);
-  float* shrec = lds_f + TA.o_sh;  // [n_shell][TILE_REC]
-  double* sums = (double*)(lds_f + TA.o_sum);  // [TILE_W][3], per-step walker sums
-  float* pw = pos + wt * 3 * N;
+  T* pos = lds + TA.o_pos;  // [TILE_W][3N]
+  T* X = lds + TA.o_x;      // [TILE_W][XS], AO values of the proposals
+  T* Phi = lds + TA.o_phi;  // [2 K halves][TILE_W][TILE_MO]
+  int* spin_of = (int*)(lds + TA.o_spin);
+  T* shrec = lds + TA.o_sh;  // [n_shell][TILE_REC]
+  double* sums = (double*)(lds + TA.o_sum);  // [TILE_W][3], per-step walker sums
+  T* pw = pos + wt * 3 * N;
 
   // ---- state in: positions and tables to LDS, owned inverse columns to VGPRs ----
   for (int i = g; i < 3 * N; i += 16) pw[i] = E.pos[(int64_t)i * E.ld + wl];
-  for (int i = g; i < TA.XS; i += 16) X[wt * TA.XS + i] = 0.f;
+  for (int i = g; i < TA.XS; i += 16) X[wt * TA.XS + i] = T(0);
   for (int i = tid; i < N; i += TILE_T) spin_of[i] = S.elec_spin[i];
   for (int i = tid; i < TA.n_shell; i += TILE_T) tile_write_record(S, i, shrec + i * TILE_REC);
   int my_sh[TILE_MAXSH];
 #pragma unroll
   for (int j = 0; j < TILE_MAXSH; ++j) my_sh[j] = TA.lane_sh[g][j];
   // this lane's Jastrow partners p = g + 16 t: electrons, nuclei (preloaded) or none
-  float pnx[4], pny[4], pnz[4], pnZ[4];
+  T pnx[4], pny[4], pnz[4], pnZ[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int I = g + 16 * t - N;
     const bool nuc = I >= 0 && I < M_;
-    pnx[t] = nuc ? S.nuc[I][0] : 0.f;
-    pny[t] = nuc ? S.nuc[I][1] : 0.f;
-    pnz[t] = nuc ? S.nuc[I][2] : 0.f;
-    pnZ[t] = nuc ? S.Z[I] : 0.f;
+    pnx[t] = nuc ? S.nuc[I][0] : T(0);
+    pny[t] = nuc ? S.nuc[I][1] : T(0);
+    pnz[t] = nuc ? S.nuc[I][2] : T(0);
+    pnZ[t] = nuc ? S.Z[I] : T(0);
   }
-  float cu[2][NU], cd[2][ND];
+  T cu[2][NU], cd[2][ND];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int k = g + 16 * t;
 #pragma unroll
-    for (int m = 0; m < NU; ++m) cu[t][m] = k < NU ? E.ainv[(int64_t)(m * NU + k) * E.ld + wl] : 0.f;
+    for (int m = 0; m < NU; ++m) cu[t][m] = k < NU ? E.ainv[(int64_t)(m * NU + k) * E.ld + wl] : T(0);
 #pragma unroll
-    for (int m = 0; m < ND; ++m) cd[t][m] = k < ND ? E.ainv[(int64_t)(NU * NU + m * ND + k) * E.ld + wl] : 0.f;
+    for (int m = 0; m < ND; ++m) cd[t][m] = k < ND ? E.ainv[(int64_t)(NU * NU + m * ND + k) * E.ld + wl] : T(0);
   }
   // MFMA B operands: wavefront q computes MO tile q & 1 over K half q >> 1;
   // lane l holds C[mo = 16 tile + l%16][ao = (l/16) KS + half KH + s]
   const int tb = q & 1, kh = q >> 1;
-  float Creg[2][KH];
+  T Creg[2][KH];
 #pragma unroll
   for (int sp = 0; sp < 2; ++sp)
 #pragma unroll
     for (int s = 0; s < KH; ++s) {
       const int kk = kh * KH + s;
       const int ao = (l >> 4) * KS + kk, mo = tb * 16 + (l & 15);
-      Creg[sp][s] = (kk < KS && ao < TA.A && mo < (sp ? ND : NU)) ? S.moT[sp][ao][mo] : 0.f;
+      Creg[sp][s] = (kk < KS && ao < TA.A && mo < (sp ? ND : NU)) ? S.moT[sp][ao][mo] : T(0);
     }
   double logdet = E.logdet[wl], sgn = E.sign[wl], ju = E.ju[wl], el = 0.0;
   const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
   const uint32_t wid = (uint32_t)(A.walker0 + wl);
-  const float sigma = (float)A.step_size;
+  const T sigma = (T)A.step_size;
   const bool en_on = S.en_on;
   __syncthreads();
 
@@ -189,11 +223,11 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<float>& S, EnsView
       for (int k = 0; k < n; ++k) {
         const int e = S.sector_elec[spin][k];
         // A: proposal
-        float z[3];
-        float U;
-        Rng<float>::move(philox(wid, step, (uint32_t)e, RNG_SINGLE, k0, k1), z, U);
-        const float ox = pw[3 * e], oy = pw[3 * e + 1], oz = pw[3 * e + 2];
-        const float nx = ox + sigma * z[0], ny = oy + sigma * z[1], nz = oz + sigma * z[2];
+        T z[3];
+        T U;
+        Rng<T>::move(philox(wid, step, (uint32_t)e, RNG_SINGLE, k0, k1), z, U);
+        const T ox = pw[3 * e], oy = pw[3 * e + 1], oz = pw[3 * e + 2];
+        const T nx = ox + sigma * z[0], ny = oy + sigma * z[1], nz = oz + sigma * z[2];
         // B: AO values of the proposal
 #pragma unroll
         for (int j = 0; j < TILE_MAXSH; ++j)
@@ -201,85 +235,84 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<float>& S, EnsView
         __syncthreads();
         // C: Phi[half][walker][mo] = sum over the half's AOs of X[walker][ao] C[mo][ao]
         if (tb * 16 < n) {
-          f32x4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
-          const float* xr = X + (l & 15) * TA.XS + (l >> 4) * KS + kh * KH;
+          typename MF::V d0 = {T(0), T(0), T(0), T(0)}, d1 = {T(0), T(0), T(0), T(0)};
+          const T* xr = X + (l & 15) * TA.XS + (l >> 4) * KS + kh * KH;
 #pragma unroll
           for (int s = 0; s < KH; ++s) {
             if (kh * KH + s >= KS) break;
-            const float b = spin ? Creg[1][s] : Creg[0][s];
+            const T b = spin ? Creg[1][s] : Creg[0][s];
             if (s & 1)
-              d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[s], b, d1, 0, 0, 0);
+              d1 = MF::step(xr[s], b, d1);
             else
-              d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[s], b, d0, 0, 0, 0);
+              d0 = MF::step(xr[s], b, d0);
           }
-          // lane l, register r holds D[4 (l/16) + r][l % 16]
-          float* ph = Phi + kh * TILE_W * TILE_MO;
+          T* ph = Phi + kh * TILE_W * TILE_MO;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) ph[(4 * (l >> 4) + r) * TILE_MO + tb * 16 + (l & 15)] = d0[r] + d1[r];
+          for (int r = 0; r < 4; ++r) ph[MF::row(l, r) * TILE_MO + tb * 16 + (l & 15)] = d0[r] + d1[r];
         }
         __syncthreads();
         // D: ratio, Jastrow difference, Metropolis test, Sherman-Morrison
-        const float* ph0 = Phi + wt * TILE_MO;
-        const float* ph1 = ph0 + TILE_W * TILE_MO;
-        float v0 = 0.f, v1 = 0.f;
+        const T* ph0 = Phi + wt * TILE_MO;
+        const T* ph1 = ph0 + TILE_W * TILE_MO;
+        T v0 = T(0), v1 = T(0);
         if (spin == 0) {
 #pragma unroll
           for (int m = 0; m < NU; ++m) {
-            const float pm = ph0[m] + ph1[m];
+            const T pm = ph0[m] + ph1[m];
             v0 += pm * cu[0][m];
             v1 += pm * cu[1][m];
           }
         } else {
 #pragma unroll
           for (int m = 0; m < ND; ++m) {
-            const float pm = ph0[m] + ph1[m];
+            const T pm = ph0[m] + ph1[m];
             v0 += pm * cd[0][m];
             v1 += pm * cd[1][m];
           }
         }
         const int owner = (l & 48) | (k & 15);
         const bool hi = k >= 16;
-        const float R = __shfl(hi ? v1 : v0, owner, 64);
-        float du = 0.f;
+        const T R = __shfl(hi ? v1 : v0, owner, 64);
+        T du = T(0);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const int p = g + 16 * t;
           if (p < N) {
             if (p != e) {
-              const float px = pw[3 * p], py = pw[3 * p + 1], pz = pw[3 * p + 2];
+              const T px = pw[3 * p], py = pw[3 * p + 1], pz = pw[3 * p + 2];
               const bool same = spin_of[p] == spin;
-              const float ax = nx - px, ay = ny - py, az = nz - pz;
-              const float bx = ox - px, by = oy - py, bz = oz - pz;
-              du += uee_diff<float, Js>(S, M<float>::sq(ax * ax + ay * ay + az * az),
-                                        M<float>::sq(bx * bx + by * by + bz * bz), same);
+              const T ax = nx - px, ay = ny - py, az = nz - pz;
+              const T bx = ox - px, by = oy - py, bz = oz - pz;
+              du += uee_diff<T, Js>(S, M<T>::dist(ax * ax + ay * ay + az * az),
+                                    M<T>::dist(bx * bx + by * by + bz * bz), same);
             }
           } else if (en_on && p < N + M_) {
-            const float ax = nx - pnx[t], ay = ny - pny[t], az = nz - pnz[t];
-            const float bx = ox - pnx[t], by = oy - pny[t], bz = oz - pnz[t];
-            const float rn = M<float>::sq(ax * ax + ay * ay + az * az), ro = M<float>::sq(bx * bx + by * by + bz * bz);
+            const T ax = nx - pnx[t], ay = ny - pny[t], az = nz - pnz[t];
+            const T bx = ox - pnx[t], by = oy - pny[t], bz = oz - pnz[t];
+            const T rn = M<T>::dist(ax * ax + ay * ay + az * az), ro = M<T>::dist(bx * bx + by * by + bz * bz);
             if constexpr (Js::no_series)  // -Z (r' - r) / ((1+b r')(1+b r)), one reciprocal
-              du += -pnZ[t] * (rn - ro) * M<float>::rcp((1.f + S.b_en * rn) * (1.f + S.b_en * ro));
+              du += -pnZ[t] * (rn - ro) * M<T>::rcp((T(1) + S.b_en * rn) * (T(1) + S.b_en * ro));
             else
               du += tile_uen<Js>(S, rn, pnZ[t], p - N) - tile_uen<Js>(S, ro, pnZ[t], p - N);
           }
         }
         du = gsum(du);
-        const float pacc = R * R * M<float>::ex(2.f * du);
+        const T pacc = R * R * M<T>::ex(T(2) * du);
         if (U < pacc) {  // uniform within the walker's group
-          const float iR = M<float>::rcp(R);
+          const T iR = M<T>::rcp(R);
           const bool own0 = g == (k & 15) && !hi, own1 = g == (k & 15) && hi;
-          const float f0 = v0 * iR, f1 = v1 * iR;
+          const T f0 = v0 * iR, f1 = v1 * iR;
           if (spin == 0) {
 #pragma unroll
             for (int m = 0; m < NU; ++m) {
-              const float c = __shfl(hi ? cu[1][m] : cu[0][m], owner, 64);
+              const T c = __shfl(hi ? cu[1][m] : cu[0][m], owner, 64);
               cu[0][m] = own0 ? c * iR : cu[0][m] - c * f0;
               cu[1][m] = own1 ? c * iR : cu[1][m] - c * f1;
             }
           } else {
 #pragma unroll
             for (int m = 0; m < ND; ++m) {
-              const float c = __shfl(hi ? cd[1][m] : cd[0][m], owner, 64);
+              const T c = __shfl(hi ? cd[1][m] : cd[0][m], owner, 64);
               cd[0][m] = own0 ? c * iR : cd[0][m] - c * f0;
               cd[1][m] = own1 ? c * iR : cd[1][m] - c * f1;
             }
@@ -289,8 +322,8 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<float>& S, EnsView
             pw[3 * e + 1] = ny;
             pw[3 * e + 2] = nz;
           }
-          logdet += (double)M<float>::lg(R < 0.f ? -R : R);
-          if (R < 0.f) sgn = -sgn;
+          logdet += (double)M<T>::lg(R < T(0) ? -R : R);
+          if (R < T(0)) sgn = -sgn;
           ju += (double)du;
           ++acc;
         }
@@ -307,9 +340,9 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<float>& S, EnsView
         const int k = g + 16 * t;
         if (k >= n) continue;
         const int i = S.sector_elec[spin][k];
-        const float x = pw[3 * i], y = pw[3 * i + 1], z = pw[3 * i + 2];
+        const T x = pw[3 * i], y = pw[3 * i + 1], z = pw[3 * i + 2];
         auto wgt = [&](int ao) {
-          float s = 0.f;
+          T s = T(0);
           if (spin == 0) {
 #pragma unroll
             for (int m = 0; m < NU; ++m) s += S.moT[0][ao][m] * cu[t][m];
@@ -319,45 +352,45 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<float>& S, EnsView
           }
           return s;
         };
-        float gdx, gdy, gdz, ld;
+        T gdx, gdy, gdz, ld;
         det_terms_w(S, x, y, z, wgt, gdx, gdy, gdz, ld, NoShell{});
-        float gux = 0.f, guy = 0.f, guz = 0.f, lu = 0.f, pairsq = 0.f, pee = 0.f, pen = 0.f;
+        T gux = T(0), guy = T(0), guz = T(0), lu = T(0), pairsq = T(0), pee = T(0), pen = T(0);
         for (int j = 0; j < N; ++j) {
           if (j == i) continue;
-          const float dx = x - pw[3 * j], dy = y - pw[3 * j + 1], dz = z - pw[3 * j + 2];
-          const float r = M<float>::sq(dx * dx + dy * dy + dz * dz);
-          const float ir = M<float>::rcp(r);
-          float u1, lp;
-          uee_d<float, Js>(S, r, ir, spin_of[j] == spin, u1, lp);
-          const bool ok = r >= 1e-10f;
-          const float f = ok ? u1 * ir : 0.f;
+          const T dx = x - pw[3 * j], dy = y - pw[3 * j + 1], dz = z - pw[3 * j + 2];
+          T r, ir;
+          M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
+          T u1, lp;
+          uee_d<T, Js>(S, r, ir, spin_of[j] == spin, u1, lp);
+          const bool ok = r >= T(1e-10);
+          const T f = ok ? u1 * ir : T(0);
           gux += f * dx;
           guy += f * dy;
           guz += f * dz;
-          lu += ok ? lp : 0.f;
-          pairsq += ok ? u1 * u1 : 0.f;
+          lu += ok ? lp : T(0);
+          pairsq += ok ? u1 * u1 : T(0);
           pee += ir;
         }
         for (int I = 0; I < M_; ++I) {
-          const float dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-          const float r = M<float>::sq(dx * dx + dy * dy + dz * dz);
-          const float ir = M<float>::rcp(r);
+          const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+          T r, ir;
+          M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
           pen += S.Z[I] * ir;
           if (en_on) {
-            float u1, lp;
-            uen_d<float, Js>(S, r, ir, I, u1, lp);
-            const bool ok = r >= 1e-10f;
-            const float f = ok ? u1 * ir : 0.f;
+            T u1, lp;
+            uen_d<T, Js>(S, r, ir, I, u1, lp);
+            const bool ok = r >= T(1e-10);
+            const T f = ok ? u1 * ir : T(0);
             gux += f * dx;
             guy += f * dy;
             guz += f * dz;
-            lu += ok ? lp : 0.f;
+            lu += ok ? lp : T(0);
           }
         }
-        const float gs = S.grad_scale;
-        const float gjx = gs * gux, gjy = gs * guy, gjz = gs * guz;
-        const float lj = (S.flags & F_J2_LAP_QUIRK) ? (lu + pairsq) : (lu + gux * gux + guy * guy + guz * guz);
-        const float lap = ld + lj + 2.f * (gdx * gjx + gdy * gjy + gdz * gjz);
+        const T gs = S.grad_scale;
+        const T gjx = gs * gux, gjy = gs * guy, gjz = gs * guz;
+        const T lj = (S.flags & F_J2_LAP_QUIRK) ? (lu + pairsq) : (lu + gux * gux + guy * guy + guz * guz);
+        const T lap = ld + lj + T(2) * (gdx * gjx + gdy * gjy + gdz * gjz);
         kin += -0.5 * (double)lap;
         pot += 0.5 * (double)pee - (double)pen;
       }
@@ -400,9 +433,10 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<float>& S, EnsView
   }
 }
 
-// kernel arguments of a tile launch: LDS carve-out and lane shell lists
+// kernel arguments of a tile launch: LDS carve-out (in elements of the
+// state type, of esz bytes) and lane shell lists
 template <int KS>
-inline TileArgs make_tile_args(const TilePlan& P) {
+inline TileArgs make_tile_args(const TilePlan& P, int esz = (int)sizeof(float)) {
   TileArgs T;
   T.N = P.N;
   T.A = P.A;
@@ -411,13 +445,14 @@ inline TileArgs make_tile_args(const TilePlan& P) {
   T.n_shell = P.n_shell;
   for (int g = 0; g < 16; ++g)
     for (int j = 0; j < TILE_MAXSH; ++j) T.lane_sh[g][j] = P.lane_sh[g][j];
+  const int per8 = 8 / esz;  // elements per double
   T.o_pos = 0;
   T.o_x = T.o_pos + TILE_W * 3 * P.N;
   T.o_phi = T.o_x + TILE_W * T.XS;
   T.o_spin = T.o_phi + 2 * TILE_W * TILE_MO;
-  T.o_sh = T.o_spin + (P.N + 3) / 4 * 4;
-  T.o_sum = (T.o_sh + P.n_shell * TILE_REC + 1) / 2 * 2;  // 8-byte aligned
-  T.lds_floats = T.o_sum + 2 * 3 * TILE_W;
+  T.o_sh = T.o_spin + ((P.N + 3) / 4 * 4 * 4 + esz - 1) / esz;  // N ints
+  T.o_sum = (T.o_sh + P.n_shell * TILE_REC + per8 - 1) / per8 * per8;  // 8-byte aligned
+  T.lds_floats = T.o_sum + per8 * 3 * TILE_W;
   return T;
 }
 

@@ -102,9 +102,12 @@ def test_wave_engine_mixed_ensemble(Q, oracle):
     assert abs(a - b) < 3 * np.hypot(sa, sb) + 1e-3
 
 
-def test_tile_engine_selected_for_mixed_benzene(Q):
+def test_tile_engine_selected_for_benzene(Q):
     assert Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="mixed").engine_name() == "tile"
-    assert Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="fp64").engine_name() == "wave"
+    assert Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="fp64").engine_name() == "tile"
+    g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="fp64")
+    g.set_engine("wave")
+    assert g.engine_name() == "wave"
 
 
 def test_tile_engine_trajectory_vs_oracle(Q, oracle):
@@ -141,6 +144,44 @@ def test_tile_engine_matches_wave_engine(Q):
     assert np.allclose(p1, p2, rtol=0, atol=2e-4)
     assert _close(t1, t2, 2e-4, scale=np.abs(t2) + 10.0)
     assert _close(s1[0], s2[0], 1e-6, scale=np.abs(s2[0]))
+
+
+def test_tile_engine_fp64_trajectory_vs_oracle(Q, oracle):
+    """FP64 tile-engine sweeps (FP64 MFMA MO contraction v_mfma_f64_16x16x4f64,
+    register-resident FP64 inverse columns) follow the oracle's FP64
+    brute-force chain step for step: same decisions, positions and E_L to
+    1e-9.  3 walkers leave 13 dead walkers in the tile."""
+    g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="fp64")
+    g.set_engine("tile")
+    assert g.engine_name() == "tile"
+    o = oracle.System.benzene(1.5, 2.0, 2.64, 2.05)
+    W, steps, seed, step = 3, 2, 7, 0.2
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    es, e2, acc, tr = ens.sweeps(steps, step, trace=True, refresh_interval=0)
+    pos_o, lp_o, el_o, acc_o = o.single_electron_steps(p0, seed, 0, steps, step)
+    assert acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=1e-9)
+    assert np.allclose(tr, el_o, rtol=1e-9, atol=0), np.abs(tr - el_o).max()
+
+
+def test_tile_engine_fp64_matches_wave_engine(Q):
+    """FP64 sweeps of 37 walkers (three tiles, the last ragged) on the tile
+    and the wave engine from the same seed, with a refresh between: the same
+    decisions, positions, ln|psi| and E_L to 1e-10."""
+    out = []
+    for eng in ("tile", "wave"):
+        g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="fp64")
+        g.set_engine(eng)
+        ens = Q.Ensemble(g, 37, 9)
+        es, e2, acc, tr = ens.sweeps(3, 0.2, trace=True, refresh_interval=2, steps_per_launch=2)
+        out.append((ens.positions(), acc, tr, ens.state()))
+    (p1, a1, t1, s1), (p2, a2, t2, s2) = out
+    assert np.array_equal(a1, a2)
+    assert np.allclose(p1, p2, rtol=0, atol=1e-10)
+    assert np.allclose(t1, t2, rtol=1e-10, atol=0)
+    assert np.allclose(s1[0], s2[0], rtol=1e-11, atol=0)
+    assert np.array_equal(s1[1], s2[1])
 
 
 def test_wave_all_electron_benzene_parity_fp64(Q, oracle):
