@@ -138,6 +138,8 @@ struct qmc_system {
     (void)prec;
     return wave() && tile.ok && engine != QMC_ENGINE_WAVE;
   }
+  // drift-diffusion sweeps of a plain ensemble on the tile engine, the same rule
+  bool tile_dd(int32_t prec) const { return tile_sweeps(prec); }
   WaveShape shape() const { return WaveShape{h.n_elec(), h.n_up(), h.n_dn(), h.n_ao()}; }
   // the Jastrow has no Pade series terms (the kernels compiled without them apply)
   bool no_series() const { return h.k_ee == 0 && h.k_en == 0; }
@@ -1530,9 +1532,13 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
   if (e->sys_version != s->version && (rc = ens_refresh(e, 1))) return rc;  // parameters or nuclei moved
   if (sm && e->inverse_stale && (rc = ens_refresh(e, 0))) return rc;
   const bool wave = s->wave();
-  // drift-diffusion and all-electron moves on the wave engine (also for
-  // tile-engine systems): one block per walker
-  const int nblk = p->move == QMC_MOVE_SINGLE ? (int)sweep_blocks(e) : (int)(wave ? e->W : (e->W + 63) / 64);
+  // drift-diffusion moves on the tile engine where it runs single-electron
+  // sweeps; all-electron moves on the wave engine (also for tile-engine
+  // systems): one block per walker
+  const bool tdd = p->move == QMC_MOVE_DRIFT && s->tile_dd(e->prec);
+  const int nblk = p->move == QMC_MOVE_SINGLE ? (int)sweep_blocks(e)
+                   : tdd                       ? (int)tile_blocks(e->W)
+                                               : (int)(wave ? e->W : (e->W + 63) / 64);
   const int refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
   int chunk = p->steps_per_launch > 0 ? p->steps_per_launch : 16;
   if ((rc = ensure_cap(&e->partial, &e->partial_cap, (size_t)chunk * nblk * 3))) return rc;
@@ -1561,7 +1567,11 @@ int qmc_ensemble_sweeps(qmc_ensemble* e, const qmc_sweep_params* p, double* e_su
     if (p->move == QMC_MOVE_SINGLE) {
       if ((rc = enqueue_single_sweeps(e, A, trace, e->st))) return rc;
     } else if (p->move == QMC_MOVE_DRIFT) {
-      if (wave && e->prec == QMC_FP64)
+      if (tdd && e->prec == QMC_FP64)
+        HIP_TRY(tile_dd<double>(s->d64, e->view<double>(), s->tile, A, e->partial, trace, e->st));
+      else if (tdd)
+        HIP_TRY(tile_dd<float>(s->d32, e->view<float>(), s->tile, A, e->partial, trace, e->st));
+      else if (wave && e->prec == QMC_FP64)
         HIP_TRY(wave_dd<double>(s->d64, e->view<double>(), s->shape(), A, e->partial, trace, e->st));
       else if (wave)
         HIP_TRY(wave_dd<float>(s->d32, e->view<float>(), s->shape(), A, e->partial, trace, e->st));

@@ -108,16 +108,6 @@ __device__ __forceinline__ void rf_shell_values(const double* rec, double x, dou
   }
 }
 
-// Orders one wavefront's LDS accesses across its lanes: a release fence,
-// the wavefront barrier and an acquire fence at wavefront scope, so a value
-// one lane stores is what another lane of the same wavefront loads after
-// it (the compiler may not move either access across).  No s_barrier.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <int NS, int KS, class T>
 __global__ void __launch_bounds__(TILE_T) tile_slater_kernel(const DevSys<double>* __restrict__ Sp, EnsView<T> E,
                                                              RefreshArgs RA) {

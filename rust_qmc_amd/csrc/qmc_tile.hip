@@ -40,6 +40,23 @@ __global__ void __launch_bounds__(TILE_T, TileOcc<T>::v)
                                       (T*)lds_d);
 }
 
+// drift-diffusion sweeps of a tile: the Metropolis kernel's layout and
+// MO contraction, with the moved electron's AO weights staged per walker
+// in LDS and the quantum force at both ends of the move
+template <class T>
+struct TileDdOcc {
+  static constexpr int v = 1;
+};
+
+template <int NU, int ND, int KS, class Js, class T>
+__global__ void __launch_bounds__(TILE_T, TileDdOcc<T>::v)
+    tile_dd_kernel(const DevSys<T>* __restrict__ Sp, EnsView<T> E, TileArgs TA, SweepArgs A,
+                   double* __restrict__ partial, double* __restrict__ el_trace) {
+  extern __shared__ double lds_d[];
+  tile_sweep_block<NU, ND, KS, Js, T, true>(*Sp, E, TA, A, partial, el_trace, blockIdx.x, gridDim.x, threadIdx.x,
+                                            (T*)lds_d);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -100,8 +117,32 @@ hipError_t tile_sweep(const void* St, EnsView<T> E, const TilePlan& P, SweepArgs
   return launch_tile<21, 21, 17, JsRt, T>(St, E, P, A, partial, trace, st);
 }
 
+template <int NU, int ND, int KS, class Js, class T>
+static hipError_t launch_tile_dd(const void* St, EnsView<T> E, const TilePlan& P, SweepArgs A, double* partial,
+                                 double* trace, hipStream_t st) {
+  const TileArgs TA = make_tile_args<KS>(P, (int)sizeof(T), true);
+  const size_t bytes = sizeof(T) * (size_t)TA.lds_floats;
+  auto k = tile_dd_kernel<NU, ND, KS, Js, T>;
+  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  k<<<(unsigned)tile_blocks(E.W), TILE_T, bytes, st>>>((const DevSys<T>*)St, E, TA, A, partial, trace);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t tile_dd(const void* St, EnsView<T> E, const TilePlan& P, SweepArgs A, double* partial, double* trace,
+                   hipStream_t st) {
+  if (!P.ok || E.n_live || !(A.step_size > 0.0)) return hipErrorInvalidValue;
+  if (P.no_series) return launch_tile_dd<21, 21, 17, JsNone, T>(St, E, P, A, partial, trace, st);
+  return launch_tile_dd<21, 21, 17, JsRt, T>(St, E, P, A, partial, trace, st);
+}
+
 template hipError_t tile_sweep<float>(const void*, EnsView<float>, const TilePlan&, SweepArgs, double*, double*,
                                       hipStream_t);
+template hipError_t tile_dd<float>(const void*, EnsView<float>, const TilePlan&, SweepArgs, double*, double*,
+                                   hipStream_t);
+template hipError_t tile_dd<double>(const void*, EnsView<double>, const TilePlan&, SweepArgs, double*, double*,
+                                    hipStream_t);
 template hipError_t tile_sweep<double>(const void*, EnsView<double>, const TilePlan&, SweepArgs, double*, double*,
                                        hipStream_t);
 

@@ -33,6 +33,12 @@ template <class T>
 hipError_t tile_sweep(const void* St, EnsView<T> E, const TilePlan& P, SweepArgs A, double* partial, double* trace,
                       hipStream_t st);
 
+// drift-diffusion sweeps (A.step_size the time step, A.max_drift the drift
+// cap) of an ensemble without a branching population, as tile_sweep
+template <class T>
+hipError_t tile_dd(const void* St, EnsView<T> E, const TilePlan& P, SweepArgs A, double* partial, double* trace,
+                   hipStream_t st);
+
 inline int64_t tile_blocks(int64_t W) { return (W + TILE_W - 1) / TILE_W; }
 
 }  // namespace qmc

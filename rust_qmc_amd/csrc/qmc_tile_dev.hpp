@@ -5,6 +5,7 @@
 
 #include "qmc_engine.hpp"
 #include "qmc_tile.hpp"
+#include "qmc_wave_dev.hpp"
 
 namespace qmc {
 
@@ -38,12 +39,16 @@ constexpr int TILE_T = 256;   // threads per tile
 constexpr int TILE_MO = 32;   // MO columns of Phi (two 16-wide MFMA tiles)
 constexpr int TILE_MAXP = 6;  // primitives per shell
 constexpr int TILE_REC = 32;  // elements per shell record in LDS
+constexpr int TILE_CK = 24;   // row stride of an inverse column or MO-coefficient row in LDS
+constexpr int TILE_AJ = 5;    // AOs per lane in the AO-weight pass: a = g + 16 j, j < 5
+constexpr int TILE_WS = 16 * TILE_AJ;  // row stride of a walker's AO weights
 
 struct TileArgs {
   int N, A, XS, n_nuc, n_shell;
   int lane_sh[16][TILE_MAXSH];  // shells evaluated by group lane g (-1: none)
-  // LDS carve-out in floats
-  int o_pos, o_x, o_phi, o_spin, o_sh, o_sum;
+  // LDS carve-out in elements of the state type; o_cmo, o_ck and o_w only
+  // in a drift-diffusion launch
+  int o_pos, o_x, o_phi, o_spin, o_sh, o_sum, o_cmo, o_ck, o_w;
   int lds_floats;  // total carve-out, in elements of the state type
 };
 
@@ -129,6 +134,79 @@ __device__ __forceinline__ void tile_shell_values(const T* rec, T x, T y, T z, T
   }
 }
 
+// Values and gradients of the AOs of one shell from its LDS record, handed
+// to f(ao, value, d/dx, d/dy, d/dz); the same arithmetic as
+// shell_values_grad (contracted Gaussians with s / p / sp angular parts,
+// radial STO sums whose gradient at the centre is taken as zero).
+template <class T, class F>
+__device__ __forceinline__ void tile_shell_grad(const T* rec, T x, T y, T z, F f) {
+  const int kind = rec_int(rec[0]), a0 = rec_int(rec[1]), np = rec_int(rec[2]);
+  const T dx = x - rec[4], dy = y - rec[5], dz = z - rec[6];
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  if (kind <= SH_P1) {
+    T rs = T(0), rsa = T(0), rp = T(0), rpa = T(0);
+    for (int k = 0; k < np; ++k) {
+      const T a = rec[8 + k];
+      const T e = M<T>::ex(-a * r2);
+      const T cs = rec[14 + k] * e, cp = rec[20 + k] * e;
+      rs += cs;
+      rsa += cs * a;
+      rp += cp;
+      rpa += cp * a;
+    }
+    if (kind == SH_S || kind == SH_SP) {  // grad = -2 Ra d
+      const T t = T(-2) * rsa;
+      f(a0, rs, t * dx, t * dy, t * dz);
+    }
+    if (kind == SH_S) return;
+    // p_m = d_m R: grad_j = delta_jm R - 2 d_m Ra d_j
+    auto pm = [&](int ao, int m) {
+      const T dm = m == 0 ? dx : (m == 1 ? dy : dz);
+      const T t = T(2) * dm * rpa;
+      f(ao, dm * rp, (m == 0 ? rp : T(0)) - t * dx, (m == 1 ? rp : T(0)) - t * dy, (m == 2 ? rp : T(0)) - t * dz);
+    };
+    if (kind == SH_P1) {
+      pm(a0, rec_int(rec[3]));
+    } else {
+      const int b = kind == SH_SP ? a0 + 1 : a0;
+      pm(b, 0);
+      pm(b + 1, 1);
+      pm(b + 2, 2);
+    }
+    return;
+  }
+  // radial STO: f(s) = sum c s^p e^{-zeta s}; grad = f'(s) d / s
+  const T s = M<T>::sq(r2);
+  T v = T(0), d1 = T(0);
+  for (int k = 0; k < np; ++k) {
+    const T zt = rec[8 + k];
+    const int p = rec_int(rec[26 + k]);
+    T sp = T(1);
+    for (int q = 0; q < p; ++q) sp *= s;
+    const T spm1 = p >= 1 ? sp / s : T(0);
+    const T e = rec[14 + k] * M<T>::ex(-zt * s);
+    v += e * sp;
+    d1 += e * (-zt * sp + T(p) * spm1);
+  }
+  const T gs = s == T(0) ? T(0) : d1 / s;
+  f(a0, v, gs * dx, gs * dy, gs * dz);
+}
+
+// e-n Jastrow value and u'(r) with the charge already in a register
+// (uen_val, uen_d)
+template <class Js, class T>
+__device__ __forceinline__ void tile_uen_vd(const DevSys<T>& S, T r, T ir, T Z, int I, T& v, T& u1) {
+  if constexpr (Js::no_series) {
+    const T iD = M<T>::rcp(T(1) + S.b_en * r);
+    v = -Z * (r * iD);
+    u1 = -Z * (iD * iD);
+  } else {
+    T lp;
+    uen_d<T, Js>(S, r, ir, I, u1, lp);
+    v = uen_val<T, Js>(S, r, I);
+  }
+}
+
 // e-n Jastrow value with the charge already in a register (uen_val)
 template <class Js, class T>
 __device__ __forceinline__ T tile_uen(const DevSys<T>& S, T r, T Z, int I) {
@@ -146,7 +224,12 @@ __device__ __forceinline__ T tile_uen(const DevSys<T>& S, T r, T Z, int I) {
 
 // n_steps single-electron sweeps of tile blk (16 walkers) by a 256-thread
 // workgroup; lds is the workgroup's LDS carve-out described by TA, in T.
-template <int NU, int ND, int KS, class Js, class T>
+// DD selects drift-diffusion moves (DriftDiffusionVMC::sweep,
+// vmc.rs:382-438) over Metropolis moves (MCMCSimulation::
+// single_electron_step): the proposal from the quantum force of the moved
+// electron, the Green's function ratio in the acceptance, with the random
+// streams of the register and wave engines.
+template <int NU, int ND, int KS, class Js, class T, bool DD = false>
 __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> E, const TileArgs& TA, SweepArgs A,
                                                  double* __restrict__ partial, double* __restrict__ el_trace,
                                                  int64_t blk, int64_t nblk, int tid, T* lds) {
@@ -171,6 +254,15 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   for (int i = g; i < TA.XS; i += 16) X[wt * TA.XS + i] = T(0);
   for (int i = tid; i < N; i += TILE_T) spin_of[i] = S.elec_spin[i];
   for (int i = tid; i < TA.n_shell; i += TILE_T) tile_write_record(S, i, shrec + i * TILE_REC);
+  if constexpr (DD) {  // MO coefficients, [spin][ao][mo] with rows of TILE_CK
+    static_assert(NU <= TILE_CK && ND <= TILE_CK, "sector too large for the AO-weight rows");
+    T* cmo = lds + TA.o_cmo;
+    for (int i = tid; i < 2 * TA.A * TILE_CK; i += TILE_T) {
+      const int sp = i / (TA.A * TILE_CK), rem = i - sp * TA.A * TILE_CK;
+      const int a = rem / TILE_CK, m = rem - a * TILE_CK;
+      cmo[i] = m < (sp ? ND : NU) ? S.moT[sp][a][m] : T(0);
+    }
+  }
   int my_sh[TILE_MAXSH];
 #pragma unroll
   for (int j = 0; j < TILE_MAXSH; ++j) my_sh[j] = TA.lane_sh[g][j];
@@ -211,6 +303,9 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   const uint32_t wid = (uint32_t)(A.walker0 + wl);
   const T sigma = (T)A.step_size;
   const bool en_on = S.en_on;
+  // drift-diffusion: time step, drift cap, sqrt(2 tau), 1/(4 tau)
+  const T tau = (T)A.step_size, md = (T)A.max_drift;
+  const T s2t = M<T>::sq(T(2) * tau), i4t = T(0.25) / tau;
   __syncthreads();
 
   for (int st = 0; st < A.n_steps; ++st) {
@@ -222,16 +317,154 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
 #pragma unroll 1
       for (int k = 0; k < n; ++k) {
         const int e = S.sector_elec[spin][k];
-        // A: proposal
-        T z[3];
-        T U;
-        Rng<T>::move(philox(wid, step, (uint32_t)e, RNG_SINGLE, k0, k1), z, U);
+        const int owner = (l & 48) | (k & 15);
+        const bool hi = k >= 16;
         const T ox = pw[3 * e], oy = pw[3 * e + 1], oz = pw[3 * e + 2];
-        const T nx = ox + sigma * z[0], ny = oy + sigma * z[1], nz = oz + sigma * z[2];
-        // B: AO values of the proposal
+        T nx, ny, nz, U;
+        [[maybe_unused]] T Rd, dud, pd;  // drift-diffusion: ratio, Jastrow difference, acceptance
+        if constexpr (DD) {
+          // W: AO weights w[a] = sum_m C[a][m] Ainv[m][k] of the moved
+          // electron's inverse column, staged through the group's LDS rows
+          T* ck = lds + TA.o_ck + wt * TILE_CK;
+          T* wa = lds + TA.o_w + wt * TILE_WS;
+          const T* cmo = lds + TA.o_cmo + spin * TA.A * TILE_CK;
+          if (g == (k & 15)) {
 #pragma unroll
-        for (int j = 0; j < TILE_MAXSH; ++j)
-          if (my_sh[j] >= 0) tile_shell_values(shrec + my_sh[j] * TILE_REC, nx, ny, nz, X + wt * TA.XS);
+            for (int m = 0; m < n; ++m) ck[m] = spin ? (hi ? cd[1][m] : cd[0][m]) : (hi ? cu[1][m] : cu[0][m]);
+          }
+          wave_sync();
+          {
+            T c[NU > ND ? NU : ND];
+#pragma unroll
+            for (int m = 0; m < n; ++m) c[m] = ck[m];
+#pragma unroll
+            for (int j = 0; j < TILE_AJ; ++j) {
+              const int a = g + 16 * j;
+              if (a < TA.A) {
+                const T* cr = cmo + a * TILE_CK;
+                T wq = T(0);
+#pragma unroll
+                for (int m = 0; m < n; ++m) wq += cr[m] * c[m];
+                wa[a] = wq;
+              }
+            }
+          }
+          wave_sync();
+          // the Jastrow value and gradient of electron e at (x, y, z) over
+          // this lane's partners p = g + 16 t
+          auto jpart = [&](T x, T y, T z, T& u, T& jx, T& jy, T& jz) {
+            u = jx = jy = jz = T(0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const int p = g + 16 * t;
+              T dx, dy, dz, v = T(0), u1 = T(0), r = T(0), ir = T(0);
+              bool on = false;
+              if (p < N) {
+                if (p != e) {
+                  dx = x - pw[3 * p];
+                  dy = y - pw[3 * p + 1];
+                  dz = z - pw[3 * p + 2];
+                  M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
+                  const bool same = spin_of[p] == spin;
+                  T lp;
+                  uee_d<T, Js>(S, r, ir, same, u1, lp);
+                  v = uee_val<T, Js>(S, r, same);
+                  on = true;
+                }
+              } else if (en_on && p < N + M_) {
+                dx = x - pnx[t];
+                dy = y - pny[t];
+                dz = z - pnz[t];
+                M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
+                tile_uen_vd<Js>(S, r, ir, pnZ[t], p - N, v, u1);
+                on = true;
+              }
+              if (on) {
+                u += v;
+                const T f = r >= T(1e-10) ? u1 * ir : T(0);
+                jx += f * dx;
+                jy += f * dy;
+                jz += f * dz;
+              }
+            }
+          };
+          // G: quantum force at r (grad ln|D| from the weighted AO
+          // gradients, the Jastrow gradient), the Langevin proposal, then at
+          // r' the ratio R = sum w chi, grad D'/D = sum w grad chi / R and the
+          // Jastrow terms; chi(r') goes to X for the update
+          T gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+          for (int j = 0; j < TILE_MAXSH; ++j)
+            if (my_sh[j] >= 0)
+              tile_shell_grad(shrec + my_sh[j] * TILE_REC, ox, oy, oz, [&](int a, T, T ddx, T ddy, T ddz) {
+                const T wq = wa[a];
+                gx += wq * ddx;
+                gy += wq * ddy;
+                gz += wq * ddz;
+              });
+          T uo, jx, jy, jz;
+          jpart(ox, oy, oz, uo, jx, jy, jz);
+          gx = rsum(gx);
+          gy = rsum(gy);
+          gz = rsum(gz);
+          uo = rsum(uo);
+          jx = rsum(jx);
+          jy = rsum(jy);
+          jz = rsum(jz);
+          const T gs = S.grad_scale;
+          T dold[3];
+          cap_drift(T(2) * (gx + gs * jx), T(2) * (gy + gs * jy), T(2) * (gz + gs * jz), tau, md, dold);
+          T z[3];
+          Rng<T>::move(philox(wid, step, (uint32_t)e, RNG_DRIFT, k0, k1), z, U);
+          nx = ox + dold[0] + s2t * z[0];
+          ny = oy + dold[1] + s2t * z[1];
+          nz = oz + dold[2] + s2t * z[2];
+          T R = T(0), hx = T(0), hy = T(0), hz = T(0);
+          T* xw = X + wt * TA.XS;
+#pragma unroll
+          for (int j = 0; j < TILE_MAXSH; ++j)
+            if (my_sh[j] >= 0)
+              tile_shell_grad(shrec + my_sh[j] * TILE_REC, nx, ny, nz, [&](int a, T v, T ddx, T ddy, T ddz) {
+                const T wq = wa[a];
+                xw[a] = v;
+                R += wq * v;
+                hx += wq * ddx;
+                hy += wq * ddy;
+                hz += wq * ddz;
+              });
+          T un, vx, vy, vz;
+          jpart(nx, ny, nz, un, vx, vy, vz);
+          R = rsum(R);
+          hx = rsum(hx);
+          hy = rsum(hy);
+          hz = rsum(hz);
+          un = rsum(un);
+          vx = rsum(vx);
+          vy = rsum(vy);
+          vz = rsum(vz);
+          const T iR = M<T>::rcp(R);
+          T dnew[3];
+          cap_drift(T(2) * (hx * iR + gs * vx), T(2) * (hy * iR + gs * vy), T(2) * (hz * iR + gs * vz), tau, md,
+                    dnew);
+          const T f0 = nx - ox - dold[0], f1 = ny - oy - dold[1], f2 = nz - oz - dold[2];
+          const T b0 = ox - nx - dnew[0], b1 = oy - ny - dnew[1], b2 = oz - nz - dnew[2];
+          const T gf = -(f0 * f0 + f1 * f1 + f2 * f2) * i4t;
+          const T gb = -(b0 * b0 + b1 * b1 + b2 * b2) * i4t;
+          Rd = R;
+          dud = un - uo;
+          pd = R * R * M<T>::ex(T(2) * dud + gb - gf);
+        } else {
+          // A: proposal
+          T z[3];
+          Rng<T>::move(philox(wid, step, (uint32_t)e, RNG_SINGLE, k0, k1), z, U);
+          nx = ox + sigma * z[0];
+          ny = oy + sigma * z[1];
+          nz = oz + sigma * z[2];
+          // B: AO values of the proposal
+#pragma unroll
+          for (int j = 0; j < TILE_MAXSH; ++j)
+            if (my_sh[j] >= 0) tile_shell_values(shrec + my_sh[j] * TILE_REC, nx, ny, nz, X + wt * TA.XS);
+        }
         __syncthreads();
         // C: Phi[half][walker][mo] = sum over the half's AOs of X[walker][ao] C[mo][ao]
         if (tb * 16 < n) {
@@ -270,49 +503,57 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
             v1 += pm * cd[1][m];
           }
         }
-        const int owner = (l & 48) | (k & 15);
-        const bool hi = k >= 16;
-        const T R = __shfl(hi ? v1 : v0, owner, 64);
-        T du = T(0);
+        T R, du, pacc;
+        if constexpr (DD) {
+          R = Rd;
+          du = dud;
+          pacc = pd;
+        } else {
+          R = __shfl(hi ? v1 : v0, owner, 64);
+          du = T(0);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int p = g + 16 * t;
-          if (p < N) {
-            if (p != e) {
-              const T px = pw[3 * p], py = pw[3 * p + 1], pz = pw[3 * p + 2];
-              const bool same = spin_of[p] == spin;
-              const T ax = nx - px, ay = ny - py, az = nz - pz;
-              const T bx = ox - px, by = oy - py, bz = oz - pz;
-              du += uee_diff<T, Js>(S, M<T>::dist(ax * ax + ay * ay + az * az),
-                                    M<T>::dist(bx * bx + by * by + bz * bz), same);
+          for (int t = 0; t < 4; ++t) {
+            const int p = g + 16 * t;
+            if (p < N) {
+              if (p != e) {
+                const T px = pw[3 * p], py = pw[3 * p + 1], pz = pw[3 * p + 2];
+                const bool same = spin_of[p] == spin;
+                const T ax = nx - px, ay = ny - py, az = nz - pz;
+                const T bx = ox - px, by = oy - py, bz = oz - pz;
+                du += uee_diff<T, Js>(S, M<T>::dist(ax * ax + ay * ay + az * az),
+                                      M<T>::dist(bx * bx + by * by + bz * bz), same);
+              }
+            } else if (en_on && p < N + M_) {
+              const T ax = nx - pnx[t], ay = ny - pny[t], az = nz - pnz[t];
+              const T bx = ox - pnx[t], by = oy - pny[t], bz = oz - pnz[t];
+              const T rn = M<T>::dist(ax * ax + ay * ay + az * az), ro = M<T>::dist(bx * bx + by * by + bz * bz);
+              if constexpr (Js::no_series)  // -Z (r' - r) / ((1+b r')(1+b r)), one reciprocal
+                du += -pnZ[t] * (rn - ro) * M<T>::rcp((T(1) + S.b_en * rn) * (T(1) + S.b_en * ro));
+              else
+                du += tile_uen<Js>(S, rn, pnZ[t], p - N) - tile_uen<Js>(S, ro, pnZ[t], p - N);
             }
-          } else if (en_on && p < N + M_) {
-            const T ax = nx - pnx[t], ay = ny - pny[t], az = nz - pnz[t];
-            const T bx = ox - pnx[t], by = oy - pny[t], bz = oz - pnz[t];
-            const T rn = M<T>::dist(ax * ax + ay * ay + az * az), ro = M<T>::dist(bx * bx + by * by + bz * bz);
-            if constexpr (Js::no_series)  // -Z (r' - r) / ((1+b r')(1+b r)), one reciprocal
-              du += -pnZ[t] * (rn - ro) * M<T>::rcp((T(1) + S.b_en * rn) * (T(1) + S.b_en * ro));
-            else
-              du += tile_uen<Js>(S, rn, pnZ[t], p - N) - tile_uen<Js>(S, ro, pnZ[t], p - N);
           }
+          du = gsum(du);
+          pacc = R * R * M<T>::ex(T(2) * du);
         }
-        du = gsum(du);
-        const T pacc = R * R * M<T>::ex(T(2) * du);
         if (U < pacc) {  // uniform within the walker's group
           const T iR = M<T>::rcp(R);
           const bool own0 = g == (k & 15) && !hi, own1 = g == (k & 15) && hi;
           const T f0 = v0 * iR, f1 = v1 * iR;
+          // the moved electron's old inverse column: staged in LDS by a
+          // drift-diffusion move, else read from its owner lane
+          const T* ck = lds + TA.o_ck + wt * TILE_CK;
           if (spin == 0) {
 #pragma unroll
             for (int m = 0; m < NU; ++m) {
-              const T c = __shfl(hi ? cu[1][m] : cu[0][m], owner, 64);
+              const T c = DD ? ck[m] : __shfl(hi ? cu[1][m] : cu[0][m], owner, 64);
               cu[0][m] = own0 ? c * iR : cu[0][m] - c * f0;
               cu[1][m] = own1 ? c * iR : cu[1][m] - c * f1;
             }
           } else {
 #pragma unroll
             for (int m = 0; m < ND; ++m) {
-              const T c = __shfl(hi ? cd[1][m] : cd[0][m], owner, 64);
+              const T c = DD ? ck[m] : __shfl(hi ? cd[1][m] : cd[0][m], owner, 64);
               cd[0][m] = own0 ? c * iR : cd[0][m] - c * f0;
               cd[1][m] = own1 ? c * iR : cd[1][m] - c * f1;
             }
@@ -330,6 +571,7 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
       }
     }
     __syncthreads();
+
     // local energy: lane g takes the electrons of sector slots g and g + 16
     double kin = 0.0, pot = 0.0;
 #pragma unroll
@@ -436,7 +678,7 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
 // kernel arguments of a tile launch: LDS carve-out (in elements of the
 // state type, of esz bytes) and lane shell lists
 template <int KS>
-inline TileArgs make_tile_args(const TilePlan& P, int esz = (int)sizeof(float)) {
+inline TileArgs make_tile_args(const TilePlan& P, int esz = (int)sizeof(float), bool dd = false) {
   TileArgs T;
   T.N = P.N;
   T.A = P.A;
@@ -452,7 +694,10 @@ inline TileArgs make_tile_args(const TilePlan& P, int esz = (int)sizeof(float)) 
   T.o_spin = T.o_phi + 2 * TILE_W * TILE_MO;
   T.o_sh = T.o_spin + ((P.N + 3) / 4 * 4 * 4 + esz - 1) / esz;  // N ints
   T.o_sum = (T.o_sh + P.n_shell * TILE_REC + per8 - 1) / per8 * per8;  // 8-byte aligned
-  T.lds_floats = T.o_sum + per8 * 3 * TILE_W;
+  T.o_cmo = T.o_sum + per8 * 3 * TILE_W;
+  T.o_ck = T.o_cmo + (dd ? 2 * P.A * TILE_CK : 0);
+  T.o_w = T.o_ck + (dd ? TILE_W * TILE_CK : 0);
+  T.lds_floats = T.o_w + (dd ? TILE_W * TILE_WS : 0);
   return T;
 }
 

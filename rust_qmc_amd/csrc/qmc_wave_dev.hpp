@@ -32,6 +32,27 @@ __device__ __forceinline__ double read_lane(double v, int l) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
                           __builtin_amdgcn_readlane(__double2loint(v), l));
 }
+// Orders one wavefront's LDS accesses across its lanes: a release fence,
+// the wavefront barrier and an acquire fence at wavefront scope, so a value
+// one lane stores is what another lane of the same wavefront loads after
+// it (the compiler may not move either access across).  No s_barrier.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum over each row of 16 lanes (all lanes active): the first four DPP
+// steps of wsum; every lane of a row gets the bit-identical row sum.
+template <class T>
+__device__ __forceinline__ T rsum(T v) {
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
+  return v;
+}
+
 template <class T>
 __device__ __forceinline__ T wsum(T v) {
   v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]

@@ -125,10 +125,10 @@ def test_dd_wave_engine_matches_register_engine(Q, oracle, name):
 
 
 def test_dd_benzene_mixed_and_ddvmc_run(Q):
-    """Mixed-precision benzene (a tile-engine system) runs its drift-
-    diffusion moves on the FP32 wave kernel; DriftDiffusionVMC::run on
-    benzene gives a finite energy with the time step inside the clamp, and
-    the mixed chain's energy agrees with the FP64 chain's."""
+    """DriftDiffusionVMC::run on benzene (drift-diffusion on the tile
+    engine in both precisions) gives a finite energy with the time step
+    inside the clamp, and the mixed chain's energy agrees with the FP64
+    chain's."""
     res = {}
     for prec in ("fp64", "mixed"):
         s = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision=prec)
@@ -141,3 +141,75 @@ def test_dd_benzene_mixed_and_ddvmc_run(Q):
     a, b = res["fp64"], res["mixed"]
     assert abs(a.energy - b.energy) < 5 * math.hypot(a.error, b.error) + 0.05
 
+
+
+def test_dd_tile_engine_benzene_fp64_vs_oracle(Q, oracle):
+    """FP64 drift-diffusion on the tile engine (16 lanes per walker: the
+    moved electron's AO weights staged in LDS, the quantum force at both
+    ends of the move, the FP64 MFMA MO contraction for the Sherman-Morrison
+    update) follows the oracle's log-domain chain: same decisions, positions
+    and E_L to 1e-9.  17 walkers make a full tile and a ragged one."""
+    g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="fp64")
+    assert g.engine_name() == "tile"
+    o = oracle.System.benzene(1.5, 2.0, 2.64, 2.05)
+    W, steps, seed, tau = 17, 2, 7, 0.02
+    ens = Q.Ensemble(g, W, seed)
+    p0 = ens.positions()
+    es, e2, acc, tr = ens.sweeps(steps, tau, Q.QMC_MOVE_DRIFT, trace=True, refresh_interval=1, max_drift=1.0)
+    pos_o, el_o, acc_o = o.dd_steps(p0, seed, 0, steps, tau, 1.0, log_domain=True)
+    assert acc_o > 0 and acc.sum() == acc_o
+    assert np.allclose(ens.positions(), pos_o, rtol=0, atol=1e-9)
+    assert _close(tr, el_o, 1e-9)
+
+
+def test_dd_tile_engine_fp64_matches_wave_engine(Q):
+    """FP64 drift-diffusion sweeps of 37 walkers (three tiles, the last
+    ragged) on the tile and the wave engine from the same seed, a refresh
+    between launches, a drift cap that binds: the same decisions,
+    positions, ln|psi| and E_L to 1e-10."""
+    out = []
+    for eng in ("tile", "wave"):
+        g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="fp64")
+        g.set_engine(eng)
+        ens = Q.Ensemble(g, 37, 9)
+        es, e2, acc, tr = ens.sweeps(3, 0.05, Q.QMC_MOVE_DRIFT, trace=True, refresh_interval=2, steps_per_launch=2,
+                                     max_drift=0.3)
+        out.append((ens.positions(), acc, tr, ens.state()))
+    (p1, a1, t1, s1), (p2, a2, t2, s2) = out
+    assert np.array_equal(a1, a2)
+    assert np.allclose(p1, p2, rtol=0, atol=1e-10)
+    assert np.allclose(t1, t2, rtol=1e-10, atol=0)
+    assert np.allclose(s1[0], s2[0], rtol=1e-11, atol=0)
+    assert np.array_equal(s1[1], s2[1])
+
+
+def _walker_dev(p1, p2, t1, t2):
+    """per-walker largest position and relative E_L deviation of two chains"""
+    dp = np.abs(np.asarray(p1) - np.asarray(p2)).reshape(len(p1), -1).max(axis=1)
+    de = (np.abs(np.asarray(t1) - np.asarray(t2)) / (np.abs(np.asarray(t2)) + 10.0)).max(axis=0)
+    return dp, de
+
+
+def test_dd_tile_engine_mixed_vs_wave_fp32_and_oracle(Q, oracle):
+    """Mixed-precision benzene drift-diffusion on the tile engine against
+    the wave engine's FP32 chain and the oracle's FP64 log-domain chain from
+    the same start: the same decisions and, to FP32 rounding, the same
+    positions and E_L walker by walker (17 walkers, 2 sweeps, 1,428 moves)."""
+    o = oracle.System.benzene(1.5, 2.0, 2.64, 2.05)
+    W, steps, seed, tau = 17, 2, 7, 0.02
+    out = []
+    for eng in ("tile", "wave"):
+        g = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision="mixed")
+        g.set_engine(eng)
+        assert g.engine_name() == eng
+        ens = Q.Ensemble(g, W, seed)
+        p0 = ens.positions()
+        es, e2, acc, tr = ens.sweeps(steps, tau, Q.QMC_MOVE_DRIFT, trace=True, refresh_interval=0, max_drift=1.0)
+        out.append((p0, ens.positions(), acc, tr))
+    (q1, p1, a1, t1), (q2, p2, a2, t2) = out
+    assert np.array_equal(q1, q2)
+    pos_o, el_o, acc_o = o.dd_steps(q1, seed, 0, steps, tau, 1.0, log_domain=True)
+    assert a1.sum() == a2.sum() == acc_o
+    for p, t in ((p2, t2), (pos_o, el_o)):
+        dp, de = _walker_dev(p1, p, t1, t)
+        assert dp.max() < 5e-4 and de.max() < 5e-4, (dp.max(), de.max())
