@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(TILE_T, TileOcc<T>::v)
 // in LDS and the quantum force at both ends of the move
 template <class T>
 struct TileDdOcc {
-  static constexpr int v = 1;
+  static constexpr int v = sizeof(T) == 4 ? 2 : 1;
 };
 
 template <int NU, int ND, int KS, class Js, class T>
