@@ -48,7 +48,7 @@ struct TileArgs {
   int lane_sh[16][TILE_MAXSH];  // shells evaluated by group lane g (-1: none)
   // LDS carve-out in elements of the state type; o_cmo, o_ck and o_w only
   // in a drift-diffusion launch
-  int o_pos, o_x, o_phi, o_spin, o_sh, o_sum, o_cmo, o_ck, o_w;
+  int o_pos, o_x, o_phi, o_spin, o_sh, o_sum, o_park, o_cmo, o_ck, o_w;
   int lds_floats;  // total carve-out, in elements of the state type
 };
 
@@ -277,14 +277,21 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
     pnz[t] = nuc ? S.nuc[I][2] : T(0);
     pnZ[t] = nuc ? S.Z[I] : T(0);
   }
-  T cu[2][NU], cd[2][ND];
+  // Inverse columns g and g + 16 of the sector that moves live in VGPRs
+  // (cur); the other sector's wait in this lane's slots of the LDS park
+  // [walker][column][row], swapped at each sector boundary.
+  constexpr int NX = NU > ND ? NU : ND;
+  T* park = lds + TA.o_park + wt * NX * NX;
+  T cur[2][NX];
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int k = g + 16 * t;
 #pragma unroll
-    for (int m = 0; m < NU; ++m) cu[t][m] = k < NU ? E.ainv[(int64_t)(m * NU + k) * E.ld + wl] : T(0);
+    for (int m = 0; m < NX; ++m) cur[t][m] = k < NU && m < NU ? E.ainv[(int64_t)(m * NU + k) * E.ld + wl] : T(0);
+    if (k < NX)
 #pragma unroll
-    for (int m = 0; m < ND; ++m) cd[t][m] = k < ND ? E.ainv[(int64_t)(NU * NU + m * ND + k) * E.ld + wl] : T(0);
+      for (int m = 0; m < NX; ++m)
+        park[k * NX + m] = k < ND && m < ND ? E.ainv[(int64_t)(NU * NU + m * ND + k) * E.ld + wl] : T(0);
   }
   // MFMA B operands: wavefront q computes MO tile q & 1 over K half q >> 1;
   // lane l holds C[mo = 16 tile + l%16][ao = (l/16) KS + half KH + s]
@@ -325,12 +332,13 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
         if constexpr (DD) {
           // W: AO weights w[a] = sum_m C[a][m] Ainv[m][k] of the moved
           // electron's inverse column, staged through the group's LDS rows
-          T* ck = lds + TA.o_ck + wt * TILE_CK;
+          // rows of TILE_CK elements from 16-byte aligned bases: vector LDS reads
+          T* ck = (T*)__builtin_assume_aligned(lds + TA.o_ck + wt * TILE_CK, 16);
           T* wa = lds + TA.o_w + wt * TILE_WS;
-          const T* cmo = lds + TA.o_cmo + spin * TA.A * TILE_CK;
+          const T* cmo = (const T*)__builtin_assume_aligned(lds + TA.o_cmo + spin * TA.A * TILE_CK, 16);
           if (g == (k & 15)) {
 #pragma unroll
-            for (int m = 0; m < n; ++m) ck[m] = spin ? (hi ? cd[1][m] : cd[0][m]) : (hi ? cu[1][m] : cu[0][m]);
+            for (int m = 0; m < n; ++m) ck[m] = hi ? cur[1][m] : cur[0][m];
           }
           wave_sync();
           {
@@ -341,7 +349,7 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
             for (int j = 0; j < TILE_AJ; ++j) {
               const int a = g + 16 * j;
               if (a < TA.A) {
-                const T* cr = cmo + a * TILE_CK;
+                const T* cr = (const T*)__builtin_assume_aligned(cmo + a * TILE_CK, 16);
                 T wq = T(0);
 #pragma unroll
                 for (int m = 0; m < n; ++m) wq += cr[m] * c[m];
@@ -470,9 +478,11 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
         if (tb * 16 < n) {
           typename MF::V d0 = {T(0), T(0), T(0), T(0)}, d1 = {T(0), T(0), T(0), T(0)};
           const T* xr = X + (l & 15) * TA.XS + (l >> 4) * KS + kh * KH;
+          // past K (kh KH + s >= KS) the B operand is zero and X is in its
+          // zero padding: no early exit, so the loop unrolls and Creg stays
+          // statically indexed
 #pragma unroll
           for (int s = 0; s < KH; ++s) {
-            if (kh * KH + s >= KS) break;
             const T b = spin ? Creg[1][s] : Creg[0][s];
             if (s & 1)
               d1 = MF::step(xr[s], b, d1);
@@ -488,20 +498,11 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
         const T* ph0 = Phi + wt * TILE_MO;
         const T* ph1 = ph0 + TILE_W * TILE_MO;
         T v0 = T(0), v1 = T(0);
-        if (spin == 0) {
 #pragma unroll
-          for (int m = 0; m < NU; ++m) {
-            const T pm = ph0[m] + ph1[m];
-            v0 += pm * cu[0][m];
-            v1 += pm * cu[1][m];
-          }
-        } else {
-#pragma unroll
-          for (int m = 0; m < ND; ++m) {
-            const T pm = ph0[m] + ph1[m];
-            v0 += pm * cd[0][m];
-            v1 += pm * cd[1][m];
-          }
+        for (int m = 0; m < n; ++m) {
+          const T pm = ph0[m] + ph1[m];
+          v0 += pm * cur[0][m];
+          v1 += pm * cur[1][m];
         }
         T R, du, pacc;
         if constexpr (DD) {
@@ -543,20 +544,11 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           // the moved electron's old inverse column: staged in LDS by a
           // drift-diffusion move, else read from its owner lane
           const T* ck = lds + TA.o_ck + wt * TILE_CK;
-          if (spin == 0) {
 #pragma unroll
-            for (int m = 0; m < NU; ++m) {
-              const T c = DD ? ck[m] : __shfl(hi ? cu[1][m] : cu[0][m], owner, 64);
-              cu[0][m] = own0 ? c * iR : cu[0][m] - c * f0;
-              cu[1][m] = own1 ? c * iR : cu[1][m] - c * f1;
-            }
-          } else {
-#pragma unroll
-            for (int m = 0; m < ND; ++m) {
-              const T c = DD ? ck[m] : __shfl(hi ? cd[1][m] : cd[0][m], owner, 64);
-              cd[0][m] = own0 ? c * iR : cd[0][m] - c * f0;
-              cd[1][m] = own1 ? c * iR : cd[1][m] - c * f1;
-            }
+          for (int m = 0; m < n; ++m) {
+            const T c = DD ? ck[m] : __shfl(hi ? cur[1][m] : cur[0][m], owner, 64);
+            cur[0][m] = own0 ? c * iR : cur[0][m] - c * f0;
+            cur[1][m] = own1 ? c * iR : cur[1][m] - c * f1;
           }
           if (g == 0) {
             pw[3 * e] = nx;
@@ -567,6 +559,17 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           if (R < T(0)) sgn = -sgn;
           ju += (double)du;
           ++acc;
+        }
+      }
+      // sector boundary: the moved sector's columns to the park, the other's back
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int k = g + 16 * t;
+#pragma unroll
+        for (int m = 0; m < NX; ++m) {
+          const T o = k < NX ? park[k * NX + m] : T(0);
+          if (k < NX) park[k * NX + m] = cur[t][m];
+          cur[t][m] = o;
         }
       }
     }
@@ -583,15 +586,14 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
         if (k >= n) continue;
         const int i = S.sector_elec[spin][k];
         const T x = pw[3 * i], y = pw[3 * i + 1], z = pw[3 * i + 2];
+        // after both sectors moved, spin 0's columns are in cur, spin 1's parked
+        T col[NX];
+#pragma unroll
+        for (int m = 0; m < NX; ++m) col[m] = spin ? park[k * NX + m] : cur[t][m];
         auto wgt = [&](int ao) {
           T s = T(0);
-          if (spin == 0) {
 #pragma unroll
-            for (int m = 0; m < NU; ++m) s += S.moT[0][ao][m] * cu[t][m];
-          } else {
-#pragma unroll
-            for (int m = 0; m < ND; ++m) s += S.moT[1][ao][m] * cd[t][m];
-          }
+          for (int m = 0; m < n; ++m) s += S.moT[spin][ao][m] * col[m];
           return s;
         };
         T gdx, gdy, gdz, ld;
@@ -661,10 +663,10 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
       const int k = g + 16 * t;
       if (k < NU)
 #pragma unroll
-        for (int m = 0; m < NU; ++m) E.ainv[(int64_t)(m * NU + k) * E.ld + w] = cu[t][m];
+        for (int m = 0; m < NU; ++m) E.ainv[(int64_t)(m * NU + k) * E.ld + w] = cur[t][m];
       if (k < ND)
 #pragma unroll
-        for (int m = 0; m < ND; ++m) E.ainv[(int64_t)(NU * NU + m * ND + k) * E.ld + w] = cd[t][m];
+        for (int m = 0; m < ND; ++m) E.ainv[(int64_t)(NU * NU + m * ND + k) * E.ld + w] = park[k * NX + m];
     }
     if (g == 0) {
       E.logdet[w] = logdet;
@@ -694,7 +696,11 @@ inline TileArgs make_tile_args(const TilePlan& P, int esz = (int)sizeof(float), 
   T.o_spin = T.o_phi + 2 * TILE_W * TILE_MO;
   T.o_sh = T.o_spin + ((P.N + 3) / 4 * 4 * 4 + esz - 1) / esz;  // N ints
   T.o_sum = (T.o_sh + P.n_shell * TILE_REC + per8 - 1) / per8 * per8;  // 8-byte aligned
-  T.o_cmo = T.o_sum + per8 * 3 * TILE_W;
+  const int nx = P.nu > P.nd ? P.nu : P.nd;
+  const int per16 = 16 / esz;  // the park and the drift-diffusion rows start 16-byte aligned
+  auto al16 = [&](int o) { return (o + per16 - 1) / per16 * per16; };
+  T.o_park = al16(T.o_sum + per8 * 3 * TILE_W);
+  T.o_cmo = al16(T.o_park + TILE_W * nx * nx);
   T.o_ck = T.o_cmo + (dd ? 2 * P.A * TILE_CK : 0);
   T.o_w = T.o_ck + (dd ? TILE_W * TILE_CK : 0);
   T.lds_floats = T.o_w + (dd ? TILE_W * TILE_WS : 0);
