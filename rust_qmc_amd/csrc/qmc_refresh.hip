@@ -3,24 +3,29 @@
 // workgroup (four wavefronts) per tile, 16 lanes per walker, in two
 // kernels.
 //
-// tile_slater_kernel, one workgroup per (tile, spin sector), electron slot
-// by electron slot:
-//   A  AO values of the slot's electron for the 16 walkers, in FP64, the
-//      shells spread over each walker's 16 lanes (the tile plan's lists),
-//      into the LDS matrix X[walker][ao]
-//   B  the slot's Slater-matrix row of every walker, Phi = X C^T, on the FP64
-//      matrix cores (v_mfma_f64_16x16x4f64): wavefront q takes MO tile q & 1
-//      over K half q >> 1, the MO coefficients held as the B operand
-//   C  the rows to a work buffer in HBM, walker-major.
+// tile_slater_kernel, resident workgroups looping over units (tile, spin
+// sector, batch of four electron slots); each wavefront works alone on its
+// four walkers, synchronised within itself only:
+//   A  AO values of the batch's four electrons of each walker, in FP64, the
+//      shells spread over the walker's 16 lanes (the tile plan's lists),
+//      into the wavefront's LDS matrix X[4 walkers x 4 slots][ao]
+//   B  the 16 Slater-matrix rows, Phi = X C^T, on the FP64 matrix cores
+//      (v_mfma_f64_16x16x4f64), both MO tiles over the full AO range, the MO
+//      coefficients held as the B operand
+//   C  the rows from the accumulators to a work buffer in HBM, walker-major.
 // tile_invert_kernel, one workgroup per tile: lane g of a walker's group
 // loads rows g and g + 16 into VGPRs, then Gauss-Jordan with partial
 // pivoting in place on the register rows, the inverse written with the
-// pivots' relabelling undone; ln J summed over the electron pairs and
-// electron-nucleus pairs spread on the 16 lanes.  Two kernels instead of
-// one, so each holds fewer registers and more waves per SIMD hide the FP64
-// latencies.  The arithmetic is the wave engine's build (wave_sector_inverse,
-// wave_jastrow_u) in another summation and pivoting order.
+// pivots' relabelling undone.
+// tile_lnj_kernel, one wavefront per walker: ln J over the electron pairs
+// and electron-nucleus pairs, balanced over the lanes.
+// Three kernels instead of one, so each holds fewer registers and more
+// waves per SIMD hide the FP64 latencies.  The arithmetic is the wave
+// engine's build (wave_sector_inverse, wave_jastrow_u) in another summation
+// and pivoting order.
 #include "qmc_refresh.hpp"
+
+#include <algorithm>
 
 #include "qmc_tile_dev.hpp"
 #include "qmc_wave_dev.hpp"
@@ -29,16 +34,14 @@ namespace qmc {
 
 typedef double rdbl4 __attribute__((ext_vector_type(4)));
 
-constexpr int RF_REC = 32;  // doubles per shell record
-constexpr int RF_MO = 32;   // MO columns of Phi (two 16-wide MFMA tiles)
+constexpr int RF_REC = 33;  // doubles per shell record: 32 used, odd stride against bank conflicts
 
 struct RefreshArgs {
   int N, A, n_nuc, n_shell;
   int lane_sh[16][TILE_MAXSH];  // shells evaluated by group lane g (-1: none)
-  // LDS carve-outs in doubles: positions first in both kernels, then the
-  // Slater kernel's X, Phi and shell records, or the inversion's pivot rows
-  // and pivot lists
-  int o_pos, o_x, o_phi, o_sh, o_prow, o_piv;
+  // LDS carve-outs in doubles: the Slater kernel's four X matrices and the
+  // shell records; the inversion's pivot rows and pivot lists
+  int o_x, o_sh, o_prow, o_piv;
   int lds_slater, lds_invert;
   double* work;  // [W][2][NS][NS] Slater rows, walker-major
 };
@@ -109,73 +112,84 @@ __device__ __forceinline__ void rf_shell_values(const double* rec, double x, dou
 }
 
 template <int NS, int KS, class T>
-__global__ void __launch_bounds__(TILE_T) tile_slater_kernel(const DevSys<double>* __restrict__ Sp, EnsView<T> E,
-                                                             RefreshArgs RA) {
+__global__ void __launch_bounds__(TILE_T, 3) tile_slater_kernel(const DevSys<double>* __restrict__ Sp, EnsView<T> E,
+                                                             RefreshArgs RA, int n_units) {
   extern __shared__ double lds_d[];
-  constexpr int KH = (KS + 1) / 2;  // MFMA steps per K half
-  constexpr int XS = 4 * KS + 1;    // X row stride
+  constexpr int XS = 4 * KS + 1;           // X row stride
+  constexpr int NB = (NS + 3) / 4;         // batches of four electron slots
   const DevSys<double>& S = *Sp;
-  const int tid = threadIdx.x, spin = blockIdx.y;
-  const int q = tid >> 6, l = tid & 63;   // wavefront, lane
-  const int wt = tid >> 4, g = tid & 15;  // walker in the tile, lane in its group
-  const int64_t w0 = (int64_t)blockIdx.x * TILE_W;
-  const int64_t w = w0 + wt;
-  const int64_t wl = w < E.W ? w : E.W - 1;  // dead walkers shadow the last one
-  const int N = RA.N;
-  double* pos = lds_d + RA.o_pos;  // [TILE_W][3N]
-  double* X = lds_d + RA.o_x;      // [TILE_W][XS]
-  double* Phi = lds_d + RA.o_phi;  // [2 K halves][TILE_W][RF_MO]
-  double* shrec = lds_d + RA.o_sh; // [n_shell][RF_REC]
-  double* pw = pos + wt * 3 * N;
-  for (int i = g; i < 3 * N; i += 16) pw[i] = (double)E.pos[(int64_t)i * E.ld + wl];
-  for (int i = g; i < XS; i += 16) X[wt * XS + i] = 0.0;
+  const int tid = threadIdx.x;
+  const int q = tid >> 6, l = tid & 63;    // wavefront, lane
+  const int ww = l >> 4, g = l & 15;       // walker of the wavefront, lane in its group
+  double* X = lds_d + RA.o_x + q * 16 * XS;  // this wavefront's [16 rows][XS], row 4 ww + slot
+  double* shrec = lds_d + RA.o_sh;           // [n_shell][RF_REC]
   for (int i = tid; i < RA.n_shell; i += TILE_T) rf_write_record(S, i, shrec + i * RF_REC);
+  for (int i = l; i < 16 * XS; i += 64) X[i] = 0.0;
   int my_sh[TILE_MAXSH];
 #pragma unroll
   for (int j = 0; j < TILE_MAXSH; ++j) my_sh[j] = RA.lane_sh[g][j];
-  const int tb = q & 1, kh = q >> 1;
-  // B operand: lane l holds C[mo = 16 tb + l % 16][ao = (l / 16) KS + kh KH + s]
-  double Creg[KH];
-#pragma unroll
-  for (int s = 0; s < KH; ++s) {
-    const int kk = kh * KH + s;
-    const int ao = (l >> 4) * KS + kk, mo = tb * 16 + (l & 15);
-    Creg[s] = (kk < KS && ao < RA.A && mo < NS) ? S.moT[spin][ao][mo] : 0.0;
-  }
   __syncthreads();
-#pragma unroll 1
-  for (int k = 0; k < NS; ++k) {
-    const int e = S.sector_elec[spin][k];
-    const double x = pw[3 * e], y = pw[3 * e + 1], z = pw[3 * e + 2];
+  // units (tile, spin, batch), spread over the resident workgroups
+  for (int u = blockIdx.x; u < n_units; u += gridDim.x) {
+    const int b = u % NB, spin = (u / NB) & 1, tile = u / (2 * NB);
+    const int64_t w = (int64_t)tile * TILE_W + 4 * q + ww;
+    const int64_t wl = w < E.W ? w : E.W - 1;  // dead walkers shadow the last one
+    // B operand: lane l holds C[ao = (l / 16) KS + s][mo = 16 t + l % 16]
+    double Creg[2][KS];
 #pragma unroll
-    for (int j = 0; j < TILE_MAXSH; ++j)
-      if (my_sh[j] >= 0) rf_shell_values(shrec + my_sh[j] * RF_REC, x, y, z, X + wt * XS);
-    __syncthreads();
-    {
-      rdbl4 acc = {0.0, 0.0, 0.0, 0.0};
-      const double* xr = X + (l & 15) * XS + (l >> 4) * KS + kh * KH;
-      // past K (the second half's last step when KS is odd) Creg is zero and
-      // X stays inside its padded row
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int s = 0; s < KH; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[s], Creg[s], acc, 0, 0, 0);
-      // register r of lane l holds D[4 r + l / 16][l % 16]
-      double* ph = Phi + kh * TILE_W * RF_MO;
+      for (int s2 = 0; s2 < KS; ++s2) {
+        const int ao = (l >> 4) * KS + s2, mo = t * 16 + g;
+        Creg[t][s2] = (ao < RA.A && mo < NS) ? S.moT[spin][ao][mo] : 0.0;
+      }
+    // A: AO values of electron slots 4 b .. 4 b + 3 of the group's walker,
+    // the shells spread over its 16 lanes, into rows 4 ww + i
+    wave_sync();  // the previous unit's reads of X are done
 #pragma unroll
-      for (int r = 0; r < 4; ++r) ph[(4 * r + (l >> 4)) * RF_MO + tb * 16 + (l & 15)] = acc[r];
+    for (int i = 0; i < 4; ++i) {
+      const int k = 4 * b + i;
+      if (k < NS) {
+        const int e = S.sector_elec[spin][k];
+        const double x = (double)E.pos[(int64_t)(3 * e) * E.ld + wl];
+        const double y = (double)E.pos[(int64_t)(3 * e + 1) * E.ld + wl];
+        const double z = (double)E.pos[(int64_t)(3 * e + 2) * E.ld + wl];
+#pragma unroll
+        for (int j = 0; j < TILE_MAXSH; ++j)
+          if (my_sh[j] >= 0) rf_shell_values(shrec + my_sh[j] * RF_REC, x, y, z, X + (4 * ww + i) * XS);
+      }
     }
-    __syncthreads();
-    // row k of the 16 walkers: a[k][m] = phi_m(r_k), the two K halves summed
-    for (int i = tid; i < TILE_W * NS; i += TILE_T) {
-      const int ww = i / NS, m = i - ww * NS;
-      if (w0 + ww < E.W)
-        RA.work[((w0 + ww) * 2 + spin) * NS * NS + k * NS + m] =
-            Phi[ww * RF_MO + m] + Phi[TILE_W * RF_MO + ww * RF_MO + m];
+    wave_sync();
+    // B: Phi[row][mo] = sum_ao X[row][ao] C[mo][ao] on the FP64 matrix cores,
+    // two MO tiles, each over two interleaved accumulators
+    rdbl4 acc[2][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) acc[t][0] = acc[t][1] = rdbl4{0.0, 0.0, 0.0, 0.0};
+    const double* xr = X + g * XS + (l >> 4) * KS;
+#pragma unroll
+    for (int s2 = 0; s2 < KS; ++s2) {
+      const double a = xr[s2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) acc[t][s2 & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Creg[t][s2], acc[t][s2 & 1], 0, 0, 0);
+    }
+    // C: register r of lane l holds Phi[4 r + l / 16][16 t + l % 16]: walker
+    // 4 q + r of the tile, electron slot 4 b + l / 16
+    const int k = 4 * b + (l >> 4);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t wr = (int64_t)tile * TILE_W + 4 * q + r;
+      if (wr < E.W && k < NS)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const int m = 16 * t + g;
+          if (m < NS) RA.work[(wr * 2 + spin) * NS * NS + k * NS + m] = acc[t][0][r] + acc[t][1][r];
+        }
     }
   }
 }
 
 template <int NS, class T>
-__global__ void __launch_bounds__(TILE_T) tile_invert_kernel(const DevSys<double>* __restrict__ Sp, EnsView<T> E,
+__global__ void __launch_bounds__(TILE_T, 2) tile_invert_kernel(const DevSys<double>* __restrict__ Sp, EnsView<T> E,
                                                              RefreshArgs RA, int* singular_count) {
   extern __shared__ double lds_d[];
   const DevSys<double>& S = *Sp;
@@ -184,12 +198,8 @@ __global__ void __launch_bounds__(TILE_T) tile_invert_kernel(const DevSys<double
   const int64_t w = (int64_t)blockIdx.x * TILE_W + wt;
   const bool live = w < E.W;
   const int64_t wl = live ? w : E.W - 1;  // dead walkers shadow the last one and write nothing
-  const int N = RA.N;
-  double* pos = lds_d + RA.o_pos;    // [TILE_W][3N]
   double* prow = lds_d + RA.o_prow;  // [TILE_W][NS] pivot rows
   int* piv = (int*)(lds_d + RA.o_piv);  // [TILE_W][NS + 1] pivots and sign, then [TILE_W][2 NS] kinv and marks
-  double* pw = pos + wt * 3 * N;
-  for (int i = g; i < 3 * N; i += 16) pw[i] = (double)E.pos[(int64_t)i * E.ld + wl];
   int* pv_of = piv + wt * (NS + 1);                  // pivots p_c, then the permutation's sign
   int* dst = piv + TILE_W * (NS + 1) + wt * 2 * NS;  // kinv[r], then cycle marks
   double ltot = 0.0, stot = 1.0;
@@ -329,29 +339,52 @@ __global__ void __launch_bounds__(TILE_T) tile_invert_kernel(const DevSys<double
     sing_any = sing_any || sing;
     __syncthreads();  // dst and the LDS buffers are reused by the next sector
   }
-  // ln J: electron pairs (i, j > i) and electron-nucleus pairs of the
-  // electrons g, g + 16, g + 32
-  double u = 0.0;
-  for (int i = g; i < N; i += 16) {
-    const double xi = pw[3 * i], yi = pw[3 * i + 1], zi = pw[3 * i + 2];
-    for (int j = i + 1; j < N; ++j) {
-      const double dx = xi - pw[3 * j], dy = yi - pw[3 * j + 1], dz = zi - pw[3 * j + 2];
-      const bool same = S.elec_spin[i] == S.elec_spin[j];
-      u += uee_val(S, sqrt(dx * dx + dy * dy + dz * dz), same) + S.cst_ee;
-    }
-    if (S.en_on)
-      for (int I = 0; I < RA.n_nuc; ++I) {
-        const double dx = xi - S.nuc[I][0], dy = yi - S.nuc[I][1], dz = zi - S.nuc[I][2];
-        u += uen_val(S, sqrt(dx * dx + dy * dy + dz * dz), I);
-      }
-  }
-  u = gsum(u);
   if (live && g == 0) {
     E.logdet[w] = ltot;
     E.sign[w] = stot;
-    E.ju[w] = u;
     if (sing_any && singular_count) atomicAdd(singular_count, 1);
   }
+}
+
+// ln J of every walker from its positions (wave_jastrow_u's terms), one
+// wavefront per walker: the N (N - 1) / 2 electron pairs (i, j > i) and the
+// N M electron-nucleus pairs spread evenly over the lanes.  Electron rows i
+// and N - 1 - i are folded together (N - 1 pair terms per folded row), and
+// each folded row is split over P = 64 / ceil(N / 2) lanes.
+template <class T>
+__global__ void __launch_bounds__(TILE_T) tile_lnj_kernel(const DevSys<double>* __restrict__ Sp, EnsView<T> E,
+                                                          int N) {
+  extern __shared__ double lds_d[];
+  const DevSys<double>& S = *Sp;
+  const int q = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * 4 + q;
+  const int64_t wl = w < E.W ? w : E.W - 1;  // dead walkers shadow the last one and write nothing
+  double* pw = lds_d + q * 3 * N;
+  for (int i = l; i < 3 * N; i += 64) pw[i] = (double)E.pos[(int64_t)i * E.ld + wl];
+  wave_sync();
+  const int F = (N + 1) / 2, P = 64 / F;
+  const int f = l / P, part = l - f * P;
+  double u = 0.0;
+  if (f < F) {
+    const int i1 = f, i2 = N - 1 - f;
+    const int n1 = N - 1 - i1, n2 = i2 != i1 ? N - 1 - i2 : 0;
+    for (int t = part; t < n1 + n2; t += P) {
+      const int i = t < n1 ? i1 : i2, j = t < n1 ? i1 + 1 + t : i2 + 1 + (t - n1);
+      const double dx = pw[3 * i] - pw[3 * j], dy = pw[3 * i + 1] - pw[3 * j + 1], dz = pw[3 * i + 2] - pw[3 * j + 2];
+      const bool same = S.elec_spin[i] == S.elec_spin[j];
+      u += uee_val(S, sqrt(dx * dx + dy * dy + dz * dz), same) + S.cst_ee;
+    }
+    if (S.en_on) {
+      const int M = S.n_nuc, ne = (i2 != i1 ? 2 : 1) * M;
+      for (int t = part; t < ne; t += P) {
+        const int i = t < M ? i1 : i2, I = t < M ? t : t - M;
+        const double dx = pw[3 * i] - S.nuc[I][0], dy = pw[3 * i + 1] - S.nuc[I][1], dz = pw[3 * i + 2] - S.nuc[I][2];
+        u += uen_val(S, sqrt(dx * dx + dy * dy + dz * dz), I);
+      }
+    }
+  }
+  u = wsum(u);
+  if (l == 0 && w < E.W) E.ju[w] = u;
 }
 
 static RefreshArgs make_refresh_args(const TilePlan& P, int NS, int KS, double* work) {
@@ -362,12 +395,10 @@ static RefreshArgs make_refresh_args(const TilePlan& P, int NS, int KS, double* 
   R.n_shell = P.n_shell;
   for (int g = 0; g < 16; ++g)
     for (int j = 0; j < TILE_MAXSH; ++j) R.lane_sh[g][j] = P.lane_sh[g][j];
-  R.o_pos = 0;
-  R.o_x = R.o_pos + TILE_W * 3 * P.N;
-  R.o_phi = R.o_x + TILE_W * (4 * KS + 1);
-  R.o_sh = R.o_phi + 2 * TILE_W * RF_MO;
+  R.o_x = 0;  // Slater kernel: four wavefronts' X, then the shell records
+  R.o_sh = R.o_x + 4 * 16 * (4 * KS + 1);
   R.lds_slater = R.o_sh + P.n_shell * RF_REC;
-  R.o_prow = R.o_pos + TILE_W * 3 * P.N;
+  R.o_prow = 0;
   R.o_piv = R.o_prow + TILE_W * NS;
   R.lds_invert = R.o_piv + (TILE_W * (NS + 1) + TILE_W * 2 * NS + 1) / 2;
   R.work = work;
@@ -390,8 +421,18 @@ hipError_t tile_refresh(const void* Sd, EnsView<T> E, const TilePlan& P, double*
   hipError_t e = hipFuncSetAttribute((const void*)ka, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ba);
   if (e == hipSuccess) e = hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bb);
   if (e != hipSuccess) return e;
-  ka<<<dim3(tiles, 2), TILE_T, ba, st>>>((const DevSys<double>*)Sd, E, R);
+  // the Slater kernel's units over as many workgroups as are resident at once
+  const int n_units = (int)tiles * 2 * ((NS + 3) / 4);
+  int dev = 0, n_cu = 0, occ = 0;
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)ka, TILE_T, ba);
+  if (e != hipSuccess) return e;
+  const int grid = std::max(1, std::min(n_units, n_cu * std::max(occ, 1)));
+  ka<<<grid, TILE_T, ba, st>>>((const DevSys<double>*)Sd, E, R, n_units);
   kb<<<tiles, TILE_T, bb, st>>>((const DevSys<double>*)Sd, E, R, singular_count);
+  tile_lnj_kernel<T><<<(unsigned)((E.W + 3) / 4), TILE_T, sizeof(double) * 4 * 3 * P.N, st>>>(
+      (const DevSys<double>*)Sd, E, P.N);
   return hipGetLastError();
 }
 

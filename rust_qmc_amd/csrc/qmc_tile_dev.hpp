@@ -37,9 +37,19 @@ struct TileMfma<double> {
 
 constexpr int TILE_T = 256;   // threads per tile
 constexpr int TILE_MO = 32;   // MO columns of Phi (two 16-wide MFMA tiles)
+// Row stride of Phi in LDS: a walker's row is read by its 16 lanes at one
+// address (a broadcast), the four walkers of a wavefront at 36-element
+// offsets that fall in different banks; a stride of 32 put all four in the
+// same bank.
+constexpr int TILE_PS = 36;
 constexpr int TILE_MAXP = 6;  // primitives per shell
-constexpr int TILE_REC = 32;  // elements per shell record in LDS
-constexpr int TILE_CK = 24;   // row stride of an inverse column or MO-coefficient row in LDS
+// elements per shell record in LDS: 32 used, an odd stride so that the 16
+// lanes of a group, reading the same field of 16 different records, hit 16
+// different banks (a stride of 32 elements put them all in one bank)
+constexpr int TILE_REC = 33;
+// row stride of an inverse column or MO-coefficient row in LDS: odd, so
+// the 16 lanes of a group reading rows a = g + 16 j hit 16 different banks
+constexpr int TILE_CK = 25;
 constexpr int TILE_AJ = 5;    // AOs per lane in the AO-weight pass: a = g + 16 j, j < 5
 constexpr int TILE_WS = 16 * TILE_AJ;  // row stride of a walker's AO weights
 
@@ -243,7 +253,7 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   const int N = TA.N, M_ = TA.n_nuc;
   T* pos = lds + TA.o_pos;  // [TILE_W][3N]
   T* X = lds + TA.o_x;      // [TILE_W][XS], AO values of the proposals
-  T* Phi = lds + TA.o_phi;  // [2 K halves][TILE_W][TILE_MO]
+  T* Phi = lds + TA.o_phi;  // [2 K halves][TILE_W][TILE_PS]
   int* spin_of = (int*)(lds + TA.o_spin);
   T* shrec = lds + TA.o_sh;  // [n_shell][TILE_REC]
   double* sums = (double*)(lds + TA.o_sum);  // [TILE_W][3], per-step walker sums
@@ -332,10 +342,9 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
         if constexpr (DD) {
           // W: AO weights w[a] = sum_m C[a][m] Ainv[m][k] of the moved
           // electron's inverse column, staged through the group's LDS rows
-          // rows of TILE_CK elements from 16-byte aligned bases: vector LDS reads
-          T* ck = (T*)__builtin_assume_aligned(lds + TA.o_ck + wt * TILE_CK, 16);
+          T* ck = lds + TA.o_ck + wt * TILE_CK;
           T* wa = lds + TA.o_w + wt * TILE_WS;
-          const T* cmo = (const T*)__builtin_assume_aligned(lds + TA.o_cmo + spin * TA.A * TILE_CK, 16);
+          const T* cmo = lds + TA.o_cmo + spin * TA.A * TILE_CK;
           if (g == (k & 15)) {
 #pragma unroll
             for (int m = 0; m < n; ++m) ck[m] = hi ? cur[1][m] : cur[0][m];
@@ -349,7 +358,7 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
             for (int j = 0; j < TILE_AJ; ++j) {
               const int a = g + 16 * j;
               if (a < TA.A) {
-                const T* cr = (const T*)__builtin_assume_aligned(cmo + a * TILE_CK, 16);
+                const T* cr = cmo + a * TILE_CK;
                 T wq = T(0);
 #pragma unroll
                 for (int m = 0; m < n; ++m) wq += cr[m] * c[m];
@@ -489,14 +498,14 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
             else
               d0 = MF::step(xr[s], b, d0);
           }
-          T* ph = Phi + kh * TILE_W * TILE_MO;
+          T* ph = Phi + kh * TILE_W * TILE_PS;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) ph[MF::row(l, r) * TILE_MO + tb * 16 + (l & 15)] = d0[r] + d1[r];
+          for (int r = 0; r < 4; ++r) ph[MF::row(l, r) * TILE_PS + tb * 16 + (l & 15)] = d0[r] + d1[r];
         }
         __syncthreads();
         // D: ratio, Jastrow difference, Metropolis test, Sherman-Morrison
-        const T* ph0 = Phi + wt * TILE_MO;
-        const T* ph1 = ph0 + TILE_W * TILE_MO;
+        const T* ph0 = Phi + wt * TILE_PS;
+        const T* ph1 = ph0 + TILE_W * TILE_PS;
         T v0 = T(0), v1 = T(0);
 #pragma unroll
         for (int m = 0; m < n; ++m) {
@@ -693,7 +702,7 @@ inline TileArgs make_tile_args(const TilePlan& P, int esz = (int)sizeof(float), 
   T.o_pos = 0;
   T.o_x = T.o_pos + TILE_W * 3 * P.N;
   T.o_phi = T.o_x + TILE_W * T.XS;
-  T.o_spin = T.o_phi + 2 * TILE_W * TILE_MO;
+  T.o_spin = T.o_phi + 2 * TILE_W * TILE_PS;
   T.o_sh = T.o_spin + ((P.N + 3) / 4 * 4 * 4 + esz - 1) / esz;  // N ints
   T.o_sum = (T.o_sh + P.n_shell * TILE_REC + per8 - 1) / per8 * per8;  // 8-byte aligned
   const int nx = P.nu > P.nd ? P.nu : P.nd;
