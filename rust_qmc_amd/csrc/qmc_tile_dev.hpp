@@ -58,7 +58,7 @@ struct TileArgs {
   int lane_sh[16][TILE_MAXSH];  // shells evaluated by group lane g (-1: none)
   // LDS carve-out in elements of the state type; o_cmo, o_ck and o_w only
   // in a drift-diffusion launch
-  int o_pos, o_x, o_phi, o_spin, o_sh, o_sum, o_park, o_cmo, o_ck, o_w;
+  int o_pos, o_x, o_phi, o_spin, o_sh, o_sum, o_park, o_rng, o_cmo, o_ck, o_w;
   int lds_floats;  // total carve-out, in elements of the state type
 };
 
@@ -331,6 +331,25 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
 #pragma unroll
     for (int spin = 0; spin < 2; ++spin) {
       const int n = spin ? ND : NU;
+      // the sector's random numbers (three normals and the acceptance
+      // uniform of each move) drawn ahead, lane g for slots g and g + 16,
+      // into the walker's LDS rows: two Philox blocks and Box-Muller
+      // transforms per lane instead of one per move on every lane
+      T* rg = lds + TA.o_rng + wt * 4 * NX;
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int k = g + 16 * t;
+        if (k < n) {
+          T z[3], u;
+          Rng<T>::move(philox(wid, step, (uint32_t)S.sector_elec[spin][k], DD ? RNG_DRIFT : RNG_SINGLE, k0, k1), z,
+                       u);
+          rg[4 * k] = z[0];
+          rg[4 * k + 1] = z[1];
+          rg[4 * k + 2] = z[2];
+          rg[4 * k + 3] = u;
+        }
+      }
+      wave_sync();
 #pragma unroll 1
       for (int k = 0; k < n; ++k) {
         const int e = S.sector_elec[spin][k];
@@ -431,8 +450,8 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           const T gs = S.grad_scale;
           T dold[3];
           cap_drift(T(2) * (gx + gs * jx), T(2) * (gy + gs * jy), T(2) * (gz + gs * jz), tau, md, dold);
-          T z[3];
-          Rng<T>::move(philox(wid, step, (uint32_t)e, RNG_DRIFT, k0, k1), z, U);
+          const T z[3] = {rg[4 * k], rg[4 * k + 1], rg[4 * k + 2]};
+          U = rg[4 * k + 3];
           nx = ox + dold[0] + s2t * z[0];
           ny = oy + dold[1] + s2t * z[1];
           nz = oz + dold[2] + s2t * z[2];
@@ -472,8 +491,8 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           pd = R * R * M<T>::ex(T(2) * dud + gb - gf);
         } else {
           // A: proposal
-          T z[3];
-          Rng<T>::move(philox(wid, step, (uint32_t)e, RNG_SINGLE, k0, k1), z, U);
+          const T z[3] = {rg[4 * k], rg[4 * k + 1], rg[4 * k + 2]};
+          U = rg[4 * k + 3];
           nx = ox + sigma * z[0];
           ny = oy + sigma * z[1];
           nz = oz + sigma * z[2];
@@ -709,7 +728,8 @@ inline TileArgs make_tile_args(const TilePlan& P, int esz = (int)sizeof(float), 
   const int per16 = 16 / esz;  // the park and the drift-diffusion rows start 16-byte aligned
   auto al16 = [&](int o) { return (o + per16 - 1) / per16 * per16; };
   T.o_park = al16(T.o_sum + per8 * 3 * TILE_W);
-  T.o_cmo = al16(T.o_park + TILE_W * nx * nx);
+  T.o_rng = al16(T.o_park + TILE_W * nx * nx);
+  T.o_cmo = al16(T.o_rng + TILE_W * 4 * nx);
   T.o_ck = T.o_cmo + (dd ? 2 * P.A * TILE_CK : 0);
   T.o_w = T.o_ck + (dd ? TILE_W * TILE_CK : 0);
   T.lds_floats = T.o_w + (dd ? TILE_W * TILE_WS : 0);
