@@ -40,8 +40,9 @@ struct RefreshArgs {
   int N, A, n_nuc, n_shell;
   int lane_sh[16][TILE_MAXSH];  // shells evaluated by group lane g (-1: none)
   // LDS carve-outs in doubles: the Slater kernel's four X matrices and the
-  // shell records; the inversion's pivot rows and pivot lists
-  int o_x, o_sh, o_prow, o_piv;
+  // shell records; the inversion's pivot rows, pivot lists and the stage of
+  // its output
+  int o_x, o_sh, o_prow, o_piv, o_stg;
   int lds_slater, lds_invert;
   double* work;  // [W][2][NS][NS] Slater rows, walker-major
 };
@@ -318,20 +319,29 @@ __global__ void __launch_bounds__(TILE_T, 2) tile_invert_kernel(const DevSys<dou
       lsec = -INFINITY;
       ssec = 0.0;
     }
-    if (live) {
-      const int64_t base = spin ? (int64_t)NS * NS : 0;
+    // the inverse staged element-major in LDS (a stride of TILE_W + 1 keeps
+    // the group's 16 rows in different banks), then written out with the
+    // tile's walkers in consecutive lanes: walker-contiguous HBM stores
+    T* stg = (T*)(lds_d + RA.o_stg);  // [NS * NS][TILE_W + 1]
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int r = g + 16 * t;  // storage row r holds inverse row kinv[r]
-        if (r >= NS) continue;
-        if (!sing) {
-          const int m = dst[r];
+    for (int t = 0; t < 2; ++t) {
+      const int r = g + 16 * t;  // storage row r holds inverse row kinv[r]
+      if (r >= NS) continue;
+      if (!sing) {
+        const int m = dst[r];
 #pragma unroll
-          for (int l = 0; l < NS; ++l) E.ainv[(base + m * NS + pv_of[l]) * E.ld + w] = (T)(t ? r1[l] : r0[l]);
-        } else {  // the identity, as the wave engine's build leaves it
+        for (int l = 0; l < NS; ++l) stg[(m * NS + pv_of[l]) * (TILE_W + 1) + wt] = (T)(t ? r1[l] : r0[l]);
+      } else {  // the identity, as the wave engine's build leaves it
 #pragma unroll
-          for (int k = 0; k < NS; ++k) E.ainv[(base + r * NS + k) * E.ld + w] = (T)(r == k ? 1.0 : 0.0);
-        }
+        for (int k = 0; k < NS; ++k) stg[(r * NS + k) * (TILE_W + 1) + wt] = (T)(r == k ? 1.0 : 0.0);
+      }
+    }
+    __syncthreads();
+    {
+      const int64_t base = spin ? (int64_t)NS * NS : 0, w0 = (int64_t)blockIdx.x * TILE_W;
+      for (int i = tid; i < NS * NS * TILE_W; i += TILE_T) {
+        const int el = i / TILE_W, ww = i - el * TILE_W;
+        if (w0 + ww < E.W) E.ainv[(base + el) * E.ld + w0 + ww] = stg[el * (TILE_W + 1) + ww];
       }
     }
     ltot += lsec;
@@ -400,7 +410,8 @@ static RefreshArgs make_refresh_args(const TilePlan& P, int NS, int KS, double* 
   R.lds_slater = R.o_sh + P.n_shell * RF_REC;
   R.o_prow = 0;
   R.o_piv = R.o_prow + TILE_W * NS;
-  R.lds_invert = R.o_piv + (TILE_W * (NS + 1) + TILE_W * 2 * NS + 1) / 2;
+  R.o_stg = R.o_piv + (TILE_W * (NS + 1) + TILE_W * 2 * NS + 1) / 2;
+  R.lds_invert = R.o_stg + NS * NS * (TILE_W + 1);  // in doubles: room for a double stage
   R.work = work;
   return R;
 }
