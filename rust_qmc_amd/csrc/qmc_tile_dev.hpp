@@ -260,7 +260,14 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   T* pw = pos + wt * 3 * N;
 
   // ---- state in: positions and tables to LDS, owned inverse columns to VGPRs ----
-  for (int i = g; i < 3 * N; i += 16) pw[i] = E.pos[(int64_t)i * E.ld + wl];
+  // HBM holds the ensemble element-major ([element][walker]): the tile's 16
+  // walkers sit in consecutive lanes for every load and store of the state
+  const int64_t w0 = blk * TILE_W;
+  auto wclamp = [&](int ww) { return w0 + ww < E.W ? w0 + ww : E.W - 1; };
+  for (int i = tid; i < 3 * N * TILE_W; i += TILE_T) {
+    const int el = i / TILE_W, ww = i - el * TILE_W;
+    pos[ww * 3 * N + el] = E.pos[(int64_t)el * E.ld + wclamp(ww)];
+  }
   for (int i = g; i < TA.XS; i += 16) X[wt * TA.XS + i] = T(0);
   for (int i = tid; i < N; i += TILE_T) spin_of[i] = S.elec_spin[i];
   for (int i = tid; i < TA.n_shell; i += TILE_T) tile_write_record(S, i, shrec + i * TILE_REC);
@@ -291,18 +298,40 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   // (cur); the other sector's wait in this lane's slots of the LDS park
   // [walker][column][row], swapped at each sector boundary.
   constexpr int NX = NU > ND ? NU : ND;
-  T* park = lds + TA.o_park + wt * NX * NX;
+  T* park0 = lds + TA.o_park;
+  T* park = park0 + wt * NX * NX;
+  // a sector's inverse between HBM (element m n + k of walker w0 + ww) and
+  // the park (park0[(ww NX + k) NX + m]), all threads of the tile
+  auto park_in = [&](int sp) {
+    const int n = sp ? ND : NU;
+    const int64_t base = sp ? (int64_t)NU * NU : 0;
+    if constexpr (NU != ND)
+      for (int i = tid; i < TILE_W * NX * NX; i += TILE_T) park0[i] = T(0);
+    __syncthreads();
+    for (int i = tid; i < n * n * TILE_W; i += TILE_T) {
+      const int el = i / TILE_W, ww = i - el * TILE_W, m = el / n, k = el - m * n;
+      park0[(ww * NX + k) * NX + m] = E.ainv[(base + el) * E.ld + wclamp(ww)];
+    }
+    __syncthreads();
+  };
+  auto park_out = [&](int sp) {
+    const int n = sp ? ND : NU;
+    const int64_t base = sp ? (int64_t)NU * NU : 0;
+    __syncthreads();
+    for (int i = tid; i < n * n * TILE_W; i += TILE_T) {
+      const int el = i / TILE_W, ww = i - el * TILE_W, m = el / n, k = el - m * n;
+      if (w0 + ww < E.W) E.ainv[(base + el) * E.ld + w0 + ww] = park0[(ww * NX + k) * NX + m];
+    }
+  };
   T cur[2][NX];
+  park_in(0);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int k = g + 16 * t;
 #pragma unroll
-    for (int m = 0; m < NX; ++m) cur[t][m] = k < NU && m < NU ? E.ainv[(int64_t)(m * NU + k) * E.ld + wl] : T(0);
-    if (k < NX)
-#pragma unroll
-      for (int m = 0; m < NX; ++m)
-        park[k * NX + m] = k < ND && m < ND ? E.ainv[(int64_t)(NU * NU + m * ND + k) * E.ld + wl] : T(0);
+    for (int m = 0; m < NX; ++m) cur[t][m] = k < NU && m < NU ? park[k * NX + m] : T(0);
   }
+  park_in(1);
   // MFMA B operands: wavefront q computes MO tile q & 1 over K half q >> 1;
   // lane l holds C[mo = 16 tile + l%16][ao = (l/16) KS + half KH + s]
   const int tb = q & 1, kh = q >> 1;
@@ -684,18 +713,21 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   }
   __syncthreads();
   // ---- state out ----
+  for (int i = tid; i < 3 * N * TILE_W; i += TILE_T) {
+    const int el = i / TILE_W, ww = i - el * TILE_W;
+    if (w0 + ww < E.W) E.pos[(int64_t)el * E.ld + w0 + ww] = pos[ww * 3 * N + el];
+  }
+  park_out(1);  // spin 1 rests in the park, spin 0 is in cur
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int k = g + 16 * t;
+    if (k < NU)
+#pragma unroll
+      for (int m = 0; m < NU; ++m) park[k * NX + m] = cur[t][m];
+  }
+  park_out(0);
   if (live) {
-    for (int i = g; i < 3 * N; i += 16) E.pos[(int64_t)i * E.ld + w] = pw[i];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int k = g + 16 * t;
-      if (k < NU)
-#pragma unroll
-        for (int m = 0; m < NU; ++m) E.ainv[(int64_t)(m * NU + k) * E.ld + w] = cur[t][m];
-      if (k < ND)
-#pragma unroll
-        for (int m = 0; m < ND; ++m) E.ainv[(int64_t)(NU * NU + m * ND + k) * E.ld + w] = park[k * NX + m];
-    }
     if (g == 0) {
       E.logdet[w] = logdet;
       E.sign[w] = sgn;
