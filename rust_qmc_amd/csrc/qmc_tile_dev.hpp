@@ -38,10 +38,10 @@ struct TileMfma<double> {
 constexpr int TILE_T = 256;   // threads per tile
 constexpr int TILE_MO = 32;   // MO columns of Phi (two 16-wide MFMA tiles)
 // Row stride of Phi in LDS: a walker's row is read by its 16 lanes at one
-// address (a broadcast), the four walkers of a wavefront at 36-element
+// address (a broadcast), the four walkers of a wavefront at 33-element
 // offsets that fall in different banks; a stride of 32 put all four in the
 // same bank.
-constexpr int TILE_PS = 36;
+constexpr int TILE_PS = 33;
 constexpr int TILE_MAXP = 6;  // primitives per shell
 // elements per shell record in LDS: 32 used, an odd stride so that the 16
 // lanes of a group, reading the same field of 16 different records, hit 16
@@ -755,11 +755,11 @@ inline TileArgs make_tile_args(const TilePlan& P, int esz = (int)sizeof(float), 
   T.o_phi = T.o_x + TILE_W * T.XS;
   T.o_spin = T.o_phi + 2 * TILE_W * TILE_PS;
   T.o_sh = T.o_spin + ((P.N + 3) / 4 * 4 * 4 + esz - 1) / esz;  // N ints
-  T.o_sum = (T.o_sh + P.n_shell * TILE_REC + per8 - 1) / per8 * per8;  // 8-byte aligned
+  T.o_sum = T.o_phi;  // the per-step walker sums reuse Phi, idle during the local energy
   const int nx = P.nu > P.nd ? P.nu : P.nd;
   const int per16 = 16 / esz;  // the park and the drift-diffusion rows start 16-byte aligned
   auto al16 = [&](int o) { return (o + per16 - 1) / per16 * per16; };
-  T.o_park = al16(T.o_sum + per8 * 3 * TILE_W);
+  T.o_park = al16(T.o_sh + P.n_shell * TILE_REC);
   T.o_rng = al16(T.o_park + TILE_W * nx * nx);
   T.o_cmo = al16(T.o_rng + TILE_W * 4 * nx);
   T.o_ck = T.o_cmo + (dd ? 2 * P.A * TILE_CK : 0);
