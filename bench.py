@@ -362,7 +362,7 @@ def run_batch(Q, sweeps, trial, precisions=("mixed", "fp64")):
     return out
 
 
-def run_benzene(Q, sweeps, W=8192):
+def run_benzene(Q, sweeps, W=8192, engine="auto"):
     """Benzene (BenzeneGTO, 42 electrons, 66 AOs, C-H 2.05 bohr), 8192
     walkers: Metropolis single-electron sweeps, drift-diffusion sweeps
     (tau = 0.01) and SR-grad samples (E_L, the 4 Jastrow O_k and dE_L/dp_k,
@@ -371,9 +371,10 @@ def run_benzene(Q, sweeps, W=8192):
     so the per-sweep paired energy differences state the mixed error."""
     import numpy as np
 
-    out, tr, keep = {"walkers": W, "sweeps": sweeps, "equilibration_sweeps": 64}, {}, {}
+    out, tr, keep = {"walkers": W, "sweeps": sweeps, "equilibration_sweeps": 64, "engine_option": engine}, {}, {}
     for prec in ("mixed", "fp64"):
         s = Q.BenzeneGTO(1.5, 2.0, 2.64, 2.05, precision=prec)
+        s.set_engine(engine)
         ens = Q.Ensemble(s, W, 3)
         ens.sweeps(64, 0.2, refresh_interval=64)
         keep[prec] = (s, ens)
@@ -525,6 +526,8 @@ def main():
                     help="timed sweeps of the benzene + CH4 + H2 batch per precision (0: skip it)")
     ap.add_argument("--dmc-steps", type=int, default=400, help="IS-DMC steps per precision (0: skip)")
     ap.add_argument("--benzene-sweeps", type=int, default=128, help="timed benzene sweeps per leg (0: skip)")
+    ap.add_argument("--benzene-engine", default="auto", choices=("auto", "wave"),
+                    help="engine of the benzene leg: auto (tile engine) or the one-wave-per-walker engine")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 8 device-to-device copies of BYTES (HBM counter calibration) and exit")
     ap.add_argument("--only", default="", help="comma-separated legs to run (profiling): headline,pairs,h2,li,"
@@ -577,7 +580,8 @@ def main():
         h = dmc["h2"]
         em, ef = h["mixed"].pop("production_energies"), h["fp64"].pop("production_energies")
         h["paired_mixed_minus_fp64"] = _paired(Q, em, ef)
-    benzene = run_benzene(Q, args.benzene_sweeps) if args.benzene_sweeps > 0 and want("benzene") else None
+    benzene = (run_benzene(Q, args.benzene_sweeps, engine=args.benzene_engine)
+               if args.benzene_sweeps > 0 and want("benzene") else None)
     log("dmc/benzene done")
     if not want("headline"):
         print(json.dumps({"legs": sorted(only), "ch4_paired_chains": pairs, "h2": h2, "li": li,
