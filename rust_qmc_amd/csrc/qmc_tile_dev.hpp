@@ -632,69 +632,98 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
     }
     __syncthreads();
 
-    // local energy: lane g takes the electrons of sector slots g and g + 16
+    // local energy in three passes over the lanes: slot g of sector 0, slot
+    // g of sector 1, then the slots from 16 up of both sectors (lane g below
+    // NU - 16 takes slot 16 + g of sector 0 with its own column, the next
+    // ND - 16 lanes take sector 1's from the park) -- three passes where
+    // slot-by-sector order would take four, two of them on five lanes
     double kin = 0.0, pot = 0.0;
-#pragma unroll
-    for (int spin = 0; spin < 2; ++spin) {
-      const int n = spin ? ND : NU;
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int k = g + 16 * t;
-        if (k >= n) continue;
-        const int i = S.sector_elec[spin][k];
-        const T x = pw[3 * i], y = pw[3 * i + 1], z = pw[3 * i + 2];
-        // after both sectors moved, spin 0's columns are in cur, spin 1's parked
-        T col[NX];
-#pragma unroll
-        for (int m = 0; m < NX; ++m) col[m] = spin ? park[k * NX + m] : cur[t][m];
-        auto wgt = [&](int ao) {
-          T s = T(0);
-#pragma unroll
-          for (int m = 0; m < n; ++m) s += S.moT[spin][ao][m] * col[m];
-          return s;
-        };
-        T gdx, gdy, gdz, ld;
-        det_terms_w(S, x, y, z, wgt, gdx, gdy, gdz, ld, NoShell{});
-        T gux = T(0), guy = T(0), guz = T(0), lu = T(0), pairsq = T(0), pee = T(0), pen = T(0);
-        for (int j = 0; j < N; ++j) {
-          if (j == i) continue;
-          const T dx = x - pw[3 * j], dy = y - pw[3 * j + 1], dz = z - pw[3 * j + 2];
-          T r, ir;
-          M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
+    auto el_add = [&](bool on, int spin, int k, auto wgt) {
+      if (!on) return;
+      const int i = S.sector_elec[spin][k];
+      const T x = pw[3 * i], y = pw[3 * i + 1], z = pw[3 * i + 2];
+      T gdx, gdy, gdz, ld;
+      det_terms_w(S, x, y, z, wgt, gdx, gdy, gdz, ld, NoShell{});
+      T gux = T(0), guy = T(0), guz = T(0), lu = T(0), pairsq = T(0), pee = T(0), pen = T(0);
+      for (int j = 0; j < N; ++j) {
+        if (j == i) continue;
+        const T dx = x - pw[3 * j], dy = y - pw[3 * j + 1], dz = z - pw[3 * j + 2];
+        T r, ir;
+        M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
+        T u1, lp;
+        uee_d<T, Js>(S, r, ir, spin_of[j] == spin, u1, lp);
+        const bool ok = r >= T(1e-10);
+        const T f = ok ? u1 * ir : T(0);
+        gux += f * dx;
+        guy += f * dy;
+        guz += f * dz;
+        lu += ok ? lp : T(0);
+        pairsq += ok ? u1 * u1 : T(0);
+        pee += ir;
+      }
+      for (int I = 0; I < M_; ++I) {
+        const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
+        T r, ir;
+        M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
+        pen += S.Z[I] * ir;
+        if (en_on) {
           T u1, lp;
-          uee_d<T, Js>(S, r, ir, spin_of[j] == spin, u1, lp);
+          uen_d<T, Js>(S, r, ir, I, u1, lp);
           const bool ok = r >= T(1e-10);
           const T f = ok ? u1 * ir : T(0);
           gux += f * dx;
           guy += f * dy;
           guz += f * dz;
           lu += ok ? lp : T(0);
-          pairsq += ok ? u1 * u1 : T(0);
-          pee += ir;
         }
-        for (int I = 0; I < M_; ++I) {
-          const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
-          T r, ir;
-          M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
-          pen += S.Z[I] * ir;
-          if (en_on) {
-            T u1, lp;
-            uen_d<T, Js>(S, r, ir, I, u1, lp);
-            const bool ok = r >= T(1e-10);
-            const T f = ok ? u1 * ir : T(0);
-            gux += f * dx;
-            guy += f * dy;
-            guz += f * dz;
-            lu += ok ? lp : T(0);
-          }
-        }
-        const T gs = S.grad_scale;
-        const T gjx = gs * gux, gjy = gs * guy, gjz = gs * guz;
-        const T lj = (S.flags & F_J2_LAP_QUIRK) ? (lu + pairsq) : (lu + gux * gux + guy * guy + guz * guz);
-        const T lap = ld + lj + T(2) * (gdx * gjx + gdy * gjy + gdz * gjz);
-        kin += -0.5 * (double)lap;
-        pot += 0.5 * (double)pee - (double)pen;
       }
+      const T gs = S.grad_scale;
+      const T gjx = gs * gux, gjy = gs * guy, gjz = gs * guz;
+      const T lj = (S.flags & F_J2_LAP_QUIRK) ? (lu + pairsq) : (lu + gux * gux + guy * guy + guz * guz);
+      const T lap = ld + lj + T(2) * (gdx * gjx + gdy * gjy + gdz * gjz);
+      kin += -0.5 * (double)lap;
+      pot += 0.5 * (double)pee - (double)pen;
+    };
+    // after both sectors moved, spin 0's columns are in cur, spin 1's parked
+    el_add(g < NU, 0, g, [&](int ao) {
+      T s = T(0);
+#pragma unroll
+      for (int m = 0; m < NU; ++m) s += S.moT[0][ao][m] * cur[0][m];
+      return s;
+    });
+    {
+      T col[NX];
+#pragma unroll
+      for (int m = 0; m < NX; ++m) col[m] = park[g * NX + m];
+      el_add(g < ND, 1, g, [&](int ao) {
+        T s = T(0);
+#pragma unroll
+        for (int m = 0; m < ND; ++m) s += S.moT[1][ao][m] * col[m];
+        return s;
+      });
+    }
+    constexpr int XU = NU > 16 ? NU - 16 : 0, XD = ND > 16 ? ND - 16 : 0;
+    static_assert(XU + XD <= 16, "more extra slots than lanes");
+    if constexpr (XU + XD > 0) {
+      // mixed sectors across the lanes: both coefficient sets stay scalar
+      // operands, each lane's other column zero
+      const bool s1 = g >= XU, on = g < XU + XD;
+      const int k = on ? 16 + (s1 ? g - XU : g) : 0;
+      T c0[NX], c1[NX];
+#pragma unroll
+      for (int m = 0; m < NX; ++m) {
+        const T pv = park[(s1 ? k : 0) * NX + m];
+        c0[m] = s1 ? T(0) : cur[1][m];
+        c1[m] = s1 ? pv : T(0);
+      }
+      el_add(on, s1 ? 1 : 0, k, [&](int ao) {
+        T s = T(0);
+#pragma unroll
+        for (int m = 0; m < NU; ++m) s += S.moT[0][ao][m] * c0[m];
+#pragma unroll
+        for (int m = 0; m < ND; ++m) s += S.moT[1][ao][m] * c1[m];
+        return s;
+      });
     }
     el = gsum(kin) + gsum(pot) + S.vnn;
     // per-step sums over the tile's walkers, in walker order
