@@ -7,12 +7,15 @@
 
 #include "qmc_basis.hpp"
 #include "qmc_sys.hpp"
+#define QMC_EXPN_QUAL __host__ __device__ __forceinline__
+#include "qmc_expn.hpp"
 
 namespace qmc {
 
 // ---------------------------------------------------------------------------
-// precision-dependent math: FP64 uses the correctly rounded library
-// routines; FP32 uses the hardware transcendental instructions.
+// precision-dependent math: FP64 uses the library routines, the primitives'
+// exponentials exp_nonpos (qmc_expn.hpp) and the reciprocals Newton-refined
+// hardware estimates; FP32 uses the hardware transcendental instructions.
 // ---------------------------------------------------------------------------
 template <class T>
 struct M;
@@ -21,6 +24,9 @@ struct M;
 template <>
 struct M<double> {
   static __device__ __forceinline__ double ex(double x) { return exp(x); }
+  // e^x of a primitive's argument (finite, not positive): exp_nonpos, the
+  // library exponential without its range checks
+  static __device__ __forceinline__ double exn(double x) { return exp_nonpos(x); }
   static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
   static __device__ __forceinline__ double lg(double x) { return log(x); }
   // 1/x and 1/sqrt(x): the hardware estimates (v_rcp_f64, v_rsq_f64)
@@ -57,6 +63,7 @@ struct M<double> {
 template <>
 struct M<float> {
   static __device__ __forceinline__ float ex(float x) { return __expf(x); }
+  static __device__ __forceinline__ float exn(float x) { return __expf(x); }
   static __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
   static __device__ __forceinline__ float lg(float x) { return __logf(x); }
@@ -172,7 +179,7 @@ __device__ __forceinline__ void mo_values(const DevSys<T>& S, int spin, T x, T y
     if (kind <= SH_P1) {
       T rs = T(0), rp = T(0);
       for (int k = 0; k < np; ++k) {
-        T e = M<T>::ex(-S.prim_exp[p0 + k] * r2);
+        T e = M<T>::exn(-S.prim_exp[p0 + k] * r2);
         if (kind == SH_S || kind == SH_SP) rs += S.coef_s[p0 + k] * e;
         if (kind != SH_S) rp += S.coef_p[p0 + k] * e;
       }
@@ -197,7 +204,7 @@ __device__ __forceinline__ void mo_values(const DevSys<T>& S, int spin, T x, T y
       for (int k = 0; k < np; ++k) {
         T sp = T(1);
         for (int q = 0; q < sh.pw(k); ++q) sp *= s;
-        v += S.coef_s[p0 + k] * sp * M<T>::ex(-S.prim_exp[p0 + k] * s);
+        v += S.coef_s[p0 + k] * sp * M<T>::exn(-S.prim_exp[p0 + k] * s);
       }
       mo_add<NS>(S, spin, a0, v, phi);
     }
@@ -224,7 +231,7 @@ __device__ __forceinline__ void det_terms_w(const DevSys<T>& S, T x, T y, T z, W
       T rs = 0, rsa = 0, rsaa = 0, rp = 0, rpa = 0, rpaa = 0;
       for (int k = 0; k < np; ++k) {
         const T a = S.prim_exp[p0 + k];
-        T e = M<T>::ex(-a * r2);
+        T e = M<T>::exn(-a * r2);
         if (kind == SH_S || kind == SH_SP) {
           T cs = S.coef_s[p0 + k] * e;
           rs += cs;
@@ -283,7 +290,7 @@ __device__ __forceinline__ void det_terms_w(const DevSys<T>& S, T x, T y, T z, W
         for (int q = 0; q < p; ++q) sp *= s;
         T spm1 = p >= 1 ? sp / s : T(0);
         T spm2 = p >= 2 ? spm1 / s : T(0);
-        T e = S.coef_s[p0 + k] * M<T>::ex(-zt * s);
+        T e = S.coef_s[p0 + k] * M<T>::exn(-zt * s);
         d1 += e * (-zt * sp + pf * spm1);
         d2 += e * (zt * zt * sp - T(2) * zt * pf * spm1 + pf * (pf - T(1)) * spm2);
       }
@@ -331,7 +338,7 @@ __device__ __forceinline__ void shell_values(const DevSys<T>& S, int sh, T x, T 
   if (kind <= SH_P1) {
     T rs = T(0), rp = T(0);
     for (int k = 0; k < np; ++k) {
-      T e = M<T>::ex(-S.prim_exp[p0 + k] * r2);
+      T e = M<T>::exn(-S.prim_exp[p0 + k] * r2);
       rs += S.coef_s[p0 + k] * e;
       rp += S.coef_p[p0 + k] * e;
     }
@@ -356,7 +363,7 @@ __device__ __forceinline__ void shell_values(const DevSys<T>& S, int sh, T x, T 
     for (int k = 0; k < np; ++k) {
       T sp = T(1);
       for (int q = 0; q < S.prim_pow[p0 + k]; ++q) sp *= s;
-      v += S.coef_s[p0 + k] * sp * M<T>::ex(-S.prim_exp[p0 + k] * s);
+      v += S.coef_s[p0 + k] * sp * M<T>::exn(-S.prim_exp[p0 + k] * s);
     }
     chi[a0] = v;
   }

@@ -81,7 +81,7 @@ __device__ __forceinline__ void rf_shell_values(const double* rec, double x, dou
 #pragma unroll
     for (int k = 0; k < TILE_MAXP; ++k) {
       if (k >= np) break;
-      const double e = M<double>::ex(-rec[8 + k] * r2);
+      const double e = M<double>::exn(-rec[8 + k] * r2);
       rs += rec[14 + k] * e;
       rp += rec[20 + k] * e;
     }
@@ -106,7 +106,7 @@ __device__ __forceinline__ void rf_shell_values(const double* rec, double x, dou
     for (int k = 0; k < np; ++k) {
       double sp = 1.0;
       for (int q = 0; q < (int)rec[26 + k]; ++q) sp *= s;
-      v += rec[14 + k] * sp * M<double>::ex(-rec[8 + k] * s);
+      v += rec[14 + k] * sp * M<double>::exn(-rec[8 + k] * s);
     }
     chi[a0] = v;
   }

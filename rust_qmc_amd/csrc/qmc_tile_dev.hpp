@@ -113,7 +113,7 @@ __device__ __forceinline__ void tile_shell_values(const T* rec, T x, T y, T z, T
   if (kind <= SH_P1) {
     T rs = T(0), rp = T(0);
     for (int k = 0; k < np; ++k) {
-      const T e = M<T>::ex(-rec[8 + k] * r2);
+      const T e = M<T>::exn(-rec[8 + k] * r2);
       rs += rec[14 + k] * e;
       rp += rec[20 + k] * e;
     }
@@ -138,7 +138,7 @@ __device__ __forceinline__ void tile_shell_values(const T* rec, T x, T y, T z, T
     for (int k = 0; k < np; ++k) {
       T sp = T(1);
       for (int q = 0; q < rec_int(rec[26 + k]); ++q) sp *= sr;
-      v += rec[14 + k] * sp * M<T>::ex(-rec[8 + k] * sr);
+      v += rec[14 + k] * sp * M<T>::exn(-rec[8 + k] * sr);
     }
     chi[a0] = v;
   }
@@ -157,7 +157,7 @@ __device__ __forceinline__ void tile_shell_grad(const T* rec, T x, T y, T z, F f
     T rs = T(0), rsa = T(0), rp = T(0), rpa = T(0);
     for (int k = 0; k < np; ++k) {
       const T a = rec[8 + k];
-      const T e = M<T>::ex(-a * r2);
+      const T e = M<T>::exn(-a * r2);
       const T cs = rec[14 + k] * e, cp = rec[20 + k] * e;
       rs += cs;
       rsa += cs * a;
@@ -194,7 +194,7 @@ __device__ __forceinline__ void tile_shell_grad(const T* rec, T x, T y, T z, F f
     T sp = T(1);
     for (int q = 0; q < p; ++q) sp *= s;
     const T spm1 = p >= 1 ? sp / s : T(0);
-    const T e = rec[14 + k] * M<T>::ex(-zt * s);
+    const T e = rec[14 + k] * M<T>::exn(-zt * s);
     v += e * sp;
     d1 += e * (-zt * sp + T(p) * spm1);
   }

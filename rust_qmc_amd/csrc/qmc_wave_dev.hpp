@@ -113,7 +113,7 @@ __device__ __forceinline__ int shell_values_grad(const DevSys<T>& S, int sh, T x
     T rs = 0, rsa = 0, rp = 0, rpa = 0;
     for (int k = 0; k < np; ++k) {
       const T a = S.prim_exp[p0 + k];
-      const T e = M<T>::ex(-a * r2);
+      const T e = M<T>::exn(-a * r2);
       const T cs = S.coef_s[p0 + k] * e, cp = S.coef_p[p0 + k] * e;
       rs += cs;
       rsa += cs * a;
@@ -144,7 +144,7 @@ __device__ __forceinline__ int shell_values_grad(const DevSys<T>& S, int sh, T x
     T sp = T(1);
     for (int q = 0; q < p; ++q) sp *= s;
     const T spm1 = p >= 1 ? sp / s : T(0);
-    const T e = S.coef_s[p0 + k] * M<T>::ex(-zt * s);
+    const T e = S.coef_s[p0 + k] * M<T>::exn(-zt * s);
     f += e * sp;
     d1 += e * (-zt * sp + T(p) * spm1);
   }
