@@ -522,7 +522,7 @@ def main():
     ap.add_argument("--pair-sweeps", type=int, default=5000, help="CH4 paired-chain sweeps (0: skip)")
     ap.add_argument("--h2-steps", type=int, default=10000, help="H2 all-electron steps incl. 10%% burn-in (0: skip)")
     ap.add_argument("--li-sweeps", type=int, default=20000, help="Li sweeps per precision (0: skip)")
-    ap.add_argument("--batch-sweeps", type=int, default=200,
+    ap.add_argument("--batch-sweeps", type=int, default=2000,
                     help="timed sweeps of the benzene + CH4 + H2 batch per precision (0: skip it)")
     ap.add_argument("--dmc-steps", type=int, default=400, help="IS-DMC steps per precision (0: skip)")
     ap.add_argument("--benzene-sweeps", type=int, default=128, help="timed benzene sweeps per leg (0: skip)")
