@@ -46,6 +46,7 @@ struct RtShell {
 };
 
 struct RtBasis {
+  static constexpr bool lds_mo = false;  // MO coefficients read from DevSys
   static constexpr int n_nuc_ct = 0;  // nucleus count unknown at compile time
   template <class T, class F>
   static __device__ __forceinline__ void shells(const DevSys<T>& S, F&& f) {
@@ -84,6 +85,7 @@ struct CtShell {
 
 template <class Bd>
 struct CtBasis {
+  static constexpr bool lds_mo = false;
   static constexpr int n_nuc_ct = Bd::n_nuc;
   template <class T, class F>
   static __device__ __forceinline__ void shells(const DevSys<T>&, F&& f) {
@@ -175,5 +177,15 @@ bool basis_matches(const DevSys<T>& D) {
   }
   return na == D.n_ao;
 }
+
+// CtBasis whose MO coefficients come from a per-workgroup LDS table
+// (lds_mo_tab, filled at kernel start) instead of DevSys: the compiler then
+// keeps them in vector registers rather than in scalar registers that it
+// spills to VGPR lanes and restores with v_readlane.  Used by the FP32
+// single-electron sweep kernels.
+template <class Bd>
+struct CtBasisL : CtBasis<Bd> {
+  static constexpr bool lds_mo = true;
+};
 
 }  // namespace qmc

@@ -37,14 +37,20 @@ __global__ void __launch_bounds__(256, 2) batch_sweep_kernel(BatchDev B) {
       tile_sweep_block<21, 21, 17, JsNone, float>(*(const DevSys<float>*)G.S, G.E, B.TA, G.A, G.partial, nullptr,
                                                   it, G.nblk, tid, lds_f);
     } else {
+      // the ensemble's MO coefficients to the LDS table (all four waves,
+      // before any of them may skip), then the register engine's blocks,
+      // as the FP32 sweep kernel runs them (CtBasisL)
       const int64_t blk = it * 4 + (tid >> 6);
-      if (blk < G.nblk) {
-        if (G.kind == SEG_REG_CH4_STO3G)
-          sweep_block<5, 5, CtBasis<BdCH4Sto3g>, JsNone, float>(*(const DevSys<float>*)G.S, G.E, G.A, G.partial,
-                                                                nullptr, blk, G.nblk, tid & 63);
-        else
-          sweep_block<1, 1, CtBasis<BdH2>, JsNone, float>(*(const DevSys<float>*)G.S, G.E, G.A, G.partial, nullptr,
-                                                          blk, G.nblk, tid & 63);
+      if (G.kind == SEG_REG_CH4_STO3G) {
+        fill_mo_tab<5, 5>(*(const DevSys<float>*)G.S);
+        if (blk < G.nblk)
+          sweep_block<5, 5, CtBasisL<BdCH4Sto3g>, JsNone, float>(*(const DevSys<float>*)G.S, G.E, G.A, G.partial,
+                                                                 nullptr, blk, G.nblk, tid & 63);
+      } else {
+        fill_mo_tab<1, 1>(*(const DevSys<float>*)G.S);
+        if (blk < G.nblk)
+          sweep_block<1, 1, CtBasisL<BdH2>, JsNone, float>(*(const DevSys<float>*)G.S, G.E, G.A, G.partial, nullptr,
+                                                           blk, G.nblk, tid & 63);
       }
     }
   }

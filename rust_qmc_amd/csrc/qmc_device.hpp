@@ -161,10 +161,37 @@ struct Rng<float> {
 // GTO shells follow methane.rs:498-590 (unnormalised primitives, s and p);
 // STO shells follow slater.rs:92-144 (and slater.rs:17-41 for Slater1s).
 // ---------------------------------------------------------------------------
-template <int NS, class T>
+// LDS table of the MO coefficients for bases with lds_mo: entry
+// [(spin * MO_TAB_AO + ao) * MO_TAB_MO + m], zero past the system's AO and
+// sector sizes; filled by fill_mo_tab at the start of a kernel.
+constexpr int MO_TAB_AO = 32, MO_TAB_MO = 8;
+__shared__ float g_mo_tab32[2 * MO_TAB_AO * MO_TAB_MO];
+__shared__ double g_mo_tab64[2 * MO_TAB_AO * MO_TAB_MO];
+__device__ __forceinline__ float* lds_mo_tab(float) { return g_mo_tab32; }
+__device__ __forceinline__ double* lds_mo_tab(double) { return g_mo_tab64; }
+template <class Bs, class T>
+__device__ __forceinline__ T mo_coef(const DevSys<T>& S, int spin, int ao, int m) {
+  if constexpr (Bs::lds_mo)
+    return lds_mo_tab(T())[(spin * MO_TAB_AO + ao) * MO_TAB_MO + m];
+  else
+    return S.moT[spin][ao][m];
+}
+// every thread of the block; ends with a block barrier
+template <int NU, int ND, class T>
+__device__ __forceinline__ void fill_mo_tab(const DevSys<T>& S) {
+  static_assert(NU <= MO_TAB_MO && ND <= MO_TAB_MO, "sector too large for the MO table");
+  T* tab = lds_mo_tab(T());
+  for (int q = threadIdx.x; q < 2 * MO_TAB_AO * MO_TAB_MO; q += blockDim.x) {
+    const int sp = q / (MO_TAB_AO * MO_TAB_MO), ao = (q / MO_TAB_MO) % MO_TAB_AO, m = q % MO_TAB_MO;
+    tab[q] = (ao < S.n_ao && m < (sp ? ND : NU)) ? S.moT[sp][ao][m] : T(0);
+  }
+  __syncthreads();
+}
+
+template <int NS, class Bs = RtBasis, class T>
 __device__ __forceinline__ void mo_add(const DevSys<T>& S, int spin, int ao, T v, T phi[Cap<NS>::v]) {
 #pragma unroll
-  for (int m = 0; m < NS; ++m) phi[m] += S.moT[spin][ao][m] * v;
+  for (int m = 0; m < NS; ++m) phi[m] += mo_coef<Bs>(S, spin, ao, m) * v;
 }
 
 // phi[m] = sum_ao C[spin][m][ao] chi_ao(r), m < NS
@@ -184,19 +211,19 @@ __device__ __forceinline__ void mo_values(const DevSys<T>& S, int spin, T x, T y
         if (kind != SH_S) rp += S.coef_p[p0 + k] * e;
       }
       if (kind == SH_S) {
-        mo_add<NS>(S, spin, a0, rs, phi);
+        mo_add<NS, Bs>(S, spin, a0, rs, phi);
       } else if (kind == SH_P1) {
         const int m = sh.m();
-        mo_add<NS>(S, spin, a0, (m == 0 ? dx : (m == 1 ? dy : dz)) * rp, phi);
+        mo_add<NS, Bs>(S, spin, a0, (m == 0 ? dx : (m == 1 ? dy : dz)) * rp, phi);
       } else {
         int b = a0;
         if (kind == SH_SP) {
-          mo_add<NS>(S, spin, a0, rs, phi);
+          mo_add<NS, Bs>(S, spin, a0, rs, phi);
           b = a0 + 1;
         }
-        mo_add<NS>(S, spin, b, dx * rp, phi);
-        mo_add<NS>(S, spin, b + 1, dy * rp, phi);
-        mo_add<NS>(S, spin, b + 2, dz * rp, phi);
+        mo_add<NS, Bs>(S, spin, b, dx * rp, phi);
+        mo_add<NS, Bs>(S, spin, b + 1, dy * rp, phi);
+        mo_add<NS, Bs>(S, spin, b + 2, dz * rp, phi);
       }
     } else {
       T s = M<T>::sq(r2);
@@ -206,7 +233,7 @@ __device__ __forceinline__ void mo_values(const DevSys<T>& S, int spin, T x, T y
         for (int q = 0; q < sh.pw(k); ++q) sp *= s;
         v += S.coef_s[p0 + k] * sp * M<T>::exn(-S.prim_exp[p0 + k] * s);
       }
-      mo_add<NS>(S, spin, a0, v, phi);
+      mo_add<NS, Bs>(S, spin, a0, v, phi);
     }
   });
 }
@@ -321,7 +348,7 @@ __device__ __forceinline__ void det_terms(const DevSys<T>& S, int spin, T x, T y
   auto wgt = [&](int ao) {
     T w = 0;
 #pragma unroll
-    for (int m = 0; m < NS; ++m) w += S.moT[spin][ao][m] * ai[m][0];
+    for (int m = 0; m < NS; ++m) w += mo_coef<Bs>(S, spin, ao, m) * ai[m][0];
     return w;
   };
   det_terms_w<T, decltype(wgt), OnShell, Bs>(S, x, y, z, wgt, gdx, gdy, gdz, ld, on_shell);
