@@ -47,7 +47,7 @@ FLOP_PER_WM = 16338
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s, MI355X vector FP64 / FP32
 # measured HBM traffic of one mixed-precision sweep launch at 65,536 walkers
 # and 16 sweeps per launch (22,532 KiB read + 23,296 KiB written)
-TRAFFIC_BYTES = int((2 * 11485.5 + 23296) * 1024)  # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB per launch
+TRAFFIC_BYTES = int((2 * 11472.0 + 23296) * 1024)  # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB per launch
 H2_README_HA = -1.15754  # README.md:22, -4.287 eV read as E - 2 E_H
 
 
