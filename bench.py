@@ -659,7 +659,7 @@ def main():
         # factors from a copy kernel of known size
         "traffic": TRAFFIC_BYTES if (W == 65536 and args.steps_per_launch == 16) else None,
         "traffic_unit": "bytes per launch",
-        "kernel": "sweep_kernel<5,5,CtBasis<BdCH4Sto3g>,JsNone,float>",
+        "kernel": "sweep_kernel<5,5,CtBasisL<BdCH4Sto3g>,JsNone,float>",
         "flop_per_walker_move": FLOP_PER_WM,
     }
     if not args.no_cpu:
