@@ -186,6 +186,7 @@ bool basis_matches(const DevSys<T>& D) {
 template <class Bd>
 struct CtBasisL : CtBasis<Bd> {
   static constexpr bool lds_mo = true;
+  static constexpr int n_ao = bd_a0<Bd>(Bd::n_shell);  // the table must hold them
 };
 
 }  // namespace qmc

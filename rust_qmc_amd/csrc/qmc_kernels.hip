@@ -59,7 +59,10 @@ template <int NU, int ND, class Bs, class Js, class T>
 __global__ void __launch_bounds__(64) sweep_kernel(const DevSys<T>* __restrict__ Sp, EnsView<T> E,
                                                    SweepArgs A, double* __restrict__ partial,
                                                    double* __restrict__ el_trace) {
-  if constexpr (Bs::lds_mo) fill_mo_tab<NU, ND>(*Sp);
+  if constexpr (Bs::lds_mo) {
+    static_assert(Bs::n_ao <= MO_TAB_AO, "too many AOs for the LDS MO table");
+    fill_mo_tab<NU, ND>(*Sp);
+  }
   sweep_block<NU, ND, Bs, Js, T>(*Sp, E, A, partial, el_trace, blockIdx.x, gridDim.x, threadIdx.x);
 }
 
