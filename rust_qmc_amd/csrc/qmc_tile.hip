@@ -1,13 +1,13 @@
 // qmc_tile.hip -- the tile engine: one 256-thread workgroup (four
 // wavefronts) owns a tile of 16 walkers, 16 lanes per walker.  Built for
 // spin sectors too large for the register engine (benzene: 21 + 21
-// electrons, 66 AOs) in mixed precision.
+// electrons, 66 AOs), with FP32 (mixed precision) or FP64 walker state.
 //
-// Per walker, lane g of its 16-lane group keeps columns g and g + 16 of both
-// inverse Slater matrices in VGPRs for the whole launch; positions live in
-// LDS.  All 16 walkers of the tile move the same electron together, so one
-// single-electron move is
-//   A  proposal (Philox, Box-Muller), computed redundantly by the group
+// Per walker, lane g of its 16-lane group keeps columns g and g + 16 of the
+// moving sector's inverse Slater matrix in VGPRs (the other sector's rest
+// in LDS); positions live in LDS.  All 16 walkers of the tile move the same
+// electron together, so one single-electron move is
+//   A  proposal from the sector's random numbers, drawn ahead into LDS
 //   B  AO values of the 16 proposals, shells spread over the group's lanes,
 //      into the LDS matrix X[walker][ao]
 //   C  MO values Phi = X C for all 16 walkers at once with
@@ -75,8 +75,12 @@ bool tile_plan(const DevSys<float>& D, TilePlan& P) {
   P.n_shell = D.n_shell;
   for (int g = 0; g < 16; ++g)
     for (int j = 0; j < TILE_MAXSH; ++j) P.lane_sh[g][j] = -1;
-  // shells spread over the 16 lanes of a group, largest primitive count first
-  int load[16] = {0}, cnt[16] = {0};
+  // shells spread over the 16 lanes of a group by slots: the lanes
+  // evaluate slot j together, so a slot costs its largest primitive count.
+  // Slot j takes shells 16 j .. 16 j + 15 of the descending primitive-count
+  // order (benzene 6-31G: six 6-primitive and ten 3-primitive shells, then
+  // two 3-primitive and twelve 1-primitive ones: 6 + 3 primitive passes),
+  // in alternating lane order so that the lanes' totals stay even.
   int order[MAXSH];
   for (int i = 0; i < D.n_shell; ++i) order[i] = i;
   for (int i = 1; i < D.n_shell; ++i)  // stable insertion sort, descending primitive count
@@ -86,12 +90,9 @@ bool tile_plan(const DevSys<float>& D, TilePlan& P) {
       order[j - 1] = t;
     }
   for (int i = 0; i < D.n_shell; ++i) {
-    int best = -1;
-    for (int g = 0; g < 16; ++g)
-      if (cnt[g] < TILE_MAXSH && (best < 0 || load[g] < load[best])) best = g;
-    if (best < 0) return false;
-    P.lane_sh[best][cnt[best]++] = order[i];
-    load[best] += D.sh_nprim[order[i]];
+    const int slot = i / 16, r = i % 16;
+    if (slot >= TILE_MAXSH) return false;
+    P.lane_sh[slot & 1 ? 15 - r : r][slot] = order[i];
   }
   P.ok = true;
   return true;
