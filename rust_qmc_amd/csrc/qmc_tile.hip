@@ -69,7 +69,8 @@ bool tile_plan(const DevSys<float>& D, TilePlan& P) {
   P.n_nuc = D.n_nuc;
   P.no_series = D.k_ee == 0 && D.k_en == 0;
   // compiled shape: 21 + 21 electrons, up to 68 AOs (benzene)
-  if (P.nu != 21 || P.nd != 21 || P.A > 68 || P.N + P.n_nuc > 64 || D.n_shell > 16 * TILE_MAXSH) return false;
+  if (P.nu != 21 || P.nd != 21 || P.A > 68 || P.N > 16 * TILE_EE || P.n_nuc > 16 || D.n_shell > 16 * TILE_MAXSH)
+    return false;
   for (int i = 0; i < D.n_shell; ++i)
     if (D.sh_nprim[i] > TILE_MAXP) return false;
   P.n_shell = D.n_shell;

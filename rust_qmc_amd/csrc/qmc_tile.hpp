@@ -9,6 +9,9 @@ namespace qmc {
 
 constexpr int TILE_W = 16;   // walkers per tile (one workgroup)
 constexpr int TILE_MAXSH = 4;  // shells per lane in the AO phase
+// Jastrow partners of a moved electron on the 16 lanes of its walker:
+// electron p = g + 16 t on lane g for t < TILE_EE, then nucleus g
+constexpr int TILE_EE = 3;
 
 template <class T>
 struct DevSys;

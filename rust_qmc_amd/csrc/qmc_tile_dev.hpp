@@ -283,17 +283,12 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   int my_sh[TILE_MAXSH];
 #pragma unroll
   for (int j = 0; j < TILE_MAXSH; ++j) my_sh[j] = TA.lane_sh[g][j];
-  // this lane's Jastrow partners p = g + 16 t: electrons, nuclei (preloaded) or none
-  T pnx[4], pny[4], pnz[4], pnZ[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int I = g + 16 * t - N;
-    const bool nuc = I >= 0 && I < M_;
-    pnx[t] = nuc ? S.nuc[I][0] : T(0);
-    pny[t] = nuc ? S.nuc[I][1] : T(0);
-    pnz[t] = nuc ? S.nuc[I][2] : T(0);
-    pnZ[t] = nuc ? S.Z[I] : T(0);
-  }
+  // this lane's Jastrow partners: electrons p = g + 16 t (t < TILE_EE), then
+  // nucleus g (preloaded), so that every lane of the group takes the e-e
+  // rounds and then the one e-n round without divergence between the two
+  const bool nuc_on = g < M_;
+  const T pnx = nuc_on ? S.nuc[g][0] : T(0), pny = nuc_on ? S.nuc[g][1] : T(0);
+  const T pnz = nuc_on ? S.nuc[g][2] : T(0), pnZ = nuc_on ? S.Z[g] : T(0);
   // Inverse columns g and g + 16 of the sector that moves live in VGPRs
   // (cur); the other sector's wait in this lane's slots of the LDS park
   // [walker][column][row], swapped at each sector boundary.
@@ -345,6 +340,8 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
       Creg[sp][s] = (kk < KS && ao < TA.A && mo < (sp ? ND : NU)) ? S.moT[sp][ao][mo] : T(0);
     }
   double logdet = E.logdet[wl], sgn = E.sign[wl], ju = E.ju[wl], el = 0.0;
+  [[maybe_unused]] double rprod = 1.0;  // FP64: product of the accepted ratios' |R| ...
+  [[maybe_unused]] int rexp = 0;        // ... times 2^rexp
   const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
   const uint32_t wid = (uint32_t)(A.walker0 + wl);
   const T sigma = (T)A.step_size;
@@ -416,41 +413,34 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           }
           wave_sync();
           // the Jastrow value and gradient of electron e at (x, y, z) over
-          // this lane's partners p = g + 16 t
+          // this lane's partners: electrons g + 16 t, then nucleus g
           auto jpart = [&](T x, T y, T z, T& u, T& jx, T& jy, T& jz) {
             u = jx = jy = jz = T(0);
+            auto add = [&](T dx, T dy, T dz, T r, T ir, T v, T u1) {
+              u += v;
+              const T f = r >= T(1e-10) ? u1 * ir : T(0);
+              jx += f * dx;
+              jy += f * dy;
+              jz += f * dz;
+            };
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < TILE_EE; ++t) {
               const int p = g + 16 * t;
-              T dx, dy, dz, v = T(0), u1 = T(0), r = T(0), ir = T(0);
-              bool on = false;
-              if (p < N) {
-                if (p != e) {
-                  dx = x - pw[3 * p];
-                  dy = y - pw[3 * p + 1];
-                  dz = z - pw[3 * p + 2];
-                  M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
-                  const bool same = spin_of[p] == spin;
-                  T lp;
-                  uee_d<T, Js>(S, r, ir, same, u1, lp);
-                  v = uee_val<T, Js>(S, r, same);
-                  on = true;
-                }
-              } else if (en_on && p < N + M_) {
-                dx = x - pnx[t];
-                dy = y - pny[t];
-                dz = z - pnz[t];
+              if (p < N && p != e) {
+                const T dx = x - pw[3 * p], dy = y - pw[3 * p + 1], dz = z - pw[3 * p + 2];
+                T r, ir, u1, lp;
                 M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
-                tile_uen_vd<Js>(S, r, ir, pnZ[t], p - N, v, u1);
-                on = true;
+                const bool same = spin_of[p] == spin;
+                uee_d<T, Js>(S, r, ir, same, u1, lp);
+                add(dx, dy, dz, r, ir, uee_val<T, Js>(S, r, same), u1);
               }
-              if (on) {
-                u += v;
-                const T f = r >= T(1e-10) ? u1 * ir : T(0);
-                jx += f * dx;
-                jy += f * dy;
-                jz += f * dz;
-              }
+            }
+            if (en_on && nuc_on) {
+              const T dx = x - pnx, dy = y - pny, dz = z - pnz;
+              T r, ir, v, u1;
+              M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
+              tile_uen_vd<Js>(S, r, ir, pnZ, g, v, u1);
+              add(dx, dy, dz, r, ir, v, u1);
             }
           };
           // G: quantum force at r (grad ln|D| from the weighted AO
@@ -570,49 +560,71 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           R = __shfl(hi ? v1 : v0, owner, 64);
           du = T(0);
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
+          for (int t = 0; t < TILE_EE; ++t) {
             const int p = g + 16 * t;
-            if (p < N) {
-              if (p != e) {
-                const T px = pw[3 * p], py = pw[3 * p + 1], pz = pw[3 * p + 2];
-                const bool same = spin_of[p] == spin;
-                const T ax = nx - px, ay = ny - py, az = nz - pz;
-                const T bx = ox - px, by = oy - py, bz = oz - pz;
-                du += uee_diff<T, Js>(S, M<T>::dist(ax * ax + ay * ay + az * az),
-                                      M<T>::dist(bx * bx + by * by + bz * bz), same);
-              }
-            } else if (en_on && p < N + M_) {
-              const T ax = nx - pnx[t], ay = ny - pny[t], az = nz - pnz[t];
-              const T bx = ox - pnx[t], by = oy - pny[t], bz = oz - pnz[t];
-              const T rn = M<T>::dist(ax * ax + ay * ay + az * az), ro = M<T>::dist(bx * bx + by * by + bz * bz);
-              if constexpr (Js::no_series)  // -Z (r' - r) / ((1+b r')(1+b r)), one reciprocal
-                du += -pnZ[t] * (rn - ro) * M<T>::rcp((T(1) + S.b_en * rn) * (T(1) + S.b_en * ro));
-              else
-                du += tile_uen<Js>(S, rn, pnZ[t], p - N) - tile_uen<Js>(S, ro, pnZ[t], p - N);
+            if (p < N && p != e) {
+              const T px = pw[3 * p], py = pw[3 * p + 1], pz = pw[3 * p + 2];
+              const bool same = spin_of[p] == spin;
+              const T ax = nx - px, ay = ny - py, az = nz - pz;
+              const T bx = ox - px, by = oy - py, bz = oz - pz;
+              du += uee_diff<T, Js>(S, M<T>::dist(ax * ax + ay * ay + az * az), M<T>::dist(bx * bx + by * by + bz * bz),
+                                    same);
             }
+          }
+          if (en_on && nuc_on) {
+            const T ax = nx - pnx, ay = ny - pny, az = nz - pnz;
+            const T bx = ox - pnx, by = oy - pny, bz = oz - pnz;
+            const T rn = M<T>::dist(ax * ax + ay * ay + az * az), ro = M<T>::dist(bx * bx + by * by + bz * bz);
+            if constexpr (Js::no_series)  // -Z (r' - r) / ((1+b r')(1+b r)), one reciprocal
+              du += -pnZ * (rn - ro) * M<T>::rcp((T(1) + S.b_en * rn) * (T(1) + S.b_en * ro));
+            else
+              du += tile_uen<Js>(S, rn, pnZ, g) - tile_uen<Js>(S, ro, pnZ, g);
           }
           du = gsum(du);
           pacc = R * R * M<T>::ex(T(2) * du);
         }
         if (U < pacc) {  // uniform within the walker's group
           const T iR = M<T>::rcp(R);
-          const bool own0 = g == (k & 15) && !hi, own1 = g == (k & 15) && hi;
           const T f0 = v0 * iR, f1 = v1 * iR;
+          // column j takes Ainv_j - c f_j, f_j = v_j / R, and the moved
+          // electron's column c becomes c / R: on its owner lane a scale 0
+          // and a factor -1/R make the same fused multiply-add that product
+          const bool own = g == (k & 15);
+          const T sk = own ? T(0) : T(1), fk = own ? -iR : (hi ? f1 : f0);
           // the moved electron's old inverse column: staged in LDS by a
-          // drift-diffusion move, else read from its owner lane
+          // drift-diffusion move, else read from its owner lane; which half
+          // holds it is uniform over the wavefront (hi), so a branch picks
+          // the half instead of a select per element
           const T* ck = lds + TA.o_ck + wt * TILE_CK;
+          auto sm_update = [&](auto H) {
+            constexpr int h = decltype(H)::value;
+            const T fo = h ? f0 : f1;
 #pragma unroll
-          for (int m = 0; m < n; ++m) {
-            const T c = DD ? ck[m] : __shfl(hi ? cur[1][m] : cur[0][m], owner, 64);
-            cur[0][m] = own0 ? c * iR : cur[0][m] - c * f0;
-            cur[1][m] = own1 ? c * iR : cur[1][m] - c * f1;
-          }
+            for (int m = 0; m < n; ++m) {
+              const T c = DD ? ck[m] : __shfl(cur[h][m], owner, 64);
+              cur[h][m] = fma(-c, fk, cur[h][m] * sk);
+              cur[1 - h][m] = fma(-c, fo, cur[1 - h][m]);
+            }
+          };
+          if (hi)
+            sm_update(std::integral_constant<int, 1>{});
+          else
+            sm_update(std::integral_constant<int, 0>{});
           if (g == 0) {
             pw[3 * e] = nx;
             pw[3 * e + 1] = ny;
             pw[3 * e + 2] = nz;
           }
-          logdet += (double)M<T>::lg(R < T(0) ? -R : R);
+          if constexpr (sizeof(T) == 8) {
+            // FP64: ln|D| gains ln|R| through a running product of the
+            // ratios kept as a mantissa in [1/2, 1) and a binary exponent,
+            // one logarithm at the end instead of one per accepted move
+            int ex2;
+            rprod = frexp(rprod * fabs(R), &ex2);
+            rexp += ex2;
+          } else {
+            logdet += (double)M<T>::lg(R < T(0) ? -R : R);
+          }
           if (R < T(0)) sgn = -sgn;
           ju += (double)du;
           ++acc;
@@ -758,6 +770,8 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   park_out(0);
   if (live) {
     if (g == 0) {
+      if constexpr (sizeof(T) == 8)  // + ln(rprod) + rexp ln 2, ln 2 split in two doubles
+        logdet += fma((double)rexp, 0x1.62e42fefa39efp-1, fma((double)rexp, 0x1.abc9e3b39803fp-56, log(rprod)));
       E.logdet[w] = logdet;
       E.sign[w] = sgn;
       E.ju[w] = ju;
