@@ -75,6 +75,17 @@ struct M<float> {
   }
 };
 
+// e^(-a x) of primitive k with exponent a (x = r^2 of a Gaussian, r of a
+// Slater term): FP64 exp_nonpos(-a x); FP32 the hardware 2^y (v_exp_f32) at
+// y = x (-a log2 e), the factor rounded once on the host (prim_nl2e)
+// instead of every argument scaled by log2 e on the device
+__device__ __forceinline__ double prim_e(const DevSys<double>& S, int k, double x) {
+  return M<double>::exn(-S.prim_exp[k] * x);
+}
+__device__ __forceinline__ float prim_e(const DevSys<float>& S, int k, float x) {
+  return __builtin_amdgcn_exp2f(S.prim_nl2e[k] * x);
+}
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11) and Box-Muller normals
 // ---------------------------------------------------------------------------
@@ -206,7 +217,7 @@ __device__ __forceinline__ void mo_values(const DevSys<T>& S, int spin, T x, T y
     if (kind <= SH_P1) {
       T rs = T(0), rp = T(0);
       for (int k = 0; k < np; ++k) {
-        T e = M<T>::exn(-S.prim_exp[p0 + k] * r2);
+        T e = prim_e(S, p0 + k, r2);
         if (kind == SH_S || kind == SH_SP) rs += S.coef_s[p0 + k] * e;
         if (kind != SH_S) rp += S.coef_p[p0 + k] * e;
       }
@@ -231,7 +242,7 @@ __device__ __forceinline__ void mo_values(const DevSys<T>& S, int spin, T x, T y
       for (int k = 0; k < np; ++k) {
         T sp = T(1);
         for (int q = 0; q < sh.pw(k); ++q) sp *= s;
-        v += S.coef_s[p0 + k] * sp * M<T>::exn(-S.prim_exp[p0 + k] * s);
+        v += S.coef_s[p0 + k] * sp * prim_e(S, p0 + k, s);
       }
       mo_add<NS, Bs>(S, spin, a0, v, phi);
     }
@@ -258,7 +269,7 @@ __device__ __forceinline__ void det_terms_w(const DevSys<T>& S, T x, T y, T z, W
       T rs = 0, rsa = 0, rsaa = 0, rp = 0, rpa = 0, rpaa = 0;
       for (int k = 0; k < np; ++k) {
         const T a = S.prim_exp[p0 + k];
-        T e = M<T>::exn(-a * r2);
+        T e = prim_e(S, p0 + k, r2);
         if (kind == SH_S || kind == SH_SP) {
           T cs = S.coef_s[p0 + k] * e;
           rs += cs;
@@ -317,7 +328,7 @@ __device__ __forceinline__ void det_terms_w(const DevSys<T>& S, T x, T y, T z, W
         for (int q = 0; q < p; ++q) sp *= s;
         T spm1 = p >= 1 ? sp / s : T(0);
         T spm2 = p >= 2 ? spm1 / s : T(0);
-        T e = S.coef_s[p0 + k] * M<T>::exn(-zt * s);
+        T e = S.coef_s[p0 + k] * prim_e(S, p0 + k, s);
         d1 += e * (-zt * sp + pf * spm1);
         d2 += e * (zt * zt * sp - T(2) * zt * pf * spm1 + pf * (pf - T(1)) * spm2);
       }
@@ -365,7 +376,7 @@ __device__ __forceinline__ void shell_values(const DevSys<T>& S, int sh, T x, T 
   if (kind <= SH_P1) {
     T rs = T(0), rp = T(0);
     for (int k = 0; k < np; ++k) {
-      T e = M<T>::exn(-S.prim_exp[p0 + k] * r2);
+      T e = prim_e(S, p0 + k, r2);
       rs += S.coef_s[p0 + k] * e;
       rp += S.coef_p[p0 + k] * e;
     }
@@ -390,7 +401,7 @@ __device__ __forceinline__ void shell_values(const DevSys<T>& S, int sh, T x, T 
     for (int k = 0; k < np; ++k) {
       T sp = T(1);
       for (int q = 0; q < S.prim_pow[p0 + k]; ++q) sp *= s;
-      v += S.coef_s[p0 + k] * sp * M<T>::exn(-S.prim_exp[p0 + k] * s);
+      v += S.coef_s[p0 + k] * sp * prim_e(S, p0 + k, s);
     }
     chi[a0] = v;
   }

@@ -340,6 +340,7 @@ int fill_devsys(const HostSys& h, DevSys<T>& D) {
     D.sh_m[nsh] = m;
     for (int k = 0; k < a.nprim; ++k) {
       D.prim_exp[np + k] = (T)h.pexp[a.prim0 + k];
+      D.prim_nl2e[np + k] = (T)(-h.pexp[a.prim0 + k] * 1.4426950408889634);  // log2(e)
       const bool s_part = (kind == SH_S || kind == SH_SP || kind == SH_STO || kind == SH_SLATER1S);
       D.coef_s[np + k] = s_part ? (T)h.pcoef[a.prim0 + k] : (T)0;
       D.coef_p[np + k] = pa ? (T)h.pcoef[pa->prim0 + k] : (T)0;

@@ -47,6 +47,7 @@ struct DevSys {
   double vnn;
   int sh_kind[MAXSH], sh_center[MAXSH], sh_nprim[MAXSH], sh_prim0[MAXSH], sh_ao0[MAXSH], sh_m[MAXSH];
   T prim_exp[MAXPRIM], coef_s[MAXPRIM], coef_p[MAXPRIM];
+  T prim_nl2e[MAXPRIM];  // -prim_exp log2(e), rounded once: the FP32 primitives' 2^y argument factor
   int prim_pow[MAXPRIM];
   int sector_elec[2][MAXE];   // global index of the k-th electron of each spin sector
   int elec_spin[MAXE];        // 0 up, 1 down
