@@ -13,9 +13,9 @@
 namespace qmc {
 
 // ---------------------------------------------------------------------------
-// precision-dependent math: FP64 uses the library routines, the primitives'
-// exponentials exp_nonpos (qmc_expn.hpp) and the reciprocals Newton-refined
-// hardware estimates; FP32 uses the hardware transcendental instructions.
+// precision-dependent math: FP64 uses the library routines, the
+// exponentials of qmc_expn.hpp and the reciprocals Newton-refined hardware
+// estimates; FP32 uses the hardware transcendental instructions.
 // ---------------------------------------------------------------------------
 template <class T>
 struct M;
@@ -23,7 +23,10 @@ struct M;
 // the distance forms of a squared distance d2 >= 0.
 template <>
 struct M<double> {
-  static __device__ __forceinline__ double ex(double x) { return exp(x); }
+  // e^x of an acceptance exponent (either sign): exp_clamped, the library
+  // exponential's reduction without its range checks (NaN gives 0, a
+  // rejection, as the library's NaN does)
+  static __device__ __forceinline__ double ex(double x) { return exp_clamped(x); }
   // e^x of a primitive's argument (finite, not positive): exp_nonpos, the
   // library exponential without its range checks
   static __device__ __forceinline__ double exn(double x) { return exp_nonpos(x); }
@@ -124,29 +127,26 @@ template <class T>
 struct Rng;
 template <>
 struct Rng<double> {
-  static __device__ __forceinline__ void normals3(U4 o, double z[3]) {
-    const double twopi = 6.283185307179586;
-    double u1 = u01d(o.x), u2 = u01d(o.y), u3 = u01d(o.z), u4 = u01d(o.w);
-    double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
-    double s, c;
-    sincos(twopi * u2, &s, &c);
+  // the Box-Muller angles 2 pi u from sincos_turns (exact quarter-turn
+  // reduction of u, no rounding of 2 pi u) instead of the library's sincos
+  static __device__ __forceinline__ void box_muller(double u1, double u2, double u3, double u4, double z[3]) {
+    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    double s, c, s4, c4;
+    sincos_turns(u2, s, c);
+    sincos_turns(u4, s4, c4);
     z[0] = r1 * c;
     z[1] = r1 * s;
-    z[2] = r2 * cos(twopi * u4);
+    z[2] = r2 * c4;
+  }
+  static __device__ __forceinline__ void normals3(U4 o, double z[3]) {
+    box_muller(u01d(o.x), u01d(o.y), u01d(o.z), u01d(o.w), z);
   }
   static __device__ __forceinline__ double uniform(uint32_t w) { return u01d(w); }
   // One single-electron move from one Philox block: three Box-Muller normals
   // from the top 24 bits of the four words and the acceptance uniform from
   // their low bytes (independent bits), as the FP32 path does.
   static __device__ __forceinline__ void move(U4 o, double z[3], double& U) {
-    const double twopi = 6.283185307179586;
-    double u1 = u24d(o.x), u2 = u24d(o.y), u3 = u24d(o.z), u4 = u24d(o.w);
-    double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
-    double s, c;
-    sincos(twopi * u2, &s, &c);
-    z[0] = r1 * c;
-    z[1] = r1 * s;
-    z[2] = r2 * cos(twopi * u4);
+    box_muller(u24d(o.x), u24d(o.y), u24d(o.z), u24d(o.w), z);
     U = u01d(low_bytes(o));
   }
 };
