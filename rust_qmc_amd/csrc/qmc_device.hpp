@@ -78,6 +78,33 @@ struct M<float> {
   }
 };
 
+// ln|D| gains ln|R| per accepted move.  FP64 keeps a running product of
+// the ratios as a mantissa in [1/2, 1) and a binary exponent (frexp) and
+// folds it into ln|D| with one logarithm; FP32 adds the hardware logarithm
+// per move.
+template <class T>
+struct LogAcc;
+template <>
+struct LogAcc<double> {
+  double m = 1.0;
+  int e = 0;
+  __device__ __forceinline__ void add(double&, double R) {
+    int k;
+    m = frexp(m * fabs(R), &k);
+    e += k;
+  }
+  __device__ __forceinline__ void fold(double& ld) {  // + ln m + e ln 2, ln 2 split in two doubles
+    ld += fma((double)e, 0x1.62e42fefa39efp-1, fma((double)e, 0x1.abc9e3b39803fp-56, log(m)));
+    m = 1.0;
+    e = 0;
+  }
+};
+template <>
+struct LogAcc<float> {
+  __device__ __forceinline__ void add(double& ld, float R) { ld += (double)__logf(R < 0.0f ? -R : R); }
+  __device__ __forceinline__ void fold(double&) {}
+};
+
 // e^(-a x) of primitive k with exponent a (x = r^2 of a Gaussian, r of a
 // Slater term): FP64 exp_nonpos(-a x); FP32 the hardware 2^y (v_exp_f32) at
 // y = x (-a log2 e), the factor rounded once on the host (prim_nl2e)
@@ -128,9 +155,11 @@ struct Rng;
 template <>
 struct Rng<double> {
   // the Box-Muller angles 2 pi u from sincos_turns (exact quarter-turn
-  // reduction of u, no rounding of 2 pi u) instead of the library's sincos
+  // reduction of u, no rounding of 2 pi u) instead of the library's sincos;
+  // the radii sqrt(-2 ln u) as x rsq(x) (x > 0 for u in (0, 1))
   static __device__ __forceinline__ void box_muller(double u1, double u2, double u3, double u4, double z[3]) {
-    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    const double x1 = -2.0 * log(u1), x3 = -2.0 * log(u3);
+    const double r1 = x1 * M<double>::rsq(x1), r2 = x3 * M<double>::rsq(x3);
     double s, c, s4, c4;
     sincos_turns(u2, s, c);
     sincos_turns(u4, s4, c4);

@@ -362,6 +362,7 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
                                              T uen[Cap<NA>::v], uint32_t w, uint32_t step, uint32_t k0,
                                              uint32_t k1, T sigma, double& logdet, double& sgn, double& ju,
                                              int& acc) {
+  LogAcc<T> lacc;
 #pragma unroll 1
   for (int k = 0; k < NA; ++k) {
     const uint32_t e = (uint32_t)S.sector_elec[spin][k];
@@ -411,7 +412,7 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
       ra[0][0] = nx;
       ra[0][1] = ny;
       ra[0][2] = nz;
-      logdet += (double)M<T>::lg(R < T(0) ? -R : R);
+      lacc.add(logdet, R);
       if (R < T(0)) sgn = -sgn;
       ju += (double)du;
       ++acc;
@@ -420,6 +421,7 @@ __device__ __forceinline__ void sector_moves(const DevSys<T>& S, int spin, T ra[
     rotate<NA>(ra, ai);
     rotate_vec<NA>(uen);
   }
+  lacc.fold(logdet);
 }
 
 // wave-wide sum (deterministic butterfly order)
@@ -613,6 +615,7 @@ __device__ __forceinline__ void sector_moves_dd(const DevSys<T>& S, int spin, T 
                                                 T max_drift, double& logdet, double& sgn, double& ju, int& acc) {
   const T s2t = M<T>::sq(T(2) * tau), i4t = T(0.25) / tau;
   const T gs = S.grad_scale;
+  LogAcc<T> lacc;
 #pragma unroll 1
   for (int k = 0; k < NA; ++k) {
     const uint32_t e = (uint32_t)S.sector_elec[spin][k];
@@ -673,13 +676,14 @@ __device__ __forceinline__ void sector_moves_dd(const DevSys<T>& S, int spin, T 
       ra[0][0] = nx;
       ra[0][1] = ny;
       ra[0][2] = nz;
-      logdet += (double)M<T>::lg(R < T(0) ? -R : R);
+      lacc.add(logdet, R);
       if (R < T(0)) sgn = -sgn;
       ju += (double)du;
       ++acc;
     }
     rotate<NA>(ra, ai);
   }
+  lacc.fold(logdet);
 }
 
 // n_steps drift-diffusion sweeps of block blk, as sweep_block

@@ -340,8 +340,7 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
       Creg[sp][s] = (kk < KS && ao < TA.A && mo < (sp ? ND : NU)) ? S.moT[sp][ao][mo] : T(0);
     }
   double logdet = E.logdet[wl], sgn = E.sign[wl], ju = E.ju[wl], el = 0.0;
-  [[maybe_unused]] double rprod = 1.0;  // FP64: product of the accepted ratios' |R| ...
-  [[maybe_unused]] int rexp = 0;        // ... times 2^rexp
+  LogAcc<T> lacc;  // ln|R| of the accepted moves, folded into ln|D| at the end
   const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
   const uint32_t wid = (uint32_t)(A.walker0 + wl);
   const T sigma = (T)A.step_size;
@@ -615,16 +614,7 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
             pw[3 * e + 1] = ny;
             pw[3 * e + 2] = nz;
           }
-          if constexpr (sizeof(T) == 8) {
-            // FP64: ln|D| gains ln|R| through a running product of the
-            // ratios kept as a mantissa in [1/2, 1) and a binary exponent,
-            // one logarithm at the end instead of one per accepted move
-            int ex2;
-            rprod = frexp(rprod * fabs(R), &ex2);
-            rexp += ex2;
-          } else {
-            logdet += (double)M<T>::lg(R < T(0) ? -R : R);
-          }
+          lacc.add(logdet, R);
           if (R < T(0)) sgn = -sgn;
           ju += (double)du;
           ++acc;
@@ -770,8 +760,7 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   park_out(0);
   if (live) {
     if (g == 0) {
-      if constexpr (sizeof(T) == 8)  // + ln(rprod) + rexp ln 2, ln 2 split in two doubles
-        logdet += fma((double)rexp, 0x1.62e42fefa39efp-1, fma((double)rexp, 0x1.abc9e3b39803fp-56, log(rprod)));
+      lacc.fold(logdet);
       E.logdet[w] = logdet;
       E.sign[w] = sgn;
       E.ju[w] = ju;
