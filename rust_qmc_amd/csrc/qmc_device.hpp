@@ -156,9 +156,10 @@ template <>
 struct Rng<double> {
   // the Box-Muller angles 2 pi u from sincos_turns (exact quarter-turn
   // reduction of u, no rounding of 2 pi u) instead of the library's sincos;
-  // the radii sqrt(-2 ln u) as x rsq(x) (x > 0 for u in (0, 1))
+  // the radii sqrt(-2 ln u) from log_unit and x rsq(x) (x > 0 for u in
+  // (0, 1))
   static __device__ __forceinline__ void box_muller(double u1, double u2, double u3, double u4, double z[3]) {
-    const double x1 = -2.0 * log(u1), x3 = -2.0 * log(u3);
+    const double x1 = -2.0 * log_unit(u1), x3 = -2.0 * log_unit(u3);
     const double r1 = x1 * M<double>::rsq(x1), r2 = x3 * M<double>::rsq(x3);
     double s, c, s4, c4;
     sincos_turns(u2, s, c);

@@ -28,6 +28,16 @@
 // result (tests/test_expn.py checks it against an extended-precision
 // reference on the host).
 //
+// log_unit gives ln x for x in (0, 1] (the Box-Muller radii's ln u): x =
+// m 2^e with m in [sqrt(1/2), sqrt(2)) (frexp), f = m - 1 exact, then
+// ln(1 + f) = 2 atanh(s) with s = f / (2 + f), evaluated as
+// f - (f^2/2 - s (f^2/2 + R(s^2))) where R(z) = sum_k 2 z^k / (2k + 1) is
+// the atanh series to k = 10 (truncation below 1e-19 relative), and
+// e ln 2 added with ln 2 split into a 32-bit head (e ln2_hi exact) and a
+// tail (the decomposition of 4.3BSD and fdlibm's log, with the series in
+// place of their minimax polynomial).  Within 1 ulp of the library's log
+// on the host (tests/test_expn.py).
+//
 // The includer defines QMC_EXPN_QUAL (the function qualifiers, e.g.
 // "__host__ __device__ __forceinline__") before including this header.
 #pragma once
@@ -61,6 +71,31 @@ QMC_EXPN_QUAL double exp_core(double x) {
 QMC_EXPN_QUAL double exp_nonpos(double x) { return exp_core(fmax(x, -2000.0)); }
 
 QMC_EXPN_QUAL double exp_clamped(double x) { return exp_core(fmin(fmax(x, -2000.0), 2000.0)); }
+
+QMC_EXPN_QUAL double log_unit(double x) {
+  int e;
+  double m = frexp(x, &e);  // x = m 2^e, m in [1/2, 1)
+  if (m < 0x1.6a09e667f3bcdp-1) {  // sqrt(1/2)
+    m *= 2.0;
+    --e;
+  }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f), z = s * s;
+  double r = 0x1.8618618618618p-4;      // 2/21
+  r = fma(r, z, 0x1.af286bca1af28p-4);  // 2/19
+  r = fma(r, z, 0x1.e1e1e1e1e1e1ep-4);  // 2/17
+  r = fma(r, z, 0x1.1111111111111p-3);  // 2/15
+  r = fma(r, z, 0x1.3b13b13b13b14p-3);  // 2/13
+  r = fma(r, z, 0x1.745d1745d1746p-3);  // 2/11
+  r = fma(r, z, 0x1.c71c71c71c71cp-3);  // 2/9
+  r = fma(r, z, 0x1.2492492492492p-2);  // 2/7
+  r = fma(r, z, 0x1.999999999999ap-2);  // 2/5
+  r = fma(r, z, 0x1.5555555555555p-1);  // 2/3
+  r *= z;
+  const double hf = 0.5 * f * f, de = (double)e;
+  // e ln2_hi + (f - (hf - (s (hf + r) + e ln2_lo)))
+  return fma(de, 0x1.62e42ff000000p-1, f - (hf - fma(s, hf + r, de * -0x1.718432a1b0e26p-35)));
+}
 
 QMC_EXPN_QUAL void sincos_turns(double u, double& sn, double& cs) {
   const double q = rint(4.0 * u);

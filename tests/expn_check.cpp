@@ -12,6 +12,9 @@
 // (whose own error, from rounding 2 pi u, is about 1e-16 absolute): over
 // uniform u in [0, 1], the 24- and 32-bit grids of the Box-Muller uniforms
 // and the quarter and eighth turns.
+// Line 4: log_unit(x), the largest distance in ulps from the library's log
+// over uniform x in (0, 1], the 24- and 32-bit uniform grids, log-uniform x
+// down to the subnormals and special points, then log_unit(1).
 #define QMC_EXPN_QUAL inline
 #include "qmc_expn.hpp"
 
@@ -98,5 +101,29 @@ int main() {
   }
   for (int j = 0; j <= 8; ++j) check_sc(j / 8.0);
   std::printf("%ld %lld %.17g %.3g\n", n, (long long)worst, worst_x, worst_abs);
+  // log_unit
+  f = qmc::log_unit;
+  worst = 0;
+  worst_x = 0.0;
+  n = 0;
+  auto check_log = [&](double x) {
+    int64_t d = ordered(qmc::log_unit(x)) - ordered(std::log(x));
+    if (d < 0) d = -d;
+    if (d > worst) {
+      worst = d;
+      worst_x = x;
+    }
+    ++n;
+  };
+  for (int i = 0; i < 1000000; ++i) {
+    check_log(u(gen));
+    check_log(((double)(gen() >> 40) + 0.5) * 5.9604644775390625e-08);  // 24-bit grid
+    check_log(((double)(gen() >> 32) + 0.5) * 2.3283064365386963e-10);  // 32-bit grid
+    check_log(std::pow(2.0, -1074.0 * u(gen)));                          // log-uniform down to subnormals
+  }
+  const double special3[] = {1.0, 0.5, 0.70710678118654746, 0.70710678118654757, 1.0 - 0x1p-53, 0x1p-1074,
+                             0x1p-1022, 0.1, 2.9802322387695312e-08};
+  for (double x : special3) check_log(x);
+  std::printf("%ld %lld %.17g %.17g\n", n, (long long)worst, worst_x, qmc::log_unit(1.0));
   return 0;
 }

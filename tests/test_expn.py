@@ -5,7 +5,8 @@ the underflow threshold and at -inf; exp_clamped (acceptance exponents of
 either sign) within 1 ulp over [-745.2, 709.78], infinite past the overflow
 threshold and at +inf, zero at -inf and at NaN; sincos_turns (Box-Muller
 angles) within 1 ulp of an extended-precision sin and cos of the exactly
-reduced angle and within 1e-15 of the library's sin and cos of 2 pi u."""
+reduced angle and within 1e-15 of the library's sin and cos of 2 pi u;
+log_unit (Box-Muller radii) within 1 ulp of the library's log over (0, 1]."""
 import os
 import shutil
 import subprocess
@@ -40,3 +41,8 @@ def test_fp64_exponentials_within_one_ulp_of_libm():
     assert n == 3000009
     assert worst <= 1, (worst, worst_x)
     assert worst_abs < 1e-15
+    out = lines[3].split()
+    n, worst, worst_x, at_one = int(out[0]), int(out[1]), float(out[2]), float(out[3])
+    assert n == 4000009
+    assert worst <= 1, (worst, worst_x)
+    assert at_one == 0.0
