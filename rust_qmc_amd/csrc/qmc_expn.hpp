@@ -34,8 +34,8 @@
 // f - (f^2/2 - s (f^2/2 + R(s^2))) where R(z) = sum_k 2 z^k / (2k + 1) is
 // the atanh series to k = 10 (truncation below 1e-19 relative), and
 // e ln 2 added with ln 2 split into a 32-bit head (e ln2_hi exact) and a
-// tail (the decomposition of 4.3BSD and fdlibm's log, with the series in
-// place of their minimax polynomial).  Within 1 ulp of the library's log
+// tail (the decomposition of fdlibm's log, Sun Microsystems 1993, with the
+// series in place of its minimax polynomial).  Within 1 ulp of the library's log
 // on the host (tests/test_expn.py).
 //
 // The includer defines QMC_EXPN_QUAL (the function qualifiers, e.g.
