@@ -19,6 +19,8 @@
 //      Sherman-Morrison update of the owned columns.
 // The local energy after each sweep puts the electron of sector slot k on
 // lane k mod 16, the owner of its inverse column.
+#include <algorithm>
+
 #include "qmc_tile_dev.hpp"
 
 namespace qmc {
@@ -94,7 +96,12 @@ bool tile_plan(const DevSys<float>& D, TilePlan& P) {
     const int slot = i / 16, r = i % 16;
     if (slot >= TILE_MAXSH) return false;
     P.lane_sh[slot & 1 ? 15 - r : r][slot] = order[i];
+    P.slot_np[slot] = std::max(P.slot_np[slot], D.sh_nprim[order[i]]);
+    P.n_slot = slot + 1;
   }
+  P.gauss = true;
+  for (int i = 0; i < D.n_shell; ++i)
+    if (D.sh_kind[i] != SH_S && D.sh_kind[i] != SH_SP && D.sh_kind[i] != SH_P) P.gauss = false;
   P.ok = true;
   return true;
 }

@@ -22,11 +22,16 @@ struct TilePlan {
   int N = 0, nu = 0, nd = 0, A = 0, n_nuc = 0, n_shell = 0;
   bool no_series = true;
   int lane_sh[16][TILE_MAXSH];  // shells of group lane g (-1: none)
+  // every shell a contracted Gaussian of kind s, p or sp: the AO values of
+  // a move take the branch-free path, slot j looping over slot_np[j]
+  // primitives (the slot's largest count; records are zero-padded)
+  bool gauss = false;
+  int n_slot = 0, slot_np[TILE_MAXSH] = {};
 };
 
 // fills P; false when no tile kernel fits the system (21 + 21 electrons,
-// at most 68 AOs, electrons + nuclei <= 64, at most 64 shells of at most
-// 6 primitives)
+// at most 68 AOs, at most 48 electrons and 16 nuclei, at most 64 shells of
+// at most 6 primitives)
 bool tile_plan(const DevSys<float>& D, TilePlan& P);
 
 // single-electron sweeps of an ensemble with FP32 state (mixed precision,

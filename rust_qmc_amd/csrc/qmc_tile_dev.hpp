@@ -43,9 +43,9 @@ constexpr int TILE_MO = 32;   // MO columns of Phi (two 16-wide MFMA tiles)
 // same bank.
 constexpr int TILE_PS = 33;
 constexpr int TILE_MAXP = 6;  // primitives per shell
-// elements per shell record in LDS: 32 used, an odd stride so that the 16
-// lanes of a group, reading the same field of 16 different records, hit 16
-// different banks (a stride of 32 elements put them all in one bank)
+// elements per shell record in LDS: 33 fields, an odd stride so that the
+// 16 lanes of a group, reading the same field of 16 different records, hit
+// 16 different banks (a stride of 32 elements put them all in one bank)
 constexpr int TILE_REC = 33;
 // row stride of an inverse column or MO-coefficient row in LDS: odd, so
 // the 16 lanes of a group reading rows a = g + 16 j hit 16 different banks
@@ -56,6 +56,7 @@ constexpr int TILE_WS = 16 * TILE_AJ;  // row stride of a walker's AO weights
 struct TileArgs {
   int N, A, XS, n_nuc, n_shell;
   int lane_sh[16][TILE_MAXSH];  // shells evaluated by group lane g (-1: none)
+  int gauss, n_slot, slot_np[TILE_MAXSH];  // the Gaussian path of a move's AO values (TilePlan)
   // LDS carve-out in elements of the state type; o_cmo, o_ck and o_w only
   // in a drift-diffusion launch
   int o_pos, o_x, o_phi, o_spin, o_sh, o_sum, o_park, o_rng, o_cmo, o_ck, o_w;
@@ -81,10 +82,12 @@ __device__ __forceinline__ int rec_int(float v) { return as_int(v); }
 __device__ __forceinline__ int rec_int(double v) { return (int)v; }
 
 // One shell record: kind, first AO, primitive count, P1 component (ints),
-// centre, then exponents, s and p coefficients and STO powers of up to
-// TILE_MAXP primitives (the DevSys shell and primitive tables, per shell).
+// centre, the Gaussian path's s destination, then exponents, s and p
+// coefficients and STO powers of up to TILE_MAXP primitives (the DevSys
+// shell and primitive tables, per shell), and the Gaussian path's p
+// destination.
 template <class T>
-__device__ __forceinline__ void tile_write_record(const DevSys<T>& S, int sh, T* rec) {
+__device__ __forceinline__ void tile_write_record(const DevSys<T>& S, int sh, T* rec, int pad) {
   rec[0] = rec_of(S.sh_kind[sh], T());
   rec[1] = rec_of(S.sh_ao0[sh], T());
   rec[2] = rec_of(S.sh_nprim[sh], T());
@@ -93,7 +96,6 @@ __device__ __forceinline__ void tile_write_record(const DevSys<T>& S, int sh, T*
   rec[4] = S.nuc[c][0];
   rec[5] = S.nuc[c][1];
   rec[6] = S.nuc[c][2];
-  rec[7] = T(0);
   for (int k = 0; k < TILE_MAXP; ++k) {
     const bool v = k < np;
     rec[8 + k] = v ? S.prim_exp[p0 + k] : T(0);
@@ -101,6 +103,43 @@ __device__ __forceinline__ void tile_write_record(const DevSys<T>& S, int sh, T*
     rec[20 + k] = v ? S.coef_p[p0 + k] : T(0);
     rec[26 + k] = rec_of(v ? S.prim_pow[p0 + k] : 0, T());
   }
+  // the Gaussian path's destinations in a row of X (pad: its last column,
+  // whose MO coefficients are zero): the s value's, and the first of the
+  // p values' (s shells write zeros to the pad)
+  const int kind = S.sh_kind[sh], a0 = S.sh_ao0[sh];
+  rec[7] = rec_of(kind == SH_P ? pad : a0, T());
+  rec[32] = rec_of(kind == SH_S ? pad : (kind == SH_SP ? a0 + 1 : a0), T());
+}
+
+// the Gaussian path's record for lanes without a shell in a slot: zero
+// coefficients, both destinations the pad
+template <class T>
+__device__ __forceinline__ void tile_null_record(T* rec, int pad) {
+  for (int i = 0; i < TILE_REC; ++i) rec[i] = T(0);
+  rec[0] = rec_of(SH_S, T());
+  rec[7] = rec_of(pad, T());
+  rec[32] = rec_of(pad, T());
+}
+
+// AO values of one s, p or sp shell from its record, branch-free: npu
+// primitives (the slot's count, zero-padded records), the s value to
+// rec[7] and the p values to rec[32] + 0, 1, 2 (+ 0, 0, 0 for s shells,
+// whose p part is zero); the same arithmetic as tile_shell_values
+template <class T>
+__device__ __forceinline__ void tile_gauss_values(const T* rec, T x, T y, T z, T* chi, int npu) {
+  const T dx = x - rec[4], dy = y - rec[5], dz = z - rec[6];
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  T rs = T(0), rp = T(0);
+  for (int k = 0; k < npu; ++k) {
+    const T e = M<T>::exn(-rec[8 + k] * r2);
+    rs += rec[14 + k] * e;
+    rp += rec[20 + k] * e;
+  }
+  const int sd = rec_int(rec[7]), pd = rec_int(rec[32]), ps = rec_int(rec[0]) == SH_S ? 0 : 1;
+  chi[sd] = rs;
+  chi[pd] = dx * rp;
+  chi[pd + ps] = dy * rp;
+  chi[pd + 2 * ps] = dz * rp;
 }
 
 // AO values of one shell from its LDS record, written to chi[ao]; the same
@@ -270,7 +309,8 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   }
   for (int i = g; i < TA.XS; i += 16) X[wt * TA.XS + i] = T(0);
   for (int i = tid; i < N; i += TILE_T) spin_of[i] = S.elec_spin[i];
-  for (int i = tid; i < TA.n_shell; i += TILE_T) tile_write_record(S, i, shrec + i * TILE_REC);
+  for (int i = tid; i < TA.n_shell; i += TILE_T) tile_write_record(S, i, shrec + i * TILE_REC, TA.XS - 1);
+  if (tid == 0) tile_null_record(shrec + TA.n_shell * TILE_REC, TA.XS - 1);
   if constexpr (DD) {  // MO coefficients, [spin][ao][mo] with rows of TILE_CK
     static_assert(NU <= TILE_CK && ND <= TILE_CK, "sector too large for the AO-weight rows");
     T* cmo = lds + TA.o_cmo;
@@ -515,9 +555,17 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           ny = oy + sigma * z[1];
           nz = oz + sigma * z[2];
           // B: AO values of the proposal
+          if (TA.gauss) {  // uniform branches and loop counts only
 #pragma unroll
-          for (int j = 0; j < TILE_MAXSH; ++j)
-            if (my_sh[j] >= 0) tile_shell_values(shrec + my_sh[j] * TILE_REC, nx, ny, nz, X + wt * TA.XS);
+            for (int j = 0; j < TILE_MAXSH; ++j)
+              if (j < TA.n_slot)
+                tile_gauss_values(shrec + (my_sh[j] >= 0 ? my_sh[j] : TA.n_shell) * TILE_REC, nx, ny, nz,
+                                  X + wt * TA.XS, TA.slot_np[j]);
+          } else {
+#pragma unroll
+            for (int j = 0; j < TILE_MAXSH; ++j)
+              if (my_sh[j] >= 0) tile_shell_values(shrec + my_sh[j] * TILE_REC, nx, ny, nz, X + wt * TA.XS);
+          }
         }
         __syncthreads();
         // C: Phi[half][walker][mo] = sum over the half's AOs of X[walker][ao] C[mo][ao]
@@ -781,6 +829,9 @@ inline TileArgs make_tile_args(const TilePlan& P, int esz = (int)sizeof(float), 
   T.n_shell = P.n_shell;
   for (int g = 0; g < 16; ++g)
     for (int j = 0; j < TILE_MAXSH; ++j) T.lane_sh[g][j] = P.lane_sh[g][j];
+  T.gauss = P.gauss ? 1 : 0;
+  T.n_slot = P.n_slot;
+  for (int j = 0; j < TILE_MAXSH; ++j) T.slot_np[j] = P.slot_np[j];
   const int per8 = 8 / esz;  // elements per double
   T.o_pos = 0;
   T.o_x = T.o_pos + TILE_W * 3 * P.N;
@@ -791,7 +842,7 @@ inline TileArgs make_tile_args(const TilePlan& P, int esz = (int)sizeof(float), 
   const int nx = P.nu > P.nd ? P.nu : P.nd;
   const int per16 = 16 / esz;  // the park and the drift-diffusion rows start 16-byte aligned
   auto al16 = [&](int o) { return (o + per16 - 1) / per16 * per16; };
-  T.o_park = al16(T.o_sh + P.n_shell * TILE_REC);
+  T.o_park = al16(T.o_sh + (P.n_shell + 1) * TILE_REC);  // the shells' records and the null record
   T.o_rng = al16(T.o_park + TILE_W * nx * nx);
   T.o_cmo = al16(T.o_rng + TILE_W * 4 * nx);
   T.o_ck = T.o_cmo + (dd ? 2 * P.A * TILE_CK : 0);
