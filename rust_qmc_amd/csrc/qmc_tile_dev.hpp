@@ -241,6 +241,39 @@ __device__ __forceinline__ void tile_shell_grad(const T* rec, T x, T y, T z, F f
   f(a0, v, gs * dx, gs * dy, gs * dz);
 }
 
+// values and gradients of one s, p or sp shell from its record, branch-free
+// as tile_gauss_values: f(ao, value, d/dx, d/dy, d/dz) for the s part at
+// rec[7] and the three p components at rec[32] + 0, 1, 2 (s shells and the
+// null record hand zeros to the pad); the same arithmetic as
+// tile_shell_grad, in the same order
+template <class T, class F>
+__device__ __forceinline__ void tile_gauss_grad(const T* rec, T x, T y, T z, int npu, F f) {
+  const T dx = x - rec[4], dy = y - rec[5], dz = z - rec[6];
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  T rs = T(0), rsa = T(0), rp = T(0), rpa = T(0);
+  for (int k = 0; k < npu; ++k) {
+    const T a = rec[8 + k];
+    const T e = M<T>::exn(-a * r2);
+    const T cs = rec[14 + k] * e, cp = rec[20 + k] * e;
+    rs += cs;
+    rsa += cs * a;
+    rp += cp;
+    rpa += cp * a;
+  }
+  const int sd = rec_int(rec[7]), pd = rec_int(rec[32]), ps = rec_int(rec[0]) == SH_S ? 0 : 1;
+  const T t = T(-2) * rsa;  // grad = -2 Ra d
+  f(sd, rs, t * dx, t * dy, t * dz);
+  // p_m = d_m R: grad_j = delta_jm R - 2 d_m Ra d_j
+  auto pm = [&](int ao, int m) {
+    const T dm = m == 0 ? dx : (m == 1 ? dy : dz);
+    const T u = T(2) * dm * rpa;
+    f(ao, dm * rp, (m == 0 ? rp : T(0)) - u * dx, (m == 1 ? rp : T(0)) - u * dy, (m == 2 ? rp : T(0)) - u * dz);
+  };
+  pm(pd, 0);
+  pm(pd + ps, 1);
+  pm(pd + 2 * ps, 2);
+}
+
 // e-n Jastrow value and u'(r) with the charge already in a register
 // (uen_val, uen_d)
 template <class Js, class T>
@@ -311,6 +344,10 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
   for (int i = tid; i < N; i += TILE_T) spin_of[i] = S.elec_spin[i];
   for (int i = tid; i < TA.n_shell; i += TILE_T) tile_write_record(S, i, shrec + i * TILE_REC, TA.XS - 1);
   if (tid == 0) tile_null_record(shrec + TA.n_shell * TILE_REC, TA.XS - 1);
+  if constexpr (DD) {
+    // AO weights past the AOs stay zero (the Gaussian path reads the pad's)
+    for (int i = TA.A + g; i < TILE_WS; i += 16) lds[TA.o_w + wt * TILE_WS + i] = T(0);
+  }
   if constexpr (DD) {  // MO coefficients, [spin][ao][mo] with rows of TILE_CK
     static_assert(NU <= TILE_CK && ND <= TILE_CK, "sector too large for the AO-weight rows");
     T* cmo = lds + TA.o_cmo;
@@ -487,15 +524,23 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           // r' the ratio R = sum w chi, grad D'/D = sum w grad chi / R and the
           // Jastrow terms; chi(r') goes to X for the update
           T gx = T(0), gy = T(0), gz = T(0);
+          auto grad_at_r = [&](int a, T, T ddx, T ddy, T ddz) {
+            const T wq = wa[a];
+            gx += wq * ddx;
+            gy += wq * ddy;
+            gz += wq * ddz;
+          };
+          if (TA.gauss) {
 #pragma unroll
-          for (int j = 0; j < TILE_MAXSH; ++j)
-            if (my_sh[j] >= 0)
-              tile_shell_grad(shrec + my_sh[j] * TILE_REC, ox, oy, oz, [&](int a, T, T ddx, T ddy, T ddz) {
-                const T wq = wa[a];
-                gx += wq * ddx;
-                gy += wq * ddy;
-                gz += wq * ddz;
-              });
+            for (int j = 0; j < TILE_MAXSH; ++j)
+              if (j < TA.n_slot)
+                tile_gauss_grad(shrec + (my_sh[j] >= 0 ? my_sh[j] : TA.n_shell) * TILE_REC, ox, oy, oz,
+                                TA.slot_np[j], grad_at_r);
+          } else {
+#pragma unroll
+            for (int j = 0; j < TILE_MAXSH; ++j)
+              if (my_sh[j] >= 0) tile_shell_grad(shrec + my_sh[j] * TILE_REC, ox, oy, oz, grad_at_r);
+          }
           T uo, jx, jy, jz;
           jpart(ox, oy, oz, uo, jx, jy, jz);
           gx = rsum(gx);
@@ -515,17 +560,25 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           nz = oz + dold[2] + s2t * z[2];
           T R = T(0), hx = T(0), hy = T(0), hz = T(0);
           T* xw = X + wt * TA.XS;
+          auto at_rp = [&](int a, T v, T ddx, T ddy, T ddz) {
+            const T wq = wa[a];
+            xw[a] = v;
+            R += wq * v;
+            hx += wq * ddx;
+            hy += wq * ddy;
+            hz += wq * ddz;
+          };
+          if (TA.gauss) {
 #pragma unroll
-          for (int j = 0; j < TILE_MAXSH; ++j)
-            if (my_sh[j] >= 0)
-              tile_shell_grad(shrec + my_sh[j] * TILE_REC, nx, ny, nz, [&](int a, T v, T ddx, T ddy, T ddz) {
-                const T wq = wa[a];
-                xw[a] = v;
-                R += wq * v;
-                hx += wq * ddx;
-                hy += wq * ddy;
-                hz += wq * ddz;
-              });
+            for (int j = 0; j < TILE_MAXSH; ++j)
+              if (j < TA.n_slot)
+                tile_gauss_grad(shrec + (my_sh[j] >= 0 ? my_sh[j] : TA.n_shell) * TILE_REC, nx, ny, nz,
+                                TA.slot_np[j], at_rp);
+          } else {
+#pragma unroll
+            for (int j = 0; j < TILE_MAXSH; ++j)
+              if (my_sh[j] >= 0) tile_shell_grad(shrec + my_sh[j] * TILE_REC, nx, ny, nz, at_rp);
+          }
           T un, vx, vy, vz;
           jpart(nx, ny, nz, un, vx, vy, vz);
           R = rsum(R);
