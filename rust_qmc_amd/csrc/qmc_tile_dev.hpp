@@ -490,33 +490,34 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           wave_sync();
           // the Jastrow value and gradient of electron e at (x, y, z) over
           // this lane's partners: electrons g + 16 t, then nucleus g
+          // (branch-free: every lane evaluates its partner slots, a pair that
+          // does not count -- the electron itself, past the electrons, no
+          // nucleus -- adds a selected zero)
           auto jpart = [&](T x, T y, T z, T& u, T& jx, T& jy, T& jz) {
             u = jx = jy = jz = T(0);
-            auto add = [&](T dx, T dy, T dz, T r, T ir, T v, T u1) {
-              u += v;
-              const T f = r >= T(1e-10) ? u1 * ir : T(0);
+            auto add = [&](bool on, T dx, T dy, T dz, T r, T ir, T v, T u1) {
+              u += on ? v : T(0);
+              const T f = on && r >= T(1e-10) ? u1 * ir : T(0);
               jx += f * dx;
               jy += f * dy;
               jz += f * dz;
             };
 #pragma unroll
             for (int t = 0; t < TILE_EE; ++t) {
-              const int p = g + 16 * t;
-              if (p < N && p != e) {
-                const T dx = x - pw[3 * p], dy = y - pw[3 * p + 1], dz = z - pw[3 * p + 2];
-                T r, ir, u1, lp;
-                M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
-                const bool same = spin_of[p] == spin;
-                uee_d<T, Js>(S, r, ir, same, u1, lp);
-                add(dx, dy, dz, r, ir, uee_val<T, Js>(S, r, same), u1);
-              }
+              const int p = g + 16 * t, pc = p < N ? p : N - 1;
+              const T dx = x - pw[3 * pc], dy = y - pw[3 * pc + 1], dz = z - pw[3 * pc + 2];
+              T r, ir, u1, lp;
+              M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
+              const bool same = spin_of[pc] == spin;
+              uee_d<T, Js>(S, r, ir, same, u1, lp);
+              add(p < N && p != e, dx, dy, dz, r, ir, uee_val<T, Js>(S, r, same), u1);
             }
-            if (en_on && nuc_on) {
+            if (Js::no_series ? en_on : en_on && nuc_on) {  // no series: pnZ = 0 past the nuclei
               const T dx = x - pnx, dy = y - pny, dz = z - pnz;
               T r, ir, v, u1;
               M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
               tile_uen_vd<Js>(S, r, ir, pnZ, g, v, u1);
-              add(dx, dy, dz, r, ir, v, u1);
+              add(nuc_on, dx, dy, dz, r, ir, v, u1);
             }
           };
           // G: quantum force at r (grad ln|D| from the weighted AO
@@ -660,25 +661,31 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
           R = __shfl(hi ? v1 : v0, owner, 64);
           du = T(0);
 #pragma unroll
+          // branch-free: a partner slot that does not count (the electron
+          // itself, past the electrons) adds a selected zero
           for (int t = 0; t < TILE_EE; ++t) {
-            const int p = g + 16 * t;
-            if (p < N && p != e) {
-              const T px = pw[3 * p], py = pw[3 * p + 1], pz = pw[3 * p + 2];
-              const bool same = spin_of[p] == spin;
-              const T ax = nx - px, ay = ny - py, az = nz - pz;
-              const T bx = ox - px, by = oy - py, bz = oz - pz;
-              du += uee_diff<T, Js>(S, M<T>::dist(ax * ax + ay * ay + az * az), M<T>::dist(bx * bx + by * by + bz * bz),
-                                    same);
-            }
+            const int p = g + 16 * t, pc = p < N ? p : N - 1;
+            const T px = pw[3 * pc], py = pw[3 * pc + 1], pz = pw[3 * pc + 2];
+            const bool same = spin_of[pc] == spin;
+            const T ax = nx - px, ay = ny - py, az = nz - pz;
+            const T bx = ox - px, by = oy - py, bz = oz - pz;
+            const T d = uee_diff<T, Js>(S, M<T>::dist(ax * ax + ay * ay + az * az),
+                                        M<T>::dist(bx * bx + by * by + bz * bz), same);
+            du += p < N && p != e ? d : T(0);
           }
-          if (en_on && nuc_on) {
+          if constexpr (Js::no_series) {
+            if (en_on) {  // -Z (r' - r) / ((1+b r')(1+b r)), one reciprocal; pnZ = 0 past the nuclei
+              const T ax = nx - pnx, ay = ny - pny, az = nz - pnz;
+              const T bx = ox - pnx, by = oy - pny, bz = oz - pnz;
+              const T rn = M<T>::dist(ax * ax + ay * ay + az * az), ro = M<T>::dist(bx * bx + by * by + bz * bz);
+              const T d = -pnZ * (rn - ro) * M<T>::rcp((T(1) + S.b_en * rn) * (T(1) + S.b_en * ro));
+              du += nuc_on ? d : T(0);
+            }
+          } else if (en_on && nuc_on) {
             const T ax = nx - pnx, ay = ny - pny, az = nz - pnz;
             const T bx = ox - pnx, by = oy - pny, bz = oz - pnz;
             const T rn = M<T>::dist(ax * ax + ay * ay + az * az), ro = M<T>::dist(bx * bx + by * by + bz * bz);
-            if constexpr (Js::no_series)  // -Z (r' - r) / ((1+b r')(1+b r)), one reciprocal
-              du += -pnZ * (rn - ro) * M<T>::rcp((T(1) + S.b_en * rn) * (T(1) + S.b_en * ro));
-            else
-              du += tile_uen<Js>(S, rn, pnZ, g) - tile_uen<Js>(S, ro, pnZ, g);
+            du += tile_uen<Js>(S, rn, pnZ, g) - tile_uen<Js>(S, ro, pnZ, g);
           }
           du = gsum(du);
           pacc = R * R * M<T>::ex(T(2) * du);
