@@ -755,8 +755,10 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
       T gdx, gdy, gdz, ld;
       det_terms_w(S, x, y, z, wgt, gdx, gdy, gdz, ld, NoShell{});
       T gux = T(0), guy = T(0), guz = T(0), lu = T(0), pairsq = T(0), pee = T(0), pen = T(0);
+      // every lane walks all N electrons: its own (r = 0, rejected by the
+      // r >= 1e-10 guard and the selected potential term) adds zeros, so the
+      // lanes stay in step
       for (int j = 0; j < N; ++j) {
-        if (j == i) continue;
         const T dx = x - pw[3 * j], dy = y - pw[3 * j + 1], dz = z - pw[3 * j + 2];
         T r, ir;
         M<T>::dist_inv(dx * dx + dy * dy + dz * dz, r, ir);
@@ -769,7 +771,7 @@ __device__ __forceinline__ void tile_sweep_block(const DevSys<T>& S, EnsView<T> 
         guz += f * dz;
         lu += ok ? lp : T(0);
         pairsq += ok ? u1 * u1 : T(0);
-        pee += ir;
+        pee += j != i ? ir : T(0);
       }
       for (int I = 0; I < M_; ++I) {
         const T dx = x - S.nuc[I][0], dy = y - S.nuc[I][1], dz = z - S.nuc[I][2];
